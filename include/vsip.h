@@ -1,0 +1,230 @@
+/*
+ * vsip.h -- VSIPL 1.3 C API, multiple-FFT subset, for the MI355X implementation.
+ *
+ * Types, enum values and prototypes keep the VSIPL C API surface of the
+ * reference header c_VSIP_src/vsip.h so that callers recompile unchanged:
+ *   scalar typedefs            vsip.h:124-135
+ *   enums                      vsip.h:138-153  (vsip_memory_hint, vsip_major,
+ *                                               vsip_fft_dir, vsip_fft_place,
+ *                                               vsip_alg_hint)
+ *   opaque handles             vsip.h:165-208
+ *   vsip_fftm_attr_{f,d}        vsip.h:277-278
+ *   attribute structs          vsip.h:242-245 (vsip_mattr_*, vsip_cmattr_*)
+ * Each prototype below names the reference declaration it stands for.
+ *
+ * Storage model: a block's data lives in HBM while the block is admitted.
+ * Binding user memory (vsip_*blockbind_*) records the user's arrays; admitting
+ * with update = true copies them to the device, releasing with update = true
+ * copies the device data back.  Blocks made by vsip_*blockcreate_* (and the
+ * views made by vsip_*mcreate_*) live on the device only and are born
+ * admitted.  FFT execution is asynchronous on the library stream; release,
+ * get/put and copyto/copyfrom wait for it.
+ *
+ * The header is C89.
+ */
+#ifndef _vsip_h
+#define _vsip_h 1
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- scalar types (vsip.h:124-135) ---------------------------------- */
+typedef signed int vsip_scalar_bl;
+typedef vsip_scalar_bl vsip_bool;
+typedef float vsip_scalar_f;
+typedef double vsip_scalar_d;
+typedef unsigned long int vsip_scalar_vi;
+typedef vsip_scalar_vi vsip_offset;
+typedef vsip_scalar_vi vsip_length;
+typedef vsip_scalar_vi vsip_index;
+typedef signed long int vsip_stride;
+typedef struct { vsip_scalar_vi r, c; } vsip_scalar_mi;
+typedef struct { vsip_scalar_d r, i; } vsip_cscalar_d;
+typedef struct { vsip_scalar_f r, i; } vsip_cscalar_f;
+
+#define VSIP_TRUE 1
+#define VSIP_FALSE 0
+
+/* ---- enums (vsip.h:138-153) ----------------------------------------- */
+typedef enum { VSIP_MEM_NONE = 0, VSIP_MEM_RDONLY = 1, VSIP_MEM_CONST = 2,
+               VSIP_MEM_SHARED = 3, VSIP_MEM_SHARED_RDONLY = 4,
+               VSIP_MEM_SHARED_CONST = 5 } vsip_memory_hint;
+typedef enum { VSIP_CMPLX_INTERLEAVED, VSIP_CMPLX_SPLIT, VSIP_CMPLX_NONE } vsip_cmplx_mem;
+typedef enum { VSIP_ROW = 0, VSIP_COL = 1 } vsip_major;
+typedef enum { VSIP_FFT_FWD = -1, VSIP_FFT_INV = +1 } vsip_fft_dir;
+typedef enum { VSIP_FFT_IP = 0, VSIP_FFT_OP = 1 } vsip_fft_place;
+typedef enum { VSIP_ALG_SPACE = 0, VSIP_ALG_TIME = 1, VSIP_ALG_NOISE = 2 } vsip_alg_hint;
+
+/* ---- opaque handles (vsip.h:165-208) -------------------------------- */
+typedef struct vsip_blockattributes_d vsip_block_d;
+typedef struct vsip_cblockattributes_d vsip_cblock_d;
+typedef struct vsip_blockattributes_f vsip_block_f;
+typedef struct vsip_cblockattributes_f vsip_cblock_f;
+typedef struct vsip_mviewattributes_d vsip_mview_d;
+typedef struct vsip_mviewattributes_f vsip_mview_f;
+typedef struct vsip_cmviewattributes_d vsip_cmview_d;
+typedef struct vsip_cmviewattributes_f vsip_cmview_f;
+typedef struct vsip_fftmattributes_f vsip_fftm_f;
+typedef struct vsip_fftmattributes_d vsip_fftm_d;
+
+/* ---- attribute structs (vsip.h:242-245, 277-278) -------------------- */
+typedef struct { vsip_offset offset; vsip_stride row_stride; vsip_length row_length;
+                 vsip_stride col_stride; vsip_length col_length; vsip_block_d* block; } vsip_mattr_d;
+typedef struct { vsip_offset offset; vsip_stride row_stride; vsip_length row_length;
+                 vsip_stride col_stride; vsip_length col_length; vsip_block_f* block; } vsip_mattr_f;
+typedef struct { vsip_offset offset; vsip_stride row_stride; vsip_length row_length;
+                 vsip_stride col_stride; vsip_length col_length; vsip_cblock_d* block; } vsip_cmattr_d;
+typedef struct { vsip_offset offset; vsip_stride row_stride; vsip_length row_length;
+                 vsip_stride col_stride; vsip_length col_length; vsip_cblock_f* block; } vsip_cmattr_f;
+typedef struct { vsip_scalar_mi input; vsip_scalar_mi output; vsip_fft_place place;
+                 vsip_scalar_d scale; vsip_fft_dir dir; vsip_major major; } vsip_fftm_attr_d;
+typedef struct { vsip_scalar_mi input; vsip_scalar_mi output; vsip_fft_place place;
+                 vsip_scalar_f scale; vsip_fft_dir dir; vsip_major major; } vsip_fftm_attr_f;
+
+/* ---- library lifecycle ---------------------------------------------- */
+int vsip_init(void *ptr);                                         /* vsip.h:432 */
+int vsip_finalize(void *ptr);                                     /* vsip.h:427 */
+
+/* ---- real blocks ----------------------------------------------------- */
+vsip_block_f* vsip_blockbind_f(vsip_scalar_f* const, vsip_length, vsip_memory_hint);  /* vsip.h:300 */
+vsip_block_d* vsip_blockbind_d(vsip_scalar_d* const, vsip_length, vsip_memory_hint);  /* vsip.h:292 */
+vsip_block_f* vsip_blockcreate_f(vsip_length, vsip_memory_hint);                     /* vsip.h:301 */
+vsip_block_d* vsip_blockcreate_d(vsip_length, vsip_memory_hint);                     /* vsip.h:293 */
+int vsip_blockadmit_f(vsip_block_f*, vsip_scalar_bl);                                 /* vsip.h:364 */
+int vsip_blockadmit_d(vsip_block_d*, vsip_scalar_bl);                                 /* vsip.h:363 */
+vsip_scalar_f* vsip_blockrelease_f(vsip_block_f*, vsip_scalar_bl);                    /* vsip.h:1835 */
+vsip_scalar_d* vsip_blockrelease_d(vsip_block_d*, vsip_scalar_bl);                    /* vsip.h:1832 */
+void vsip_blockdestroy_f(vsip_block_f*);                                              /* vsip.h:697 */
+void vsip_blockdestroy_d(vsip_block_d*);                                              /* vsip.h:696 */
+vsip_scalar_f* vsip_blockfind_f(const vsip_block_f*);                                 /* vsip.h:1833 */
+vsip_scalar_d* vsip_blockfind_d(const vsip_block_d*);                                 /* vsip.h:1830 */
+vsip_scalar_f* vsip_blockrebind_f(vsip_block_f*, vsip_scalar_f* const);               /* vsip.h:1834 */
+vsip_scalar_d* vsip_blockrebind_d(vsip_block_d*, vsip_scalar_d* const);               /* vsip.h:1831 */
+
+/* ---- complex blocks -------------------------------------------------- */
+vsip_cblock_f* vsip_cblockbind_f(vsip_scalar_f* const, vsip_scalar_f* const,
+                                 vsip_length, vsip_memory_hint);                      /* vsip.h:356 */
+vsip_cblock_d* vsip_cblockbind_d(vsip_scalar_d* const, vsip_scalar_d* const,
+                                 vsip_length, vsip_memory_hint);                      /* vsip.h:348 */
+vsip_cblock_f* vsip_cblockcreate_f(vsip_length, vsip_memory_hint);                   /* vsip.h:357 */
+vsip_cblock_d* vsip_cblockcreate_d(vsip_length, vsip_memory_hint);                   /* vsip.h:349 */
+int vsip_cblockadmit_f(vsip_cblock_f*, vsip_scalar_bl);                               /* vsip.h:372 */
+int vsip_cblockadmit_d(vsip_cblock_d*, vsip_scalar_bl);                               /* vsip.h:371 */
+void vsip_cblockrelease_f(vsip_cblock_f*, vsip_scalar_bl,
+                          vsip_scalar_f**, vsip_scalar_f**);                          /* vsip.h:711 */
+void vsip_cblockrelease_d(vsip_cblock_d*, vsip_scalar_bl,
+                          vsip_scalar_d**, vsip_scalar_d**);                          /* vsip.h:710 */
+void vsip_cblockdestroy_f(vsip_cblock_f*);                                            /* vsip.h:705 */
+void vsip_cblockdestroy_d(vsip_cblock_d*);                                            /* vsip.h:704 */
+void vsip_cblockfind_f(const vsip_cblock_f*, vsip_scalar_f**, vsip_scalar_f**);       /* vsip.h:707 */
+void vsip_cblockfind_d(const vsip_cblock_d*, vsip_scalar_d**, vsip_scalar_d**);       /* vsip.h:706 */
+void vsip_cblockrebind_f(vsip_cblock_f*, vsip_scalar_f* const, vsip_scalar_f* const,
+                         vsip_scalar_f**, vsip_scalar_f**);                           /* vsip.h:709 */
+void vsip_cblockrebind_d(vsip_cblock_d*, vsip_scalar_d* const, vsip_scalar_d* const,
+                         vsip_scalar_d**, vsip_scalar_d**);                           /* vsip.h:708 */
+
+/* ---- complex matrix views -------------------------------------------- */
+vsip_cmview_f* vsip_cmbind_f(const vsip_cblock_f*, vsip_offset, vsip_stride, vsip_length,
+                             vsip_stride, vsip_length);                               /* vsip.h:455 */
+vsip_cmview_d* vsip_cmbind_d(const vsip_cblock_d*, vsip_offset, vsip_stride, vsip_length,
+                             vsip_stride, vsip_length);                               /* vsip.h:454 */
+vsip_cmview_f* vsip_cmcreate_f(vsip_length, vsip_length, vsip_major, vsip_memory_hint); /* vsip.h:495 */
+vsip_cmview_d* vsip_cmcreate_d(vsip_length, vsip_length, vsip_major, vsip_memory_hint); /* vsip.h:494 */
+vsip_cmview_f* vsip_cmsubview_f(const vsip_cmview_f*, vsip_index, vsip_index,
+                                vsip_length, vsip_length);                            /* vsip.h:1959 */
+vsip_cmview_d* vsip_cmsubview_d(const vsip_cmview_d*, vsip_index, vsip_index,
+                                vsip_length, vsip_length);                            /* vsip.h:1958 */
+vsip_cmview_f* vsip_cmtransview_f(const vsip_cmview_f*);                              /* vsip.h:1902 */
+vsip_cmview_d* vsip_cmtransview_d(const vsip_cmview_d*);                              /* vsip.h:1901 */
+vsip_cblock_f* vsip_cmdestroy_f(vsip_cmview_f*);                                      /* vsip.h:358 */
+vsip_cblock_d* vsip_cmdestroy_d(vsip_cmview_d*);                                      /* vsip.h:350 */
+void vsip_cmalldestroy_f(vsip_cmview_f*);                                             /* vsip.h:745 */
+void vsip_cmalldestroy_d(vsip_cmview_d*);                                             /* vsip.h:744 */
+vsip_cscalar_f vsip_cmget_f(const vsip_cmview_f*, vsip_index, vsip_index);            /* vsip.h:552 */
+vsip_cscalar_d vsip_cmget_d(const vsip_cmview_d*, vsip_index, vsip_index);            /* vsip.h:523 */
+void vsip_cmput_f(const vsip_cmview_f*, vsip_index, vsip_index, vsip_cscalar_f);      /* vsip.h:795 */
+void vsip_cmput_d(const vsip_cmview_d*, vsip_index, vsip_index, vsip_cscalar_d);      /* vsip.h:794 */
+vsip_cmattr_f* vsip_cmgetattrib_f(const vsip_cmview_f*, vsip_cmattr_f*);              /* vsip.h:767 */
+vsip_cmattr_d* vsip_cmgetattrib_d(const vsip_cmview_d*, vsip_cmattr_d*);              /* vsip.h:766 */
+vsip_cblock_f* vsip_cmgetblock_f(const vsip_cmview_f*);                               /* vsip.h:359 */
+vsip_cblock_d* vsip_cmgetblock_d(const vsip_cmview_d*);                               /* vsip.h:351 */
+void vsip_cmcopyfrom_user_f(vsip_scalar_f* const, vsip_scalar_f* const, vsip_major,
+                            const vsip_cmview_f*);                                    /* vsip.h:753 */
+void vsip_cmcopyfrom_user_d(vsip_scalar_d* const, vsip_scalar_d* const, vsip_major,
+                            const vsip_cmview_d*);                                    /* vsip.h:752 */
+void vsip_cmcopyto_user_f(const vsip_cmview_f*, vsip_major, vsip_scalar_f* const,
+                          vsip_scalar_f* const);                                      /* vsip.h:755 */
+void vsip_cmcopyto_user_d(const vsip_cmview_d*, vsip_major, vsip_scalar_d* const,
+                          vsip_scalar_d* const);                                      /* vsip.h:754 */
+
+/* ---- real matrix views ----------------------------------------------- */
+vsip_mview_f* vsip_mbind_f(const vsip_block_f*, vsip_offset, vsip_stride, vsip_length,
+                           vsip_stride, vsip_length);                                 /* vsip.h:457 */
+vsip_mview_d* vsip_mbind_d(const vsip_block_d*, vsip_offset, vsip_stride, vsip_length,
+                           vsip_stride, vsip_length);                                 /* vsip.h:456 */
+vsip_mview_f* vsip_mcreate_f(vsip_length, vsip_length, vsip_major, vsip_memory_hint);  /* vsip.h:493 */
+vsip_mview_d* vsip_mcreate_d(vsip_length, vsip_length, vsip_major, vsip_memory_hint);  /* vsip.h:492 */
+vsip_mview_f* vsip_msubview_f(const vsip_mview_f*, vsip_index, vsip_index,
+                              vsip_length, vsip_length);                              /* vsip.h:1868 */
+vsip_mview_d* vsip_msubview_d(const vsip_mview_d*, vsip_index, vsip_index,
+                              vsip_length, vsip_length);                              /* vsip.h:1867 */
+vsip_block_f* vsip_mdestroy_f(vsip_mview_f*);                                         /* vsip.h:302 */
+vsip_block_d* vsip_mdestroy_d(vsip_mview_d*);                                         /* vsip.h:294 */
+void vsip_malldestroy_f(vsip_mview_f*);                                               /* vsip.h:967 */
+void vsip_malldestroy_d(vsip_mview_d*);                                               /* vsip.h:966 */
+vsip_scalar_f vsip_mget_f(const vsip_mview_f*, vsip_index, vsip_index);               /* vsip.h:623 */
+vsip_scalar_d vsip_mget_d(const vsip_mview_d*, vsip_index, vsip_index);               /* vsip.h:590 */
+void vsip_mput_f(const vsip_mview_f*, vsip_index, vsip_index, vsip_scalar_f);         /* vsip.h:1138 */
+void vsip_mput_d(const vsip_mview_d*, vsip_index, vsip_index, vsip_scalar_d);         /* vsip.h:1137 */
+vsip_mattr_f* vsip_mgetattrib_f(const vsip_mview_f*, vsip_mattr_f*);                  /* vsip.h:1072 */
+vsip_mattr_d* vsip_mgetattrib_d(const vsip_mview_d*, vsip_mattr_d*);                  /* vsip.h:1071 */
+void vsip_mcopyfrom_user_f(vsip_scalar_f* const, vsip_major, const vsip_mview_f*);     /* vsip.h:1040 */
+void vsip_mcopyfrom_user_d(vsip_scalar_d* const, vsip_major, const vsip_mview_d*);     /* vsip.h:1039 */
+void vsip_mcopyto_user_f(const vsip_mview_f*, vsip_major, vsip_scalar_f* const);       /* vsip.h:1042 */
+void vsip_mcopyto_user_d(const vsip_mview_d*, vsip_major, vsip_scalar_d* const);       /* vsip.h:1041 */
+
+/* ---- multiple FFT ------------------------------------------------------ */
+vsip_fftm_f* vsip_ccfftmip_create_f(vsip_length M, vsip_length N, vsip_scalar_f scale,
+                                    vsip_fft_dir dir, vsip_major major,
+                                    unsigned int ntimes, vsip_alg_hint hint); /* vsip.h:1813 */
+vsip_fftm_f* vsip_ccfftmop_create_f(vsip_length M, vsip_length N, vsip_scalar_f scale,
+                                    vsip_fft_dir dir, vsip_major major,
+                                    unsigned int ntimes, vsip_alg_hint hint); /* vsip.h:1814 */
+vsip_fftm_f* vsip_rcfftmop_create_f(vsip_length M, vsip_length N, vsip_scalar_f scale,
+                                    vsip_major major, unsigned int ntimes,
+                                    vsip_alg_hint hint);                      /* vsip.h:1816 */
+vsip_fftm_f* vsip_crfftmop_create_f(vsip_length M, vsip_length N, vsip_scalar_f scale,
+                                    vsip_major major, unsigned int ntimes,
+                                    vsip_alg_hint hint);                      /* vsip.h:1815 */
+vsip_fftm_d* vsip_ccfftmip_create_d(vsip_length M, vsip_length N, vsip_scalar_d scale,
+                                    vsip_fft_dir dir, vsip_major major,
+                                    unsigned int ntimes, vsip_alg_hint hint); /* vsip.h:1809 */
+vsip_fftm_d* vsip_ccfftmop_create_d(vsip_length M, vsip_length N, vsip_scalar_d scale,
+                                    vsip_fft_dir dir, vsip_major major,
+                                    unsigned int ntimes, vsip_alg_hint hint); /* vsip.h:1810 */
+vsip_fftm_d* vsip_rcfftmop_create_d(vsip_length M, vsip_length N, vsip_scalar_d scale,
+                                    vsip_major major, unsigned int ntimes,
+                                    vsip_alg_hint hint);                      /* vsip.h:1812 */
+vsip_fftm_d* vsip_crfftmop_create_d(vsip_length M, vsip_length N, vsip_scalar_d scale,
+                                    vsip_major major, unsigned int ntimes,
+                                    vsip_alg_hint hint);                      /* vsip.h:1811 */
+
+void vsip_ccfftmip_f(const vsip_fftm_f*, const vsip_cmview_f*);                       /* vsip.h:715 */
+void vsip_ccfftmip_d(const vsip_fftm_d*, const vsip_cmview_d*);                       /* vsip.h:714 */
+void vsip_ccfftmop_f(const vsip_fftm_f*, const vsip_cmview_f*, const vsip_cmview_f*); /* vsip.h:717 */
+void vsip_ccfftmop_d(const vsip_fftm_d*, const vsip_cmview_d*, const vsip_cmview_d*); /* vsip.h:716 */
+void vsip_rcfftmop_f(const vsip_fftm_f*, const vsip_mview_f*, const vsip_cmview_f*);  /* vsip.h:1184 */
+void vsip_rcfftmop_d(const vsip_fftm_d*, const vsip_mview_d*, const vsip_cmview_d*);  /* vsip.h:1183 */
+void vsip_crfftmop_f(const vsip_fftm_f*, const vsip_cmview_f*, const vsip_mview_f*);  /* vsip.h:827 */
+void vsip_crfftmop_d(const vsip_fftm_d*, const vsip_cmview_d*, const vsip_mview_d*);  /* vsip.h:826 */
+int vsip_fftm_destroy_f(vsip_fftm_f*);                                                /* vsip.h:426 */
+int vsip_fftm_destroy_d(vsip_fftm_d*);                                                /* vsip.h:425 */
+void vsip_fftm_getattr_f(const vsip_fftm_f*, vsip_fftm_attr_f*);                      /* vsip.h:947 */
+void vsip_fftm_getattr_d(const vsip_fftm_d*, vsip_fftm_attr_d*);                      /* vsip.h:946 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* _vsip_h */

@@ -1,0 +1,55 @@
+/*
+ * vsip_amd.h -- extensions of the MI355X VSIPL FFTM library.
+ *
+ * Nothing here has a counterpart in the reference's vsip.h.  These entry
+ * points serve callers that need to see the device side: the library stream
+ * (to time kernels with HIP events or to order their own work after an FFT),
+ * device-resident block data, error text, and the HBM budget that bounds the
+ * library's allocations.
+ *
+ * C89.
+ */
+#ifndef _vsip_amd_h
+#define _vsip_amd_h 1
+
+#include <stddef.h>
+#include "vsip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t of the library stream all plain vsip_* calls run on */
+void *vsip_amd_stream(void);
+/* wait for all queued work; returns 0 or a HIP error code */
+int vsip_amd_sync(void);
+/* text of the most recent error reported by the library ("" if none) */
+const char *vsip_amd_last_error(void);
+void vsip_amd_clear_error(void);
+/* device bytes currently held by the library */
+size_t vsip_amd_bytes_in_use(void);
+/* cap the library's device allocations at `bytes` (0 = no cap); allocations
+ * beyond the cap fail as if HBM were exhausted (create returns NULL, admit 1) */
+void vsip_amd_set_hbm_budget(size_t bytes);
+
+/* device address of a block's data (NULL unless the block holds device data);
+ * for split complex blocks `im` receives the imaginary array */
+void *vsip_amd_cblock_device_f(const vsip_cblock_f *b, void **im);
+void *vsip_amd_cblock_device_d(const vsip_cblock_d *b, void **im);
+void *vsip_amd_block_device_f(const vsip_block_f *b);
+void *vsip_amd_block_device_d(const vsip_block_d *b);
+
+/* Single-pass schedule the planner picks for `nvec` transforms of length n
+ * (prec 4 or 8) on a device with `lds_max` bytes of LDS per workgroup, as
+ * JSON text; no device work.  Returns 0, -1 for an unsupported length, -2 if
+ * one vector does not fit a workgroup (such lengths run as two passes). */
+int vsip_amd_schedule(int prec, long n, long nvec, int lds_max, char *buf, size_t len);
+
+/* Schedule of a plan as text: passes, radices, launch shapes. */
+int vsip_amd_fftm_describe_f(const vsip_fftm_f *plan, char *buf, size_t len);
+int vsip_amd_fftm_describe_d(const vsip_fftm_d *plan, char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* _vsip_amd_h */

@@ -1,0 +1,13 @@
+// fftm_inst_d8.hip -- double, 8-element register budget instantiations of the multiple-FFT kernel.
+#include "fftm_core.hpp"
+
+namespace vhipk {
+kernel_fn fftm_kernel_d8(int kind) {
+    switch (kind) {
+    case VHIP_C2C: return fftm_kernel<double, 8, VHIP_C2C>;
+    case VHIP_R2C: return fftm_kernel<double, 8, VHIP_R2C>;
+    case VHIP_C2R: return fftm_kernel<double, 8, VHIP_C2R>;
+    }
+    return nullptr;
+}
+}  // namespace vhipk

@@ -1,0 +1,118 @@
+/*
+ * vhip.h -- the thin C ABI between the C89 VSIPL host layer and the HIP code.
+ *
+ * Only plain C types cross this boundary: pointers, integers, doubles and the
+ * POD descriptors below.  No HIP type appears here, so the host layer compiles
+ * with any C89 compiler and links against the hipcc-built object.
+ *
+ * Every function returns 0 on success and a nonzero code on failure; the code
+ * is the hipError_t value when a HIP call failed (negative values are this
+ * layer's own argument errors).
+ */
+#ifndef VHIP_H
+#define VHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VHIP_MAX_STAGES 24
+
+/* transform kinds */
+#define VHIP_C2C 0
+#define VHIP_R2C 1
+#define VHIP_C2R 2
+
+/* how a workgroup's butterflies map onto the vectors it owns at the global
+ * load/store: consecutive lanes walk along the vector (unit element stride,
+ * ROW-like) or across vectors (unit vector stride, COL-like). */
+#define VHIP_MAP_ALONG 0
+#define VHIP_MAP_ACROSS 1
+
+/*
+ * A strided set of vectors in device memory.  Element n of vector v sits at
+ *     base + (v % nv0) * vs0 + (v / nv0) * vs1 + n * es
+ * counted in elements of the view (complex elements for complex views, real
+ * elements for real views).  For interleaved complex data `re` points at the
+ * complex element and `im` is NULL; for split complex data `re`/`im` point at
+ * the real and imaginary arrays; for real data `re` is the real array.
+ */
+typedef struct {
+    void *re;
+    void *im;
+    long es;
+    long vs0;
+    long vs1;
+    long nv0;
+    int split;
+    int contig;   /* set by the host: interleaved/real with es == 1 and aligned */
+} vhip_view;
+
+/* One launch of the multiple-FFT kernel: nvec independent transforms. */
+typedef struct {
+    int prec;                      /* 4 = float, 8 = double */
+    int kind;                      /* VHIP_C2C / VHIP_R2C / VHIP_C2R */
+    int n;                         /* complex FFT length */
+    int nst;                       /* number of Stockham stages */
+    int radix[VHIP_MAX_STAGES];
+    int tw_off[VHIP_MAX_STAGES];   /* stage twiddle offset (complex elements) */
+    long nvec;                     /* vectors in this launch */
+    int vpb;                       /* vectors per workgroup */
+    int threads;                   /* workgroup size */
+    int lds_stride;                /* padded vector stride in LDS, complex elements */
+    int maxe;                      /* register budget class: 16 or 32 elements */
+    int map_in, map_out;           /* VHIP_MAP_ALONG / VHIP_MAP_ACROSS */
+    int conj_in;                   /* C2C: conjugate on load (inverse transforms) */
+    int conj_out;                  /* C2C/C2R: conjugate on store */
+    double scale;                  /* applied in the final store */
+    const void *tw;                /* stage twiddles, forward sign */
+    const void *euler;             /* R2C: e^{-i pi k/n}; C2R: e^{+i pi k/n}; k = 0..n */
+    const void *ptw;               /* post-twiddle table e^{-2 pi i m / ptw_n}, or NULL */
+    long ptw_n;
+    vhip_view in, out;
+} vhip_fftm_desc;
+
+/* runtime */
+int vhip_init(void);
+int vhip_device_count(int *count);
+int vhip_lds_max(int *bytes);           /* max dynamic LDS per workgroup */
+int vhip_cu_count(int *cus);
+int vhip_malloc(void **p, size_t bytes);
+int vhip_free(void *p);
+int vhip_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int vhip_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int vhip_memcpy_d2d(void *dst, const void *src, size_t bytes);
+/* 2-D strided copies: `rows` rows of `width` bytes, row pitches in bytes */
+int vhip_memcpy2d_h2d(void *dst, size_t dpitch, const void *src, size_t spitch,
+                      size_t width, size_t rows);
+int vhip_memcpy2d_d2h(void *dst, size_t dpitch, const void *src, size_t spitch,
+                      size_t width, size_t rows);
+int vhip_sync(void);
+void *vhip_stream(void);                /* the library stream (hipStream_t) */
+const char *vhip_error_string(int code);
+size_t vhip_bytes_in_use(void);         /* device bytes held by this library */
+void vhip_set_budget(size_t bytes);     /* 0 = no cap; allocations beyond it fail */
+
+/* A strided m x n matrix in device memory: element (i, j) at
+ * base + i * cs + j * rs, in elements (complex elements when complex). */
+typedef struct {
+    void *re;
+    void *im;        /* split complex imaginary array, else NULL */
+    long rs;         /* stride between consecutive j (VSIPL row_stride) */
+    long cs;         /* stride between consecutive i (VSIPL col_stride) */
+    int split;
+    int pad_;
+} vhip_mat;
+
+/* kernels */
+int vhip_mat_copy(int prec, int cplx, const vhip_mat *src, const vhip_mat *dst,
+                  long m, long n, void *stream);
+int vhip_fftm_launch(const vhip_fftm_desc *d, void *stream);
+size_t vhip_fftm_lds_bytes(const vhip_fftm_desc *d);   /* dynamic LDS of one launch */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VHIP_H */

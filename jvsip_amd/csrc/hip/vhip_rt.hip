@@ -1,0 +1,204 @@
+// vhip_rt.hip -- device memory, copies and the library stream behind vhip.h.
+//
+// All work of the plain VSIPL entry points is ordered on one library stream
+// per process; host-visible completion happens at the synchronous copies
+// (admit/release, get/put, copyto/copyfrom) and at vhip_sync().
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <atomic>
+#include <mutex>
+#include <unordered_map>
+#include "vhip.h"
+
+extern "C" int vhip_fftm_set_lds_limit(int bytes);
+
+namespace {
+hipStream_t g_stream = nullptr;
+int g_lds_max = 65536;
+int g_cus = 0;
+bool g_ready = false;
+std::mutex g_init_mu;
+std::atomic<size_t> g_bytes{0};
+std::atomic<size_t> g_budget{0};
+
+// sizes of live allocations, so frees are charged back against the budget
+std::mutex g_alloc_mu;
+std::unordered_map<void *, size_t> g_sizes;
+}  // namespace
+
+extern "C" int vhip_init(void) {
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (g_ready) return 0;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return (int)e;
+    if (n < 1) return (int)hipErrorNoDevice;
+    hipDeviceProp_t p;
+    e = hipGetDeviceProperties(&p, 0);
+    if (e != hipSuccess) return (int)e;
+    g_cus = p.multiProcessorCount;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) == hipSuccess && lds > 0)
+        g_lds_max = lds;
+    e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return (int)e;
+    int rc = vhip_fftm_set_lds_limit(g_lds_max);
+    if (rc != 0) return rc;
+    const char *b = getenv("VSIP_AMD_HBM_BUDGET");
+    if (b) g_budget.store((size_t)strtoull(b, nullptr, 10));
+    g_ready = true;
+    return 0;
+}
+
+extern "C" int vhip_device_count(int *count) {
+    hipError_t e = hipGetDeviceCount(count);
+    return (int)e;
+}
+
+extern "C" int vhip_lds_max(int *bytes) {
+    *bytes = g_lds_max;
+    return 0;
+}
+
+extern "C" int vhip_cu_count(int *cus) {
+    *cus = g_cus;
+    return 0;
+}
+
+extern "C" int vhip_malloc(void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 1;
+    size_t cap = g_budget.load();
+    size_t cur = g_bytes.fetch_add(bytes);
+    if (cap && cur + bytes > cap) {
+        g_bytes.fetch_sub(bytes);
+        return (int)hipErrorOutOfMemory;
+    }
+    void *raw = nullptr;
+    hipError_t e = hipMalloc(&raw, bytes);
+    if (e != hipSuccess) {
+        g_bytes.fetch_sub(bytes);
+        (void)hipGetLastError();
+        return (int)e;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_alloc_mu);
+        g_sizes[raw] = bytes;
+    }
+    *p = raw;
+    return 0;
+}
+
+extern "C" int vhip_free(void *p) {
+    if (!p) return 0;
+    {
+        std::lock_guard<std::mutex> lk(g_alloc_mu);
+        auto it = g_sizes.find(p);
+        if (it != g_sizes.end()) {
+            g_bytes.fetch_sub(it->second);
+            g_sizes.erase(it);
+        }
+    }
+    return (int)hipFree(p);
+}
+
+extern "C" size_t vhip_bytes_in_use(void) { return g_bytes.load(); }
+extern "C" void vhip_set_budget(size_t bytes) { g_budget.store(bytes); }
+
+extern "C" int vhip_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return 0;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    return (int)e;
+}
+
+extern "C" int vhip_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return 0;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    return (int)e;
+}
+
+extern "C" int vhip_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return 0;
+    return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream);
+}
+
+extern "C" int vhip_memcpy2d_h2d(void *dst, size_t dpitch, const void *src, size_t spitch,
+                                 size_t width, size_t rows) {
+    if (!width || !rows) return 0;
+    hipError_t e = hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyHostToDevice, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    return (int)e;
+}
+
+extern "C" int vhip_memcpy2d_d2h(void *dst, size_t dpitch, const void *src, size_t spitch,
+                                 size_t width, size_t rows) {
+    if (!width || !rows) return 0;
+    hipError_t e = hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyDeviceToHost, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    return (int)e;
+}
+
+extern "C" int vhip_sync(void) {
+    if (!g_stream) return 0;
+    return (int)hipStreamSynchronize(g_stream);
+}
+
+extern "C" void *vhip_stream(void) { return (void *)g_stream; }
+
+extern "C" const char *vhip_error_string(int code) {
+    if (code < 0) return "vhip: invalid argument";
+    return hipGetErrorString((hipError_t)code);
+}
+
+// ---------------------------------------------------------------------------
+// strided matrix copy (view <-> dense staging buffer), any element type
+// ---------------------------------------------------------------------------
+namespace {
+template <typename T, bool CPLX>
+__global__ void mat_copy_kernel(const vhip_mat src, const vhip_mat dst, long m, long n) {
+    const long total = m * n;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const long i = t / n, j = t - (t / n) * n;
+        const long so = i * src.cs + j * src.rs;
+        const long dof = i * dst.cs + j * dst.rs;
+        T re, im = (T)0;
+        if (CPLX && src.split) {
+            re = ((const T *)src.re)[so];
+            im = ((const T *)src.im)[so];
+        } else if (CPLX) {
+            re = ((const T *)src.re)[2 * so];
+            im = ((const T *)src.re)[2 * so + 1];
+        } else {
+            re = ((const T *)src.re)[so];
+        }
+        if (CPLX && dst.split) {
+            ((T *)dst.re)[dof] = re;
+            ((T *)dst.im)[dof] = im;
+        } else if (CPLX) {
+            ((T *)dst.re)[2 * dof] = re;
+            ((T *)dst.re)[2 * dof + 1] = im;
+        } else {
+            ((T *)dst.re)[dof] = re;
+        }
+    }
+}
+}  // namespace
+
+extern "C" int vhip_mat_copy(int prec, int cplx, const vhip_mat *src, const vhip_mat *dst,
+                             long m, long n, void *stream) {
+    if (m <= 0 || n <= 0) return 0;
+    const long total = m * n;
+    long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipStream_t s = stream ? (hipStream_t)stream : g_stream;
+    if (prec == 4 && cplx) hipLaunchKernelGGL((mat_copy_kernel<float, true>), dim3(blocks), dim3(256), 0, s, *src, *dst, m, n);
+    else if (prec == 4) hipLaunchKernelGGL((mat_copy_kernel<float, false>), dim3(blocks), dim3(256), 0, s, *src, *dst, m, n);
+    else if (prec == 8 && cplx) hipLaunchKernelGGL((mat_copy_kernel<double, true>), dim3(blocks), dim3(256), 0, s, *src, *dst, m, n);
+    else if (prec == 8) hipLaunchKernelGGL((mat_copy_kernel<double, false>), dim3(blocks), dim3(256), 0, s, *src, *dst, m, n);
+    else return -1;
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,138 @@
+/*
+ * va_lib.c -- library lifecycle, error reporting and the vsip_amd_*
+ * extensions.
+ *
+ * vsip_init / vsip_finalize are no-ops in the reference (vsip_init.c:17-25,
+ * vsip_finalize.c:16-26); here vsip_init brings up the device and every
+ * constructor calls it lazily, so programs that never call it still work.
+ *
+ * C89.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "va_internal.h"
+#include "vsip_amd.h"
+
+static int va_ready = 0;
+static int va_init_rc = 0;
+static char va_last[256] = "";
+
+int va_ensure_init(void)
+{
+    if (va_ready) return va_init_rc;
+    va_init_rc = vhip_init();
+    va_ready = 1;
+    if (va_init_rc) va_error("vsip_init", vhip_error_string(va_init_rc), va_init_rc);
+    return va_init_rc;
+}
+
+void va_error(const char *where, const char *what, int code)
+{
+    sprintf(va_last, "%.60s: %.150s (code %d)", where, what ? what : "", code);
+    if (getenv("VSIP_AMD_QUIET") == NULL) fprintf(stderr, "vsip_amd: %s\n", va_last);
+}
+
+int vsip_init(void *ptr)
+{
+    (void)ptr;
+    return va_ensure_init() ? 1 : 0;
+}
+
+int vsip_finalize(void *ptr)
+{
+    (void)ptr;
+    if (va_ready && !va_init_rc) vhip_sync();
+    return 0;
+}
+
+void *vsip_amd_stream(void)
+{
+    if (va_ensure_init()) return NULL;
+    return vhip_stream();
+}
+
+int vsip_amd_sync(void)
+{
+    if (va_ensure_init()) return va_init_rc;
+    return vhip_sync();
+}
+
+const char *vsip_amd_last_error(void) { return va_last; }
+void vsip_amd_clear_error(void) { va_last[0] = '\0'; }
+size_t vsip_amd_bytes_in_use(void) { return vhip_bytes_in_use(); }
+void vsip_amd_set_hbm_budget(size_t bytes) { vhip_set_budget(bytes); }
+
+void *vsip_amd_cblock_device_f(const vsip_cblock_f *b, void **im)
+{
+    if (im) *im = b ? b->b.dev_i : NULL;
+    return b ? b->b.dev_r : NULL;
+}
+
+void *vsip_amd_cblock_device_d(const vsip_cblock_d *b, void **im)
+{
+    if (im) *im = b ? b->b.dev_i : NULL;
+    return b ? b->b.dev_r : NULL;
+}
+
+void *vsip_amd_block_device_f(const vsip_block_f *b) { return b ? b->b.dev_r : NULL; }
+void *vsip_amd_block_device_d(const vsip_block_d *b) { return b ? b->b.dev_r : NULL; }
+
+static int va_describe(const VA_fftm *p, char *buf, size_t len)
+{
+    char tmp[1024];
+    int pos = 0, k, s;
+    static const char *types[4] = {"ccfftmip", "ccfftmop", "rcfftmop", "crfftmop"};
+    pos += sprintf(tmp + pos, "{\"type\": \"%s\", \"prec\": %d, \"MN\": %lu, \"mN\": %lu, "
+                   "\"npass\": %d, \"passes\": [", types[p->type & 3], p->prec,
+                   (unsigned long)p->MN, (unsigned long)p->mN, p->npass);
+    for (k = 0; k < p->npass; k++) {
+        const VA_sched *sc = &p->pass[k];
+        pos += sprintf(tmp + pos, "%s{\"n\": %d, \"radix\": [", k ? ", " : "", sc->n);
+        for (s = 0; s < sc->nst && pos < 900; s++)
+            pos += sprintf(tmp + pos, "%s%d", s ? ", " : "", sc->radix[s]);
+        pos += sprintf(tmp + pos, "], \"along\": [%d, %d, %d], \"across\": [%d, %d, %d], "
+                       "\"lds_stride\": %d}", sc->along.vpb, sc->along.threads, sc->along.maxe,
+                       sc->across.vpb, sc->across.threads, sc->across.maxe, sc->lds_stride);
+    }
+    pos += sprintf(tmp + pos, "]}");
+    if (buf && len) {
+        strncpy(buf, tmp, len - 1);
+        buf[len - 1] = '\0';
+    }
+    return pos;
+}
+
+int vsip_amd_schedule(int prec, long n, long nvec, int lds_max, char *buf, size_t len)
+{
+    VA_sched sc;
+    char tmp[1024];
+    int pos = 0, s, rc;
+    rc = va_sched_shape(&sc, prec, (int)n, (vsip_length)nvec, lds_max);
+    pos += sprintf(tmp + pos, "{\"rc\": %d", rc);
+    if (rc == 0) {
+        pos += sprintf(tmp + pos, ", \"n\": %d, \"radix\": [", sc.n);
+        for (s = 0; s < sc.nst && pos < 900; s++)
+            pos += sprintf(tmp + pos, "%s%d", s ? ", " : "", sc.radix[s]);
+        pos += sprintf(tmp + pos, "], \"along\": [%d, %d, %d], \"across\": [%d, %d, %d], "
+                       "\"lds_stride\": %d, \"tw_len\": %d", sc.along.vpb, sc.along.threads,
+                       sc.along.maxe, sc.across.vpb, sc.across.threads, sc.across.maxe,
+                       sc.lds_stride, sc.tw_len);
+    }
+    pos += sprintf(tmp + pos, "}");
+    if (buf && len) {
+        strncpy(buf, tmp, len - 1);
+        buf[len - 1] = '\0';
+    }
+    return rc;
+}
+
+int vsip_amd_fftm_describe_f(const vsip_fftm_f *plan, char *buf, size_t len)
+{
+    return va_describe(&plan->p, buf, len);
+}
+
+int vsip_amd_fftm_describe_d(const vsip_fftm_d *plan, char *buf, size_t len)
+{
+    return va_describe(&plan->p, buf, len);
+}

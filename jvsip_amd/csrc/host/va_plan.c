@@ -1,0 +1,217 @@
+/*
+ * va_plan.c -- transform schedules: factorisation, launch shapes, twiddles.
+ *
+ * The reference factors the length in the fixed order 3,5,7,8,4,2 into at
+ * most six radix groups plus a remainder handled by an O(L^2) DFT
+ * (VI_nuV.h:22-136) and builds an FP32 twiddle table from a float angle ramp
+ * (VI_ccfftmip_create_f_def.h:63-73, vsip_vramp_f.c:31-34).  Only the result
+ * of the transform has to agree, so the schedule here is chosen for the
+ * hardware instead:
+ *   - radices 16, 8, 4, 2 for the power-of-two part, split as evenly as
+ *     possible into ceil(log2(p)/4) stages, then one stage per factor
+ *     3, 5, 7, 11, 13;
+ *   - per stage s with Ns = R_0 ... R_{s-1}, twiddles W_{Ns R}^{r k},
+ *     r = 1..R-1, k = 0..Ns-1, computed in double precision from the exact
+ *     integer angle and stored in the working precision;
+ *   - launch shapes (vectors per workgroup, threads, register class) for the
+ *     two lane mappings, bounded by the per-workgroup LDS of the device.
+ * Lengths with a prime factor the kernels have no butterfly for are
+ * rejected (create returns NULL); lengths beyond one workgroup's LDS use the
+ * four-step split planned in va_fftm.c.
+ *
+ * C89.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "va_internal.h"
+
+#define VA_PI 3.14159265358979323846264338327950288
+
+static int va_pad(int prec, int i)
+{
+    return i + (i >> (prec == 4 ? 5 : 4));
+}
+
+/* factor n into stage radices; returns the stage count or -1 */
+static int va_factor(int n, int prec, int *radix)
+{
+    int m = n, p2 = 0, nst = 0, s, stages2, base, extra, i;
+    static const int odd[5] = {3, 5, 7, 11, 13};
+    int maxbits = (prec == 4) ? 4 : 3;   /* radix 16 only in the float kernels */
+    while (m % 2 == 0) { m /= 2; p2++; }
+    if (p2 > 0) {
+        stages2 = (p2 + maxbits - 1) / maxbits;
+        base = p2 / stages2;
+        extra = p2 % stages2;
+        for (s = 0; s < stages2; s++) {
+            int bits = base + (s < extra ? 1 : 0);
+            if (nst >= VHIP_MAX_STAGES) return -1;
+            radix[nst++] = 1 << bits;
+        }
+    }
+    for (i = 0; i < 5; i++) {
+        while (m % odd[i] == 0) {
+            if (nst >= VHIP_MAX_STAGES) return -1;
+            radix[nst++] = odd[i];
+            m /= odd[i];
+        }
+    }
+    if (m != 1) return -1;
+    return nst;
+}
+
+static int va_maxb(int r, int maxe)
+{
+    return (maxe / r) > 0 ? (maxe / r) : 1;
+}
+
+/* does (vpb, threads, maxe) keep every stage within the register budget? */
+static int va_shape_ok(const VA_sched *s, int vpb, int thr, int maxe)
+{
+    int i;
+    for (i = 0; i < s->nst; i++) {
+        long bt = (long)vpb * (s->n / s->radix[i]);
+        long nb = (bt + thr - 1) / thr;
+        if (nb > va_maxb(s->radix[i], maxe)) return 0;
+    }
+    return 1;
+}
+
+static long va_lds_bytes(const VA_sched *s, int prec, int vpb)
+{
+    return (long)vpb * s->lds_stride * 2 * prec + (long)vpb * 20;
+}
+
+/* smallest thread count (multiple of 64, <= 1024) that fits; 0 if none */
+static int va_fit_threads(const VA_sched *s, int vpb, int maxe)
+{
+    long want = ((long)vpb * s->n + maxe - 1) / maxe;
+    int thr = (int)(((want + 63) / 64) * 64);
+    if (thr < 64) thr = 64;
+    for (; thr <= 1024; thr += 64)
+        if (va_shape_ok(s, vpb, thr, maxe)) return thr;
+    return 0;
+}
+
+static int va_pick_launch(const VA_sched *s, int prec, int vpb_want, int lds_soft, int lds_max,
+                          VA_launch *out)
+{
+    int vpb, thr, m;
+    static const int maxes_f[2] = {16, 32};
+    static const int maxes_d[1] = {8};
+    const int *maxes = (prec == 4) ? maxes_f : maxes_d;
+    int nmax = (prec == 4) ? 2 : 1;
+    for (vpb = vpb_want; vpb >= 1; vpb--) {
+        long lds = va_lds_bytes(s, prec, vpb);
+        if (lds > lds_max) continue;
+        if (vpb > 1 && lds > lds_soft) continue;
+        for (m = 0; m < nmax; m++) {
+            thr = va_fit_threads(s, vpb, maxes[m]);
+            if (thr) {
+                out->vpb = vpb;
+                out->threads = thr;
+                out->maxe = maxes[m];
+                return 0;
+            }
+        }
+    }
+    return -1;
+}
+
+/*
+ * Shape of a single-pass schedule of length n for nvec vectors: radices,
+ * twiddle offsets, LDS stride and launch shapes, with no device work.
+ * Returns 0, or -1 when n has an unsupported prime factor, or -2 when one
+ * vector does not fit a workgroup (the caller then splits n).
+ */
+int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max)
+{
+    int i, off, Ns, vpb_along, vpb_across;
+    long want;
+    memset(s, 0, sizeof(*s));
+    s->n = n;
+    s->nst = va_factor(n, prec, s->radix);
+    if (s->nst < 0) return -1;
+    if (n < 2) return -1;
+    s->lds_stride = va_pad(prec, n);
+    if (s->lds_stride % 2 == 0) s->lds_stride += 1;
+
+    /* along: enough elements per workgroup to keep ~256 lanes busy */
+    want = (prec == 4) ? 4096 : 2048;
+    vpb_along = (int)(want / n);
+    if (vpb_along < 1) vpb_along = 1;
+    if ((vsip_length)vpb_along > nvec && nvec > 0) vpb_along = (int)nvec;
+    if (va_pick_launch(s, prec, vpb_along, 65536, lds_max, &s->along)) return -2;
+
+    /* across: 16 vectors give 128-byte (float) row segments per load */
+    vpb_across = 16;
+    if ((vsip_length)vpb_across > nvec && nvec > 0) vpb_across = (int)nvec;
+    if (va_pick_launch(s, prec, vpb_across, lds_max, lds_max, &s->across)) return -2;
+
+    /* stage twiddle offsets */
+    off = 0;
+    Ns = 1;
+    for (i = 0; i < s->nst; i++) {
+        s->tw_off[i] = off;
+        if (Ns > 1) off += (s->radix[i] - 1) * Ns;
+        Ns *= s->radix[i];
+    }
+    s->tw_len = off;
+    return 0;
+}
+
+/* shape plus the stage twiddles uploaded to the device */
+int va_sched_plan(VA_sched *s, int prec, int n, vsip_length nvec)
+{
+    int lds_max = 65536, i, r, k, Ns, off, rc;
+    vhip_lds_max(&lds_max);
+    rc = va_sched_shape(s, prec, n, nvec, lds_max);
+    if (rc) return rc;
+    off = s->tw_len;
+    if (off > 0) {
+        size_t esz = (size_t)(2 * prec);
+        char *host = (char *)malloc(esz * (size_t)off);
+        if (!host) return -3;
+        Ns = 1;
+        for (i = 0; i < s->nst; i++) {
+            int R = s->radix[i];
+            if (Ns > 1) {
+                long L = (long)Ns * R;
+                for (r = 1; r < R; r++) {
+                    for (k = 0; k < Ns; k++) {
+                        long m = ((long)r * k) % L;
+                        double a = -2.0 * VA_PI * (double)m / (double)L;
+                        double c = cos(a), sn = sin(a);
+                        long idx = s->tw_off[i] + (long)(r - 1) * Ns + k;
+                        if (prec == 4) {
+                            float *f = (float *)host;
+                            f[2 * idx] = (float)c;
+                            f[2 * idx + 1] = (float)sn;
+                        } else {
+                            double *d = (double *)host;
+                            d[2 * idx] = c;
+                            d[2 * idx + 1] = sn;
+                        }
+                    }
+                }
+            }
+            Ns *= R;
+        }
+        rc = vhip_malloc(&s->dev_tw, esz * (size_t)off);
+        if (!rc) rc = vhip_memcpy_h2d(s->dev_tw, host, esz * (size_t)off);
+        free(host);
+        if (rc) {
+            if (s->dev_tw) vhip_free(s->dev_tw);
+            s->dev_tw = NULL;
+            return -3;
+        }
+    }
+    return 0;
+}
+
+void va_sched_free(VA_sched *s)
+{
+    if (s->dev_tw) vhip_free(s->dev_tw);
+    s->dev_tw = NULL;
+}

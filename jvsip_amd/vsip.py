@@ -1,0 +1,187 @@
+"""ctypes binding of the VSIPL multiple-FFT C API.
+
+`VsipLib(path)` binds the vsip_* entry points declared in include/vsip.h from
+any shared library that exports them: the MI355X library built in
+jvsip_amd/lib/libvsip_amd.so (see `load()`), or a build of rrjudd/jvsip's own
+C sources (used only by the tests and the CPU baseline).  Handles are opaque
+pointers, exactly as in C; arguments and results keep the C meaning.
+
+`load()` returns the MI355X library and raises if it has not been built: there
+is no fallback path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libvsip_amd.so")
+
+c_vi = ctypes.c_ulong      # vsip_length / vsip_offset / vsip_index
+c_st = ctypes.c_long       # vsip_stride
+c_bl = ctypes.c_int        # vsip_scalar_bl, enums
+P = ctypes.c_void_p
+
+VSIP_ROW, VSIP_COL = 0, 1
+VSIP_FFT_FWD, VSIP_FFT_INV = -1, 1
+VSIP_FFT_IP, VSIP_FFT_OP = 0, 1
+VSIP_MEM_NONE = 0
+VSIP_ALG_TIME = 1
+
+
+class cscalar_f(ctypes.Structure):
+    _fields_ = [("r", ctypes.c_float), ("i", ctypes.c_float)]
+
+
+class cscalar_d(ctypes.Structure):
+    _fields_ = [("r", ctypes.c_double), ("i", ctypes.c_double)]
+
+
+class scalar_mi(ctypes.Structure):
+    _fields_ = [("r", c_vi), ("c", c_vi)]
+
+
+class fftm_attr_f(ctypes.Structure):
+    _fields_ = [("input", scalar_mi), ("output", scalar_mi), ("place", c_bl),
+                ("scale", ctypes.c_float), ("dir", c_bl), ("major", c_bl)]
+
+
+class fftm_attr_d(ctypes.Structure):
+    _fields_ = [("input", scalar_mi), ("output", scalar_mi), ("place", c_bl),
+                ("scale", ctypes.c_double), ("dir", c_bl), ("major", c_bl)]
+
+
+class cmattr(ctypes.Structure):
+    _fields_ = [("offset", c_vi), ("row_stride", c_st), ("row_length", c_vi),
+                ("col_stride", c_st), ("col_length", c_vi), ("block", P)]
+
+
+def _sig(t):
+    """(name suffix -> (restype, argtypes)) for element type t in {f, d}."""
+    sc = ctypes.c_float if t == "f" else ctypes.c_double
+    ps = ctypes.POINTER(sc)
+    cs = cscalar_f if t == "f" else cscalar_d
+    attr = fftm_attr_f if t == "f" else fftm_attr_d
+    return {
+        "blockbind": (P, [P, c_vi, c_bl]),
+        "blockcreate": (P, [c_vi, c_bl]),
+        "blockadmit": (ctypes.c_int, [P, c_bl]),
+        "blockrelease": (P, [P, c_bl]),
+        "blockdestroy": (None, [P]),
+        "cblockbind": (P, [P, P, c_vi, c_bl]),
+        "cblockcreate": (P, [c_vi, c_bl]),
+        "cblockadmit": (ctypes.c_int, [P, c_bl]),
+        "cblockrelease": (None, [P, c_bl, ctypes.POINTER(P), ctypes.POINTER(P)]),
+        "cblockdestroy": (None, [P]),
+        "cmbind": (P, [P, c_vi, c_st, c_vi, c_st, c_vi]),
+        "cmcreate": (P, [c_vi, c_vi, c_bl, c_bl]),
+        "cmsubview": (P, [P, c_vi, c_vi, c_vi, c_vi]),
+        "cmdestroy": (P, [P]),
+        "cmalldestroy": (None, [P]),
+        "cmget": (cs, [P, c_vi, c_vi]),
+        "cmput": (None, [P, c_vi, c_vi, cs]),
+        "cmcopyfrom_user": (None, [P, P, c_bl, P]),
+        "cmcopyto_user": (None, [P, c_bl, P, P]),
+        "cmgetattrib": (P, [P, ctypes.POINTER(cmattr)]),
+        "cmtransview": (P, [P]),
+        "mbind": (P, [P, c_vi, c_st, c_vi, c_st, c_vi]),
+        "mcreate": (P, [c_vi, c_vi, c_bl, c_bl]),
+        "mdestroy": (P, [P]),
+        "malldestroy": (None, [P]),
+        "mget": (sc, [P, c_vi, c_vi]),
+        "mput": (None, [P, c_vi, c_vi, sc]),
+        "ccfftmip_create": (P, [c_vi, c_vi, sc, c_bl, c_bl, ctypes.c_uint, c_bl]),
+        "ccfftmop_create": (P, [c_vi, c_vi, sc, c_bl, c_bl, ctypes.c_uint, c_bl]),
+        "rcfftmop_create": (P, [c_vi, c_vi, sc, c_bl, ctypes.c_uint, c_bl]),
+        "crfftmop_create": (P, [c_vi, c_vi, sc, c_bl, ctypes.c_uint, c_bl]),
+        "ccfftmip": (None, [P, P]),
+        "ccfftmop": (None, [P, P, P]),
+        "rcfftmop": (None, [P, P, P]),
+        "crfftmop": (None, [P, P, P]),
+        "fftm_destroy": (ctypes.c_int, [P]),
+        "fftm_getattr": (None, [P, ctypes.POINTER(attr)]),
+        "cmsub": (None, [P, P, P]),
+        "cmmeansqval": (sc, [P]),
+    }
+
+
+class VsipLib:
+    """The vsip_* API of one shared library; attribute `x_f` is vsip_x_f."""
+
+    def __init__(self, path, required=True):
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                "%s is not built (run `python -c \"import __graft_entry__ as g; g.build()\"`)" % path)
+        self.path = path
+        self.dll = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+        self.missing = []
+        for t in ("f", "d"):
+            for name, (res, args) in _sig(t).items():
+                full = "vsip_%s_%s" % (name, t)
+                try:
+                    fn = getattr(self.dll, full)
+                except AttributeError:
+                    self.missing.append(full)
+                    continue
+                fn.restype = res
+                fn.argtypes = args
+                setattr(self, "%s_%s" % (name, t), fn)
+        for name in ("vsip_init", "vsip_finalize"):
+            fn = getattr(self.dll, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [P]
+            setattr(self, name[5:], fn)
+        self.init(None)
+
+
+class AmdLib(VsipLib):
+    """libvsip_amd.so: the VSIPL API plus the vsip_amd_* extensions."""
+
+    def __init__(self, path=LIB_PATH):
+        super().__init__(path)
+        d = self.dll
+        ext = {
+            "vsip_amd_stream": (P, []),
+            "vsip_amd_sync": (ctypes.c_int, []),
+            "vsip_amd_last_error": (ctypes.c_char_p, []),
+            "vsip_amd_clear_error": (None, []),
+            "vsip_amd_bytes_in_use": (ctypes.c_size_t, []),
+            "vsip_amd_set_hbm_budget": (None, [ctypes.c_size_t]),
+            "vsip_amd_cblock_device_f": (P, [P, ctypes.POINTER(P)]),
+            "vsip_amd_cblock_device_d": (P, [P, ctypes.POINTER(P)]),
+            "vsip_amd_block_device_f": (P, [P]),
+            "vsip_amd_block_device_d": (P, [P]),
+            "vsip_amd_schedule": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.c_int,
+                                                 ctypes.c_char_p, ctypes.c_size_t]),
+            "vsip_amd_fftm_describe_f": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
+            "vsip_amd_fftm_describe_d": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
+        }
+        for name, (res, args) in ext.items():
+            fn = getattr(d, name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name[5:], fn)
+
+    def last_error(self):
+        return self.amd_last_error().decode()
+
+    def schedule(self, prec, n, nvec, lds_max=163840):
+        """Planner output for nvec transforms of length n (no device needed)."""
+        import json
+        buf = ctypes.create_string_buffer(2048)
+        self.amd_schedule(prec, n, nvec, lds_max, buf, 2048)
+        return json.loads(buf.value.decode())
+
+    def describe(self, plan, t="f"):
+        buf = ctypes.create_string_buffer(2048)
+        getattr(self, "amd_fftm_describe_" + t)(plan, buf, 2048)
+        return buf.value.decode()
+
+
+_LIB = None
+
+
+def load():
+    """The MI355X library (built in-tree); raises if it is missing."""
+    global _LIB
+    if _LIB is None:
+        _LIB = AmdLib(LIB_PATH)
+    return _LIB

@@ -1,0 +1,117 @@
+"""Python side of the oracle (test infrastructure only).
+
+Loads oracle/_build/liboracle.so (the double-precision restatement in
+fftm_oracle.c) and, when it was built, oracle/_ref/libjvsip_ref.so (the
+reference's own C compiled from its sources).  Only tests/, the smoke check
+and bench.py's cpu_baseline leg use this module, and only as a checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libjvsip_ref.so")
+
+
+class OView(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_long), ("col_stride", ctypes.c_long),
+                ("col_length", ctypes.c_long), ("row_stride", ctypes.c_long),
+                ("row_length", ctypes.c_long)]
+
+
+_O = None
+
+
+def lib():
+    global _O
+    if _O is None:
+        if not os.path.exists(ORACLE_SO):
+            raise FileNotFoundError("oracle not built: make -C oracle")
+        d = ctypes.CDLL(ORACLE_SO, mode=ctypes.RTLD_LOCAL)
+        P = ctypes.c_void_p
+        d.oracle_dft_ld.argtypes = [ctypes.c_long, P, ctypes.c_int, P]
+        d.oracle_dft_ld.restype = None
+        d.oracle_fft_d.argtypes = [ctypes.c_long, P, ctypes.c_int, P]
+        d.oracle_fft_d.restype = ctypes.c_int
+        for name in ("oracle_ccfftm",):
+            getattr(d, name).argtypes = [P, OView, P, OView, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+            getattr(d, name).restype = ctypes.c_int
+        for name in ("oracle_rcfftm", "oracle_crfftm"):
+            getattr(d, name).argtypes = [P, OView, P, OView, ctypes.c_int, ctypes.c_double]
+            getattr(d, name).restype = ctypes.c_int
+        _O = d
+    return _O
+
+
+def _ov(v):
+    return OView(*[int(x) for x in v])
+
+
+def dft_ld(x, sign=-1):
+    """Brute-force long-double DFT of a 1-D complex vector."""
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.empty_like(x)
+    lib().oracle_dft_ld(x.size, x.ctypes.data, sign, y.ctypes.data)
+    return y
+
+
+def fft(x, sign=-1):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.empty_like(x)
+    assert lib().oracle_fft_d(x.size, x.ctypes.data, sign, y.ctypes.data) == 0
+    return y
+
+
+def ccfftm(blk, view, major=0, direction=-1, scale=1.0, out_blk=None, out_view=None):
+    """Oracle ccfftmip (out_blk None) / ccfftmop; returns the output block (complex128)."""
+    x = np.ascontiguousarray(blk, dtype=np.complex128)
+    if out_blk is None:
+        y, yv = x.copy(), view
+    else:
+        y, yv = np.ascontiguousarray(out_blk, dtype=np.complex128).copy(), out_view
+    rc = lib().oracle_ccfftm(x.ctypes.data, _ov(view), y.ctypes.data, _ov(yv), major, direction, scale)
+    assert rc == 0, rc
+    return y
+
+
+def rcfftm(xblk, xview, zblk, zview, major=0, scale=1.0):
+    x = np.ascontiguousarray(xblk, dtype=np.float64)
+    z = np.ascontiguousarray(zblk, dtype=np.complex128).copy()
+    rc = lib().oracle_rcfftm(x.ctypes.data, _ov(xview), z.ctypes.data, _ov(zview), major, scale)
+    assert rc == 0, rc
+    return z
+
+
+def crfftm(zblk, zview, xblk, xview, major=0, scale=1.0):
+    z = np.ascontiguousarray(zblk, dtype=np.complex128)
+    x = np.ascontiguousarray(xblk, dtype=np.float64).copy()
+    rc = lib().oracle_crfftm(z.ctypes.data, _ov(zview), x.ctypes.data, _ov(xview), major, scale)
+    assert rc == 0, rc
+    return x
+
+
+_REF = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """rrjudd/jvsip's C library built from its sources, bound like libvsip_amd.so."""
+    global _REF
+    if _REF is None:
+        import sys
+        sys.path.insert(0, os.path.dirname(HERE))
+        from jvsip_amd.vsip import VsipLib
+        _REF = VsipLib(REF_SO)
+    return _REF
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))

@@ -1,0 +1,187 @@
+"""GPU parity of the multiple-FFT entry points against the oracle and jvsip.
+
+Every case runs through the C ABI (libvsip_amd.so) with the VSIPL user-block
+sequence bind -> admit -> cmbind -> create -> execute -> release, then compares
+with the double-precision oracle (relative L2) and, where the reference
+library was built, with jvsip's own output on the same input.
+
+Tolerances: FP32 1e-5 relative L2 against jvsip and against the oracle;
+FP64 1e-12 against the oracle and jvsip's _d path.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from jvsip_amd import datagen, fftm
+from jvsip_amd.vsip import VSIP_ROW, VSIP_COL, VSIP_FFT_FWD, VSIP_FFT_INV
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+TOL_F = 1e-5
+TOL_D = 1e-12
+
+
+def _case(name):
+    print("\n[case] %s" % name, flush=True)
+    sys.stdout.flush()
+
+
+def _check(name, got, want, tol):
+    err = po.rel_l2(got, want)
+    print("  %s: rel L2 %.3e" % (name, err), flush=True)
+    assert err < tol, "%s: rel L2 %.3e >= %.1e" % (name, err, tol)
+    return err
+
+
+@pytest.mark.parametrize("m,n", [(64, 1024), (4, 16), (3, 8), (5, 4096), (2, 2), (7, 32),
+                                 (9, 64), (4, 256), (3, 512), (2, 2048), (2, 8192), (1, 16384)])
+def test_ccfftmip_row_pow2(amd, ref, m, n):
+    _case("ccfftmip_f ROW %dx%d fwd" % (m, n))
+    x = datagen.complex_data(1, (m, n))
+    v = fftm.dense_view(m, n)
+    y = fftm.ccfftm(amd, x.ravel(), v)
+    _check("vs oracle", y, po.ccfftm(x.ravel(), v), TOL_F)
+    _check("vs jvsip", y, fftm.ccfftm(ref, x.ravel(), v), TOL_F)
+
+
+@pytest.mark.parametrize("n", [3, 5, 7, 11, 13, 6, 9, 10, 12, 15, 24, 60, 100, 143, 1000, 1001,
+                               1920, 2187, 3125, 6561, 1152, 4000])
+def test_ccfftmip_row_mixed(amd, ref, n):
+    m = 6
+    _case("ccfftmip_f ROW %dx%d fwd" % (m, n))
+    x = datagen.complex_data(2, (m, n))
+    v = fftm.dense_view(m, n)
+    y = fftm.ccfftm(amd, x.ravel(), v)
+    _check("vs oracle", y, po.ccfftm(x.ravel(), v), TOL_F)
+    _check("vs jvsip", y, fftm.ccfftm(ref, x.ravel(), v), TOL_F)
+
+
+@pytest.mark.parametrize("direction,scale", [(VSIP_FFT_INV, 1.0), (VSIP_FFT_INV, 1.0 / 1024),
+                                             (VSIP_FFT_FWD, 0.5)])
+def test_ccfftmip_dir_scale(amd, ref, direction, scale):
+    _case("ccfftmip_f ROW 64x1024 dir %d scale %g" % (direction, scale))
+    x = datagen.complex_data(11, (64, 1024))
+    v = fftm.dense_view(64, 1024)
+    y = fftm.ccfftm(amd, x.ravel(), v, direction=direction, scale=scale)
+    _check("vs oracle", y, po.ccfftm(x.ravel(), v, direction=direction, scale=scale), TOL_F)
+    _check("vs jvsip", y, fftm.ccfftm(ref, x.ravel(), v, direction=direction, scale=scale), TOL_F)
+
+
+def test_fwd_inv_roundtrip(amd):
+    _case("ccfftmip_f ROW 128x4096 fwd then inv 1/N")
+    x = datagen.complex_data(2, (128, 4096), dist="uniform")
+    v = fftm.dense_view(128, 4096)
+    y = fftm.ccfftm(amd, x.ravel(), v)
+    z = fftm.ccfftm(amd, y, v, direction=VSIP_FFT_INV, scale=1.0 / 4096)
+    _check("round trip", z, x.ravel(), 1e-6)
+
+
+@pytest.mark.parametrize("m,n", [(1920, 64), (64, 1920), (1000, 16), (16, 4096), (8, 3)])
+def test_ccfftmip_col(amd, ref, m, n):
+    _case("ccfftmip_f COL %dx%d fwd" % (m, n))
+    x = datagen.complex_data(3, (m, n))
+    v = fftm.dense_view(m, n)
+    y = fftm.ccfftm(amd, x.ravel(), v, major=VSIP_COL)
+    _check("vs oracle", y, po.ccfftm(x.ravel(), v, major=VSIP_COL), TOL_F)
+    _check("vs jvsip", y, fftm.ccfftm(ref, x.ravel(), v, major=VSIP_COL), TOL_F)
+
+
+def test_strided_view_kat_geometry(amd, ref):
+    # the geometry of the reference's own test: cmbind(block, 4, 2, 8, 17, 4)
+    _case("ccfftmip_f strided view offset 4 col_stride 2 row_stride 17")
+    blk = datagen.complex_data(5, (1000,))
+    v = fftm.View(4, 2, 8, 17, 4)
+    for major in (VSIP_ROW, VSIP_COL):
+        y = fftm.ccfftm(amd, blk, v, major=major)
+        want = po.ccfftm(blk, v, major=major)
+        _check("vs oracle major %d" % major, fftm.gather(y, v), fftm.gather(want, v), TOL_F)
+        np.testing.assert_array_equal(np.delete(y, _idx(v)), np.delete(blk, _idx(v)))
+        _check("vs jvsip major %d" % major, y, fftm.ccfftm(ref, blk, v, major=major), TOL_F)
+
+
+def _idx(v):
+    i = np.arange(v.col_length)[:, None]
+    j = np.arange(v.row_length)[None, :]
+    return (v.offset + i * v.col_stride + j * v.row_stride).ravel()
+
+
+@pytest.mark.parametrize("major", [VSIP_ROW, VSIP_COL])
+def test_ccfftmop_subview_parent2x(amd, ref, major):
+    # rows lie 2N apart: the left half of a 2N-wide parent
+    m, n = (64, 2187) if major == VSIP_ROW else (1920, 100)
+    _case("ccfftmop_f major %d %dx%d parent row 2N" % (major, m, n))
+    x = datagen.complex_data(3, (m * 2 * n,))
+    xv = fftm.View(0, 2 * n, m, 1, n)
+    y0 = np.zeros(m * n, np.complex64)
+    yv = fftm.dense_view(m, n)
+    xr, y = fftm.ccfftm(amd, x, xv, major=major, out_blk=y0, out_view=yv)
+    np.testing.assert_array_equal(xr, x)  # input preserved
+    _check("vs oracle", y, po.ccfftm(x, xv, major=major, out_blk=y0, out_view=yv), TOL_F)
+    _, yr = fftm.ccfftm(ref, x, xv, major=major, out_blk=y0, out_view=yv)
+    _check("vs jvsip", y, yr, TOL_F)
+
+
+@pytest.mark.parametrize("m,n", [(16, 64), (8192 // 64, 2048), (3, 2), (5, 30), (4, 2000)])
+def test_rcfftmop_row(amd, ref, m, n):
+    _case("rcfftmop_f ROW %dx%d" % (m, n))
+    x = datagen.real_data(2, (m, n))
+    z0 = np.zeros(m * (n // 2 + 1), np.complex64)
+    xv, zv = fftm.dense_view(m, n), fftm.dense_view(m, n // 2 + 1)
+    z = fftm.rcfftm(amd, x.ravel(), xv, z0, zv)
+    _check("vs oracle", z, po.rcfftm(x.ravel(), xv, z0, zv), TOL_F)
+    _check("vs jvsip", z, fftm.rcfftm(ref, x.ravel(), xv, z0, zv), TOL_F)
+
+
+@pytest.mark.parametrize("m,n", [(16, 64), (128, 2048), (5, 30)])
+def test_crfftmop_row(amd, ref, m, n):
+    _case("crfftmop_f ROW %dx%d (non-Hermitian input)" % (m, n))
+    z = datagen.complex_data(7, (m * (n // 2 + 1),))
+    x0 = np.zeros(m * n, np.float32)
+    zv, xv = fftm.dense_view(m, n // 2 + 1), fftm.dense_view(m, n)
+    x = fftm.crfftm(amd, z, zv, x0, xv, scale=0.25)
+    _check("vs oracle", x, po.crfftm(z, zv, x0, xv, scale=0.25), TOL_F)
+    _check("vs jvsip", x, fftm.crfftm(ref, z, zv, x0, xv, scale=0.25), TOL_F)
+
+
+def test_real_roundtrip_col(amd):
+    m, n = 64, 48
+    _case("rcfftmop_f/crfftmop_f COL %dx%d round trip" % (m, n))
+    x = datagen.real_data(9, (m, n))
+    xv = fftm.dense_view(m, n, VSIP_COL)
+    zv = fftm.dense_view(m // 2 + 1, n, VSIP_COL)
+    z = fftm.rcfftm(amd, x.ravel(), xv, np.zeros((m // 2 + 1) * n, np.complex64), zv, major=VSIP_COL)
+    _check("rc vs oracle", z, po.rcfftm(x.ravel(), xv, np.zeros((m // 2 + 1) * n, np.complex64), zv, major=VSIP_COL), TOL_F)
+    xb = fftm.crfftm(amd, z, zv, np.zeros(m * n, np.float32), xv, major=VSIP_COL, scale=1.0 / m)
+    _check("round trip", xb, x.ravel(), 1e-6)
+
+
+@pytest.mark.parametrize("m,n,major", [(8, 64, VSIP_ROW), (16, 2048, VSIP_ROW), (60, 40, VSIP_COL),
+                                       (4, 1001, VSIP_ROW)])
+def test_ccfftmip_d(amd, ref, m, n, major):
+    _case("ccfftmip_d %dx%d major %d" % (m, n, major))
+    x = datagen.complex_data(3, (m, n), dtype=np.complex128)
+    v = fftm.dense_view(m, n)
+    y = fftm.ccfftm(amd, x.ravel(), v, major=major)
+    _check("vs oracle", y, po.ccfftm(x.ravel(), v, major=major), TOL_D)
+    _check("vs jvsip _d", y, fftm.ccfftm(ref, x.ravel(), v, major=major), TOL_D)
+
+
+def test_split_complex(amd):
+    _case("ccfftmip_f split block 8x64")
+    x = datagen.complex_data(4, (8, 64))
+    v = fftm.dense_view(8, 64)
+    y = fftm.ccfftm(amd, x.ravel(), v, split=True)
+    _check("vs oracle", y, po.ccfftm(x.ravel(), v), TOL_F)
+
+
+@pytest.mark.parametrize("n", [65536, 32768, 40000])
+def test_four_step(amd, n):
+    m = 3
+    _case("ccfftmip_f ROW %dx%d (two-pass)" % (m, n))
+    x = datagen.complex_data(8, (m, n))
+    v = fftm.dense_view(m, n)
+    for d in (VSIP_FFT_FWD, VSIP_FFT_INV):
+        y = fftm.ccfftm(amd, x.ravel(), v, direction=d, scale=0.5)
+        _check("vs oracle dir %d" % d, y, po.ccfftm(x.ravel(), v, direction=d, scale=0.5), TOL_F)
