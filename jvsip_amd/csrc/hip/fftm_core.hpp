@@ -72,6 +72,12 @@ __device__ __forceinline__ void dft4(cpx<T> &x0, cpx<T> &x1, cpx<T> &x2, cpx<T> 
 template <typename T, int R>
 struct Dft;
 
+// length-1 transforms (the real transforms of length 2 use one)
+template <typename T>
+struct Dft<T, 1> {
+    __device__ __forceinline__ static void run(cpx<T> *) {}
+};
+
 template <typename T>
 struct Dft<T, 2> {
     __device__ __forceinline__ static void run(cpx<T> *x) { dft2(x[0], x[1]); }
@@ -387,6 +393,7 @@ __device__ __forceinline__ void run_stage(const vhip_fftm_desc &d, const Shared 
     case 7: stage_dispatch_pos<T, 7, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
     case 11: stage_dispatch_pos<T, 11, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
     case 13: stage_dispatch_pos<T, 13, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 1: stage_dispatch_pos<T, 1, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
     default: break;  // the planner only emits the radices above
     }
 }

@@ -16,10 +16,10 @@
  *   destroy                 vsip_fftm_destroy_f.c:20-40 (returns 0)
  * create returns NULL on allocation failure as the reference does
  * (VI_ccfftmip_create_f_def.h:34,48-53), and also for lengths this
- * implementation does not transform: N = 1 (undefined in the reference,
- * whose factoriser then reads pF[-1]), odd N for the real transforms
- * (undefined in the reference, which truncates N/2), and lengths with a
- * prime factor above 13.
+ * implementation does not transform: odd N for the real transforms
+ * (undefined in the reference, which truncates N/2) and lengths with a prime
+ * factor above 13.  Length 1 (undefined in the reference, whose factoriser
+ * then reads pF[-1]) is the identity times scale here.
  *
  * Execution is one kernel launch per call on the library stream
  * (two for lengths beyond one workgroup's LDS, split as MN = n1 * n2 with
@@ -125,7 +125,7 @@ int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, d
         p->MN = (major == VSIP_COL) ? M : N;
         p->mN = (major == VSIP_COL) ? N : M;
     }
-    if (p->MN < 2 && !real) return -1;
+    if (p->MN < 1) return -1;
     if (p->MN > 0x7fffffffUL) return -1;
     p->npass = 1;
     rc = va_sched_plan(&p->pass[0], prec, (int)p->MN, p->mN);

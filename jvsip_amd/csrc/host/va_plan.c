@@ -58,6 +58,7 @@ static int va_factor(int n, int prec, int *radix)
         }
     }
     if (m != 1) return -1;
+    if (n == 1) radix[nst++] = 1;   /* length 1: one identity stage */
     return nst;
 }
 
@@ -132,8 +133,7 @@ int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max)
     memset(s, 0, sizeof(*s));
     s->n = n;
     s->nst = va_factor(n, prec, s->radix);
-    if (s->nst < 0) return -1;
-    if (n < 2) return -1;
+    if (s->nst < 0 || n < 1) return -1;
     s->lds_stride = va_pad(prec, n);
     if (s->lds_stride % 2 == 0) s->lds_stride += 1;
 

@@ -55,7 +55,7 @@ def _kernel_rules_hold(s, prec, lds_max):
     n = s["n"]
     prod = 1
     for r in s["radix"]:
-        assert r in (2, 3, 4, 5, 7, 8, 11, 13) + ((16,) if prec == 4 else ())
+        assert r in (1, 2, 3, 4, 5, 7, 8, 11, 13) + ((16,) if prec == 4 else ())
         prod *= r
     assert prod == n
     S = s["lds_stride"]
@@ -72,7 +72,7 @@ def _kernel_rules_hold(s, prec, lds_max):
 def test_planner_all_small_lengths(lib, prec):
     lds_max = 163840
     planned = 0
-    for n in range(2, 3000):
+    for n in range(1, 3000):
         s = lib.schedule(prec, n, 1000, lds_max)
         m = n
         for p in (2, 3, 5, 7, 11, 13):

@@ -131,7 +131,8 @@ def test_rcfftmop_row(amd, ref, m, n):
     xv, zv = fftm.dense_view(m, n), fftm.dense_view(m, n // 2 + 1)
     z = fftm.rcfftm(amd, x.ravel(), xv, z0, zv)
     _check("vs oracle", z, po.rcfftm(x.ravel(), xv, z0, zv), TOL_F)
-    _check("vs jvsip", z, fftm.rcfftm(ref, x.ravel(), xv, z0, zv), TOL_F)
+    if n >= 4:  # jvsip divides by zero planning its n/2 = 1 point transform (VI_nuV.h)
+        _check("vs jvsip", z, fftm.rcfftm(ref, x.ravel(), xv, z0, zv), TOL_F)
 
 
 @pytest.mark.parametrize("m,n", [(16, 64), (128, 2048), (5, 30)])
