@@ -502,5 +502,7 @@ typedef void (*kernel_fn)(const vhip_fftm_desc);
 kernel_fn fftm_kernel_f16(int kind);
 kernel_fn fftm_kernel_f32(int kind);
 kernel_fn fftm_kernel_d8(int kind);
+int pow2_try_launch(const vhip_fftm_desc *d, void *stream, int cus);
+int pow2_set_lds_limit(int bytes);
 
 }  // namespace vhipk

@@ -1,6 +1,8 @@
 // fftm_launch.hip -- C ABI entry points that launch the multiple-FFT kernels.
 #include "fftm_core.hpp"
 
+extern "C" int vhip_cu_count(int *cus);
+
 using vhipk::kernel_fn;
 
 static kernel_fn pick_kernel(const vhip_fftm_desc *d) {
@@ -31,11 +33,15 @@ extern "C" int vhip_fftm_set_lds_limit(int bytes) {
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
                 if (e != hipSuccess) return (int)e;
             }
-    return 0;
+    return vhipk::pow2_set_lds_limit(bytes);
 }
 
 extern "C" int vhip_fftm_launch(const vhip_fftm_desc *d, void *stream) {
     if (!d || d->nvec <= 0 || d->vpb <= 0 || d->threads <= 0 || d->threads > 1024) return -1;
+    int cus = 256;
+    vhip_cu_count(&cus);
+    int rc = vhipk::pow2_try_launch(d, stream, cus > 0 ? cus : 256);
+    if (rc != -100) return rc;
     kernel_fn f = pick_kernel(d);
     if (!f) return -2;
     const long grid = (d->nvec + d->vpb - 1) / d->vpb;
