@@ -49,6 +49,26 @@ int vsip_amd_schedule(int prec, long n, long nvec, int lds_max, char *buf, size_
 int vsip_amd_fftm_describe_f(const vsip_fftm_f *plan, char *buf, size_t len);
 int vsip_amd_fftm_describe_d(const vsip_fftm_d *plan, char *buf, size_t len);
 
+/* ---- batched multiple FFTs: one persistent launch in a HIP graph --------
+ * njobs (plan, input view, output view) triples; ccfftmip_f plans run in
+ * place on ins[i] (outs may be NULL), ccfftmop_f plans write outs[i].  Every
+ * transform runs with the schedule the autotuner picked for its length and
+ * the batched workgroup shape.  run() replays the captured graph on the
+ * library stream (asynchronous, ordered with the plain calls).  The views'
+ * blocks must stay admitted for the life of the batch.  Returns NULL on
+ * failure (vsip_amd_last_error says why). */
+typedef struct vsip_amd_batch_f vsip_amd_batch_f;
+vsip_amd_batch_f *vsip_amd_batch_create_f(vsip_length njobs, vsip_fftm_f *const *plans,
+                                          const vsip_cmview_f *const *ins,
+                                          const vsip_cmview_f *const *outs);
+int vsip_amd_batch_run_f(vsip_amd_batch_f *b);
+void vsip_amd_batch_destroy_f(vsip_amd_batch_f *b);
+/* JSON: jobs, descriptors, work items, grid, LDS bytes, elements */
+int vsip_amd_batch_info_f(const vsip_amd_batch_f *b, char *buf, size_t len);
+/* the batched schedule the autotuner picks for length n (tunes on first use;
+ * VSIP_AMD_TUNE_CACHE=<file> keeps results across processes), as JSON */
+int vsip_amd_tune(int prec, long n, char *buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

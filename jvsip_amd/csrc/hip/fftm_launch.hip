@@ -33,7 +33,8 @@ extern "C" int vhip_fftm_set_lds_limit(int bytes) {
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
                 if (e != hipSuccess) return (int)e;
             }
-    return vhipk::pow2_set_lds_limit(bytes);
+    int rc = vhipk::pow2_set_lds_limit(bytes);
+    return rc ? rc : vhipk::batch_set_lds_limit(bytes);
 }
 
 extern "C" int vhip_fftm_launch(const vhip_fftm_desc *d, void *stream) {

@@ -84,6 +84,7 @@ int vhip_free(void *p);
 int vhip_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int vhip_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int vhip_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int vhip_memset(void *dst, int value, size_t bytes);   /* synchronous, outside graphs only */
 /* 2-D strided copies: `rows` rows of `width` bytes, row pitches in bytes */
 int vhip_memcpy2d_h2d(void *dst, size_t dpitch, const void *src, size_t spitch,
                       size_t width, size_t rows);
@@ -106,10 +107,44 @@ typedef struct {
     int pad_;
 } vhip_mat;
 
+/* One work item of a batched launch: vectors [vbase, vbase + vpb) of the
+ * transform described by descs[desc].  Items of the second pass of a
+ * two-pass transform wait until ctl[1 + wait_slot] reaches wait_count; items
+ * of its first pass add one to ctl[1 + signal_slot] when done. */
+typedef struct {
+    int desc;
+    int wait_slot;      /* -1: no wait */
+    int wait_count;
+    int signal_slot;    /* -1: no signal */
+    long vbase;
+} vhip_batch_item;
+
 /* kernels */
 int vhip_mat_copy(int prec, int cplx, const vhip_mat *src, const vhip_mat *dst,
                   long m, long n, void *stream);
 int vhip_fftm_launch(const vhip_fftm_desc *d, void *stream);
+
+/* Batched launch: one persistent kernel of `threads` threads per workgroup
+ * and `lds_bytes` of LDS per workgroup drains the item list in order; ctl
+ * holds 1 + nslots counters and is cleared by a fill kernel first.  Both
+ * launches go to `stream` (which may be capturing a graph). */
+int vhip_batch_launch(const vhip_fftm_desc *descs, const vhip_batch_item *items, long nitems,
+                      unsigned *ctl, int nslots, int threads, int maxe, int lds_bytes,
+                      int grid, void *stream);
+int vhip_batch_max_grid(int threads, int maxe, int lds_bytes, int *grid);
+
+/* graphs: capture everything issued to `stream` between begin and end */
+int vhip_stream_create(void **stream);
+int vhip_stream_destroy(void *stream);
+int vhip_graph_begin(void *stream);
+int vhip_graph_end(void *stream, void **graph_exec);
+int vhip_graph_launch(void *graph_exec, void *stream);
+int vhip_graph_destroy(void *graph_exec);
+/* HBM reference kernels over `bytes` of device memory: kind 0 in-place
+ * read-modify-write, kind 1 read-only (sink: 4 writable bytes) */
+int vhip_probe_launch(int kind, void *buf, size_t bytes, void *sink, void *stream);
+/* time `reps` runs of fn(arg) on stream with events; returns milliseconds per run */
+int vhip_event_time(void *stream, int (*fn)(void *arg), void *arg, int reps, float *ms);
 size_t vhip_fftm_lds_bytes(const vhip_fftm_desc *d);   /* dynamic LDS of one launch */
 
 #ifdef __cplusplus

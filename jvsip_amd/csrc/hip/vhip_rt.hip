@@ -125,6 +125,13 @@ extern "C" int vhip_memcpy_d2d(void *dst, const void *src, size_t bytes) {
     return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream);
 }
 
+extern "C" int vhip_memset(void *dst, int value, size_t bytes) {
+    if (!bytes) return 0;
+    hipError_t e = hipMemsetAsync(dst, value, bytes, g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    return (int)e;
+}
+
 extern "C" int vhip_memcpy2d_h2d(void *dst, size_t dpitch, const void *src, size_t spitch,
                                  size_t width, size_t rows) {
     if (!width || !rows) return 0;
@@ -200,5 +207,96 @@ extern "C" int vhip_mat_copy(int prec, int cplx, const vhip_mat *src, const vhip
     else if (prec == 8 && cplx) hipLaunchKernelGGL((mat_copy_kernel<double, true>), dim3(blocks), dim3(256), 0, s, *src, *dst, m, n);
     else if (prec == 8) hipLaunchKernelGGL((mat_copy_kernel<double, false>), dim3(blocks), dim3(256), 0, s, *src, *dst, m, n);
     else return -1;
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// streams, graphs and event timing
+// ---------------------------------------------------------------------------
+extern "C" int vhip_stream_create(void **stream) {
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    *stream = (void *)s;
+    return (int)e;
+}
+
+extern "C" int vhip_stream_destroy(void *stream) {
+    if (!stream) return 0;
+    return (int)hipStreamDestroy((hipStream_t)stream);
+}
+
+extern "C" int vhip_graph_begin(void *stream) {
+    return (int)hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal);
+}
+
+extern "C" int vhip_graph_end(void *stream, void **graph_exec) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    *graph_exec = nullptr;
+    hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g);
+    if (e != hipSuccess) return (int)e;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return (int)e;
+    *graph_exec = (void *)x;
+    return 0;
+}
+
+extern "C" int vhip_graph_launch(void *graph_exec, void *stream) {
+    return (int)hipGraphLaunch((hipGraphExec_t)graph_exec, stream ? (hipStream_t)stream : g_stream);
+}
+
+extern "C" int vhip_graph_destroy(void *graph_exec) {
+    if (!graph_exec) return 0;
+    return (int)hipGraphExecDestroy((hipGraphExec_t)graph_exec);
+}
+
+extern "C" int vhip_event_time(void *stream, int (*fn)(void *arg), void *arg, int reps, float *ms) {
+    hipStream_t s = stream ? (hipStream_t)stream : g_stream;
+    hipEvent_t a, b;
+    *ms = 0.f;
+    hipError_t e = hipEventCreate(&a);
+    if (e != hipSuccess) return (int)e;
+    e = hipEventCreate(&b);
+    if (e != hipSuccess) { (void)hipEventDestroy(a); return (int)e; }
+    int rc = fn(arg);  // warm-up
+    if (!rc) rc = (int)hipEventRecord(a, s);
+    for (int i = 0; i < reps && !rc; ++i) rc = fn(arg);
+    if (!rc) rc = (int)hipEventRecord(b, s);
+    if (!rc) rc = (int)hipEventSynchronize(b);
+    float t = 0.f;
+    if (!rc) rc = (int)hipEventElapsedTime(&t, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = reps > 0 ? t / reps : 0.f;
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// HBM reference kernels: in-place read-modify-write and read-only streams
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void probe_rmw(float4 *p, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        float4 v = p[i];
+        v.x *= 1.0000001f;
+        p[i] = v;
+    }
+}
+__global__ void probe_read(const float4 *p, long n, float *sink) {
+    float acc = 0.f;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        float4 v = p[i];
+        acc += v.x + v.y + v.z + v.w;
+    }
+    if (acc == 1.2345e30f) sink[0] = acc;  // never true for the probe's data; keeps the loads
+}
+}  // namespace
+
+extern "C" int vhip_probe_launch(int kind, void *buf, size_t bytes, void *sink, void *stream) {
+    const long n = (long)(bytes / 16);
+    hipStream_t s = stream ? (hipStream_t)stream : g_stream;
+    if (kind == 0) hipLaunchKernelGGL(probe_rmw, dim3(32768), dim3(256), 0, s, (float4 *)buf, n);
+    else hipLaunchKernelGGL(probe_read, dim3(32768), dim3(256), 0, s, (const float4 *)buf, n, (float *)sink);
     return (int)hipGetLastError();
 }

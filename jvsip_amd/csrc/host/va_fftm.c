@@ -213,13 +213,21 @@ static void va_desc_common(vhip_fftm_desc *d, const VA_fftm *p, const VA_sched *
     d->scale = 1.0;
 }
 
-int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void *stream)
+/*
+ * Launch descriptors of one execution of plan p on (in, out): one for a
+ * single pass, two for the four-step split.  sA/sB, work and ptw replace the
+ * plan's own schedules, scratch and twiddle table when given (the batched
+ * launch uses schedules tuned for its fixed workgroup size); sB != NULL
+ * selects the split with n1 = sA->n, n2 = sB->n.  Returns the descriptor
+ * count, or -1 after reporting an error.
+ */
+int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, const VA_sched *sA,
+                  const VA_sched *sB, void *work, const void *ptw, vhip_fftm_desc *d)
 {
     long es_i, vs_i, es_o, vs_o;
     vsip_length n_i, nv_i, n_o, nv_o, want_in, want_out;
     vhip_view gi, go;
-    vhip_fftm_desc d;
-    int kind, rc, inv;
+    int kind, inv, two;
     if (!p || p->markings != VA_VALID) { va_error("fftm", "invalid plan", -1); return -1; }
     if (!in || !out) { va_error("fftm", "NULL view", -1); return -1; }
     if (in->block->prec != p->prec || out->block->prec != p->prec) {
@@ -240,9 +248,16 @@ int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void
         return -1;
     }
     inv = (kind == VHIP_C2C && p->dir > 0);
-    if (p->mN == 0) return 0;
+    if (!sA) {
+        sA = &p->pass[0];
+        sB = (p->npass == 2) ? &p->pass[1] : NULL;
+        work = p->dev_work;
+        ptw = p->dev_ptw;
+    }
+    two = (sB != NULL);
+    if (two && kind != VHIP_C2C) { va_error("fftm", "two-pass real transform", -1); return -1; }
 
-    if (p->npass == 1) {
+    if (!two) {
         int map_in = va_map(es_i, vs_i, nv_i), map_out = va_map(es_o, vs_o, nv_o);
         gi.es = es_i; gi.vs0 = vs_i; gi.nv0 = (long)p->mN; gi.vs1 = 0;
         go.es = es_o; go.vs0 = vs_o; go.nv0 = (long)p->mN; go.vs1 = 0;
@@ -254,52 +269,61 @@ int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void
             go.contig = (es_o == 1 && (out->offset % 2) == 0 && (p->mN <= 1 || vs_o % 2 == 0));
         else
             go.contig = (!go.split && es_o == 1);
-        va_desc_common(&d, p, &p->pass[0], kind, (long)p->mN, map_in, map_out);
-        d.in = gi;
-        d.out = go;
-        d.conj_in = inv;
-        d.conj_out = inv;
-        d.scale = p->scale;
-        d.euler = p->dev_euler;
-        rc = vhip_fftm_launch(&d, stream);
-        if (rc) va_error("fftm", vhip_error_string(rc), rc);
-        return rc;
+        va_desc_common(&d[0], p, sA, kind, (long)p->mN, map_in, map_out);
+        d[0].in = gi;
+        d[0].out = go;
+        d[0].conj_in = inv;
+        d[0].conj_out = inv;
+        d[0].scale = p->scale;
+        d[0].euler = p->dev_euler;
+        return 1;
     }
 
     /* four-step: MN = n1 * n2, element n = n2 * j1 + j2 */
     {
-        long n1 = (long)p->n1, n2 = (long)p->n2, MN = (long)p->MN, mN = (long)p->mN;
+        long n1 = sA->n, n2 = sB->n, MN = (long)p->MN, mN = (long)p->mN;
         vhip_view t;
+        if (n1 * n2 != MN) { va_error("fftm", "two-pass split does not match the length", -1); return -1; }
         memset(&t, 0, sizeof(t));
-        t.re = p->dev_work;
+        t.re = work;
         /* pass A: vectors (row, j2), elements j1 at stride n2*es; out T[row][k1][j2] */
         gi.es = n2 * es_i; gi.vs0 = es_i; gi.nv0 = n2; gi.vs1 = vs_i;
         gi.contig = 0;
         t.es = n2; t.vs0 = 1; t.nv0 = n2; t.vs1 = MN;
-        va_desc_common(&d, p, &p->pass[0], VHIP_C2C, mN * n2, VHIP_MAP_ACROSS, VHIP_MAP_ACROSS);
-        d.in = gi;
-        d.out = t;
-        d.conj_in = inv;
-        d.conj_out = 0;
-        d.scale = 1.0;
-        d.ptw = p->dev_ptw;
-        d.ptw_n = MN;
-        rc = vhip_fftm_launch(&d, stream);
-        if (rc) { va_error("fftm", vhip_error_string(rc), rc); return rc; }
+        va_desc_common(&d[0], p, sA, VHIP_C2C, mN * n2, VHIP_MAP_ACROSS, VHIP_MAP_ACROSS);
+        d[0].in = gi;
+        d[0].out = t;
+        d[0].conj_in = inv;
+        d[0].conj_out = 0;
+        d[0].scale = 1.0;
+        d[0].ptw = ptw;
+        d[0].ptw_n = MN;
         /* pass B: vectors (row, k1), elements j2 contiguous; out X[row][k1 + n1*k2] */
         t.es = 1; t.vs0 = n2; t.nv0 = n1; t.vs1 = MN; t.contig = 1;
         go.es = n1 * es_o; go.vs0 = es_o; go.nv0 = n1; go.vs1 = vs_o;
         go.contig = 0;
-        va_desc_common(&d, p, &p->pass[1], VHIP_C2C, mN * n1, VHIP_MAP_ALONG, VHIP_MAP_ACROSS);
-        d.in = t;
-        d.out = go;
-        d.conj_in = 0;
-        d.conj_out = inv;
-        d.scale = p->scale;
-        rc = vhip_fftm_launch(&d, stream);
-        if (rc) va_error("fftm", vhip_error_string(rc), rc);
-        return rc;
+        va_desc_common(&d[1], p, sB, VHIP_C2C, mN * n1, VHIP_MAP_ALONG, VHIP_MAP_ACROSS);
+        d[1].in = t;
+        d[1].out = go;
+        d[1].conj_in = 0;
+        d[1].conj_out = inv;
+        d[1].scale = p->scale;
+        return 2;
     }
+}
+
+int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void *stream)
+{
+    vhip_fftm_desc d[2];
+    int n, k, rc = 0;
+    if (p && p->mN == 0) return 0;
+    n = va_fftm_descs(p, in, out, NULL, NULL, NULL, NULL, d);
+    if (n < 0) return -1;
+    for (k = 0; k < n && !rc; k++) {
+        rc = vhip_fftm_launch(&d[k], stream);
+        if (rc) va_error("fftm", vhip_error_string(rc), rc);
+    }
+    return rc;
 }
 
 /* ---- public API -------------------------------------------------------- */

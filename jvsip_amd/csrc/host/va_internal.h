@@ -116,6 +116,13 @@ typedef struct VA_fftm {
 struct vsip_fftmattributes_f { VA_fftm p; };
 struct vsip_fftmattributes_d { VA_fftm p; };
 
+/* Workgroup shape of the batched (persistent) kernel: every transform of a
+ * batch runs with these lanes, register class and LDS bytes per workgroup;
+ * single-pass lengths up to 8192 fit, longer ones run in two passes. */
+#define VA_BATCH_THREADS 512
+#define VA_BATCH_MAXE 16
+#define VA_BATCH_LDS 67712
+
 /* ---- shared helpers ------------------------------------------------------ */
 int va_ensure_init(void);
 void va_error(const char *where, const char *what, int code);
@@ -137,10 +144,17 @@ int va_mview_copy_host(const VA_mview *v, int to_device, void *r, void *im, vsip
 /* planning and execution */
 int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max);
 int va_sched_plan(VA_sched *s, int prec, int n, vsip_length nvec);
+int va_sched_shape_fixed(VA_sched *s, int prec, int n, const int *radix, int nst,
+                         int threads, int maxe, int lds_max, int vpb_max);
+int va_sched_upload(VA_sched *s, int prec);
+int va_factor_variant(int n, int prec, int maxbits, int odd_first, int *radix);
+int va_tune_get(int prec, int n, const VA_sched **out);
 void va_sched_free(VA_sched *s);
 int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, double scale,
                  int dir, vsip_major major, unsigned int ntimes, vsip_alg_hint hint);
 void va_fftm_release(VA_fftm *p);
 int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void *stream);
+int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, const VA_sched *sA,
+                  const VA_sched *sB, void *work, const void *ptw, vhip_fftm_desc *d);
 
 #endif /* VA_INTERNAL_H */

@@ -33,33 +33,53 @@ static int va_pad(int prec, int i)
     return i + (i >> (prec == 4 ? 5 : 4));
 }
 
-/* factor n into stage radices; returns the stage count or -1 */
-static int va_factor(int n, int prec, int *radix)
+/*
+ * Factor n into stage radices: the power-of-two part split as evenly as
+ * possible into stages of at most `maxbits` bits, then one stage per factor
+ * 3, 5, 7, 11, 13 (odd stages first when odd_first).  Returns the stage count,
+ * or -1 for a prime factor above 13.
+ */
+int va_factor_variant(int n, int prec, int maxbits, int odd_first, int *radix)
 {
-    int m = n, p2 = 0, nst = 0, s, stages2, base, extra, i;
+    int m = n, p2 = 0, nst = 0, s, stages2, base, extra, i, nodd = 0;
+    int odds[VHIP_MAX_STAGES], pw[VHIP_MAX_STAGES], npw = 0;
     static const int odd[5] = {3, 5, 7, 11, 13};
-    int maxbits = (prec == 4) ? 4 : 3;   /* radix 16 only in the float kernels */
+    if (prec != 4 && maxbits > 3) maxbits = 3;   /* radix 16 only in the float kernels */
+    if (maxbits < 1) maxbits = 1;
+    if (n < 1) return -1;
     while (m % 2 == 0) { m /= 2; p2++; }
     if (p2 > 0) {
         stages2 = (p2 + maxbits - 1) / maxbits;
         base = p2 / stages2;
         extra = p2 % stages2;
         for (s = 0; s < stages2; s++) {
-            int bits = base + (s < extra ? 1 : 0);
-            if (nst >= VHIP_MAX_STAGES) return -1;
-            radix[nst++] = 1 << bits;
+            if (npw >= VHIP_MAX_STAGES) return -1;
+            pw[npw++] = 1 << (base + (s < extra ? 1 : 0));
         }
     }
     for (i = 0; i < 5; i++) {
         while (m % odd[i] == 0) {
-            if (nst >= VHIP_MAX_STAGES) return -1;
-            radix[nst++] = odd[i];
+            if (nodd >= VHIP_MAX_STAGES) return -1;
+            odds[nodd++] = odd[i];
             m /= odd[i];
         }
     }
-    if (m != 1) return -1;
+    if (m != 1 || npw + nodd > VHIP_MAX_STAGES) return -1;
+    if (odd_first) {
+        for (i = 0; i < nodd; i++) radix[nst++] = odds[i];
+        for (i = 0; i < npw; i++) radix[nst++] = pw[i];
+    } else {
+        for (i = 0; i < npw; i++) radix[nst++] = pw[i];
+        for (i = 0; i < nodd; i++) radix[nst++] = odds[i];
+    }
     if (n == 1) radix[nst++] = 1;   /* length 1: one identity stage */
     return nst;
+}
+
+/* the default schedule's factorisation */
+static int va_factor(int n, int prec, int *radix)
+{
+    return va_factor_variant(n, prec, prec == 4 ? 4 : 3, 0, radix);
 }
 
 static int va_maxb(int r, int maxe)
@@ -161,17 +181,59 @@ int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max)
     return 0;
 }
 
-/* shape plus the stage twiddles uploaded to the device */
-int va_sched_plan(VA_sched *s, int prec, int n, vsip_length nvec)
+/*
+ * Shape for a fixed workgroup size (the batched kernel): the given radix
+ * sequence (or the default factorisation when radix == NULL), and the largest
+ * vectors-per-workgroup up to vpb_max that fits `threads` lanes with register
+ * class maxe and `lds_max` bytes of LDS.  Returns 0, -1 (unsupported length)
+ * or -2 (does not fit).
+ */
+int va_sched_shape_fixed(VA_sched *s, int prec, int n, const int *radix, int nst,
+                         int threads, int maxe, int lds_max, int vpb_max)
 {
-    int lds_max = 65536, i, r, k, Ns, off, rc;
-    vhip_lds_max(&lds_max);
-    rc = va_sched_shape(s, prec, n, nvec, lds_max);
-    if (rc) return rc;
+    int i, off, Ns, vpb;
+    memset(s, 0, sizeof(*s));
+    s->n = n;
+    if (radix) {
+        long prod = 1;
+        if (nst < 1 || nst > VHIP_MAX_STAGES) return -1;
+        for (i = 0; i < nst; i++) { s->radix[i] = radix[i]; prod *= radix[i]; }
+        if (prod != n) return -1;
+        s->nst = nst;
+    } else {
+        s->nst = va_factor(n, prec, s->radix);
+        if (s->nst < 0) return -1;
+    }
+    s->lds_stride = va_pad(prec, n);
+    if (s->lds_stride % 2 == 0) s->lds_stride += 1;
+    for (vpb = vpb_max; vpb >= 1; vpb--) {
+        if (va_lds_bytes(s, prec, vpb) > lds_max) continue;
+        if (va_shape_ok(s, vpb, threads, maxe)) break;
+    }
+    if (vpb < 1) return -2;
+    s->along.vpb = s->across.vpb = vpb;
+    s->along.threads = s->across.threads = threads;
+    s->along.maxe = s->across.maxe = maxe;
+    off = 0;
+    Ns = 1;
+    for (i = 0; i < s->nst; i++) {
+        s->tw_off[i] = off;
+        if (Ns > 1) off += (s->radix[i] - 1) * Ns;
+        Ns *= s->radix[i];
+    }
+    s->tw_len = off;
+    return 0;
+}
+
+/* the stage twiddles of a shaped schedule, uploaded to the device */
+int va_sched_upload(VA_sched *s, int prec)
+{
+    int i, r, k, Ns, off;
     off = s->tw_len;
     if (off > 0) {
         size_t esz = (size_t)(2 * prec);
         char *host = (char *)malloc(esz * (size_t)off);
+        int rc;
         if (!host) return -3;
         Ns = 1;
         for (i = 0; i < s->nst; i++) {
@@ -208,6 +270,16 @@ int va_sched_plan(VA_sched *s, int prec, int n, vsip_length nvec)
         }
     }
     return 0;
+}
+
+/* shape plus the stage twiddles uploaded to the device */
+int va_sched_plan(VA_sched *s, int prec, int n, vsip_length nvec)
+{
+    int lds_max = 65536, rc;
+    vhip_lds_max(&lds_max);
+    rc = va_sched_shape(s, prec, n, nvec, lds_max);
+    if (rc) return rc;
+    return va_sched_upload(s, prec);
 }
 
 void va_sched_free(VA_sched *s)

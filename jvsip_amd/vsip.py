@@ -153,6 +153,11 @@ class AmdLib(VsipLib):
                                                  ctypes.c_char_p, ctypes.c_size_t]),
             "vsip_amd_fftm_describe_f": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
             "vsip_amd_fftm_describe_d": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
+            "vsip_amd_batch_create_f": (P, [c_vi, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P)]),
+            "vsip_amd_batch_run_f": (ctypes.c_int, [P]),
+            "vsip_amd_batch_destroy_f": (None, [P]),
+            "vsip_amd_batch_info_f": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
+            "vsip_amd_tune": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_char_p, ctypes.c_size_t]),
         }
         for name, (res, args) in ext.items():
             fn = getattr(d, name)
@@ -168,6 +173,24 @@ class AmdLib(VsipLib):
         import json
         buf = ctypes.create_string_buffer(2048)
         self.amd_schedule(prec, n, nvec, lds_max, buf, 2048)
+        return json.loads(buf.value.decode())
+
+    def batch(self, plans, ins, outs=None):
+        """vsip_amd_batch_create_f over Python lists of handles."""
+        n = len(plans)
+        arr = lambda xs: (P * n)(*xs)
+        return self.amd_batch_create_f(n, arr(plans), arr(ins), arr(outs) if outs else None)
+
+    def batch_info(self, b):
+        import json
+        buf = ctypes.create_string_buffer(1024)
+        self.amd_batch_info_f(b, buf, 1024)
+        return json.loads(buf.value.decode())
+
+    def tune(self, n, prec=4):
+        import json
+        buf = ctypes.create_string_buffer(1024)
+        self.amd_tune(prec, n, buf, 1024)
         return json.loads(buf.value.decode())
 
     def describe(self, plan, t="f"):
