@@ -69,6 +69,43 @@ int vsip_amd_batch_info_f(const vsip_amd_batch_f *b, char *buf, size_t len);
  * VSIP_AMD_TUNE_CACHE=<file> keeps results across processes), as JSON */
 int vsip_amd_tune(int prec, long n, char *buf, size_t len);
 
+/* ---- serving: continuous batching of arriving FFT requests ---------------
+ * A request is a rows x n complex FP32 transform (ROW, dense rows), forward
+ * (dir -1) or inverse (+1), times scale, read from the input pool block at
+ * complex element in_offset and written to the output pool block at
+ * out_offset, or (out_offset < 0) to memory the server takes from its HBM
+ * budget and returns when the request completes.  Requests must be sorted by
+ * t_arrive (seconds after the run starts).  The server admits arrived
+ * requests while the budget and its launch lanes allow, launches each group
+ * as one persistent batched kernel on a free lane (lanes overlap on the
+ * device) and retires them as they finish; res[i] gets the admission and
+ * completion times.  stats receives a JSON summary.  Returns 0 or an error
+ * code. */
+typedef struct {
+    double t_arrive;
+    long rows;
+    long n;
+    int dir;
+    float scale;
+    long in_offset;
+    long out_offset;
+} vsip_amd_request;
+
+typedef struct {
+    double t_admit;
+    double t_done;
+    int batch;
+    int stream;
+} vsip_amd_result;
+
+int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_f *in_pool,
+                     const vsip_cblock_f *out_pool, size_t hbm_budget, int lanes, int max_jobs,
+                     vsip_amd_result *res, char *stats, size_t stats_len);
+/* the same stream served by sequential plain calls (launch + synchronise per request) */
+int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_f *in_pool,
+                           const vsip_cblock_f *out_pool, size_t hbm_budget, vsip_amd_result *res,
+                           char *stats, size_t stats_len);
+
 #ifdef __cplusplus
 }
 #endif

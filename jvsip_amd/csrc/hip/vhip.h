@@ -140,6 +140,15 @@ int vhip_graph_begin(void *stream);
 int vhip_graph_end(void *stream, void **graph_exec);
 int vhip_graph_launch(void *graph_exec, void *stream);
 int vhip_graph_destroy(void *graph_exec);
+/* events, pinned host memory and async copies on a given stream */
+int vhip_event_create(void **ev);
+int vhip_event_destroy(void *ev);
+int vhip_event_record(void *ev, void *stream);
+int vhip_event_query(void *ev);          /* 0 done, 1 not yet, else error */
+int vhip_host_alloc(void **p, size_t bytes);
+int vhip_host_free(void *p);
+int vhip_memcpy_h2d_async(void *dst, const void *src, size_t bytes, void *stream);
+
 /* HBM reference kernels over `bytes` of device memory: kind 0 in-place
  * read-modify-write, kind 1 read-only (sink: 4 writable bytes) */
 int vhip_probe_launch(int kind, void *buf, size_t bytes, void *sink, void *stream);

@@ -300,3 +300,43 @@ extern "C" int vhip_probe_launch(int kind, void *buf, size_t bytes, void *sink, 
     else hipLaunchKernelGGL(probe_read, dim3(32768), dim3(256), 0, s, (const float4 *)buf, n, (float *)sink);
     return (int)hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// events, pinned host memory, async copies
+// ---------------------------------------------------------------------------
+extern "C" int vhip_event_create(void **ev) {
+    hipEvent_t e = nullptr;
+    hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    *ev = (void *)e;
+    return (int)r;
+}
+
+extern "C" int vhip_event_destroy(void *ev) {
+    return ev ? (int)hipEventDestroy((hipEvent_t)ev) : 0;
+}
+
+extern "C" int vhip_event_record(void *ev, void *stream) {
+    return (int)hipEventRecord((hipEvent_t)ev, stream ? (hipStream_t)stream : g_stream);
+}
+
+extern "C" int vhip_event_query(void *ev) {
+    hipError_t r = hipEventQuery((hipEvent_t)ev);
+    if (r == hipSuccess) return 0;
+    if (r == hipErrorNotReady) return 1;
+    return (int)r;
+}
+
+extern "C" int vhip_host_alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    return (int)hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault);
+}
+
+extern "C" int vhip_host_free(void *p) {
+    return p ? (int)hipHostFree(p) : 0;
+}
+
+extern "C" int vhip_memcpy_h2d_async(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return 0;
+    return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                               stream ? (hipStream_t)stream : g_stream);
+}

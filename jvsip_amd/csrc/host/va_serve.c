@@ -1,0 +1,557 @@
+/*
+ * va_serve.c -- a continuous-batching server for multiple-FFT requests.
+ *
+ * Requests (rows x n complex FP32 transforms, forward or inverse, with a
+ * scale) arrive over time.  The server keeps every schedule and twiddle table
+ * resident on the device (the autotuner's per-length cache) and runs this
+ * loop until every request is done:
+ *   1. requests whose arrival time has passed join a FIFO;
+ *   2. finished batches are retired: their requests get their completion
+ *      time and their output/scratch memory returns to the HBM arena;
+ *   3. while a launch lane (a stream with its own descriptor tables) is free,
+ *      requests are admitted from the FIFO head as long as their output and
+ *      scratch fit the HBM budget and the batch has room, and the lane
+ *      launches them as one persistent batched kernel.
+ * Requests that arrive while others run are admitted as lanes and memory
+ * free up; batches on different lanes overlap on the device.  Inputs are read
+ * from a device-resident input pool (the request names an offset); outputs go
+ * to a caller-given output pool offset, or to memory the server allocates
+ * from its arena (released when the request completes, as if the client
+ * consumed it at once).
+ *
+ * vsip_amd_serve_plain_f runs the same request stream as sequential plain
+ * calls (one vsip_ccfftmop_f-equivalent launch per request followed by a
+ * synchronisation), the baseline the server is compared with.
+ *
+ * C89 with POSIX clock_gettime.
+ */
+#define _POSIX_C_SOURCE 199309L
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "va_internal.h"
+#include "vsip_amd.h"
+
+static double va_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ---- HBM arena: first-fit free list over one device allocation ------------ */
+
+typedef struct {
+    size_t off, len;
+} VA_span;
+
+typedef struct {
+    char *base;
+    size_t size, used, peak;
+    VA_span *free_list;
+    int nfree, cap;
+} VA_arena;
+
+static int va_arena_init(VA_arena *a, size_t size)
+{
+    memset(a, 0, sizeof(*a));
+    a->cap = 1024;
+    a->free_list = (VA_span *)malloc(sizeof(VA_span) * (size_t)a->cap);
+    if (!a->free_list) return -1;
+    if (vhip_malloc((void **)&a->base, size)) {
+        free(a->free_list);
+        a->free_list = NULL;
+        return -1;
+    }
+    a->size = size;
+    a->free_list[0].off = 0;
+    a->free_list[0].len = size;
+    a->nfree = 1;
+    return 0;
+}
+
+static void va_arena_fini(VA_arena *a)
+{
+    if (a->base) vhip_free(a->base);
+    free(a->free_list);
+    memset(a, 0, sizeof(*a));
+}
+
+/* offset of `len` bytes (256-aligned), or (size_t)-1 when the budget is full */
+static size_t va_arena_alloc(VA_arena *a, size_t len)
+{
+    int i;
+    len = (len + 255) & ~(size_t)255;
+    for (i = 0; i < a->nfree; i++) {
+        if (a->free_list[i].len >= len) {
+            size_t off = a->free_list[i].off;
+            a->free_list[i].off += len;
+            a->free_list[i].len -= len;
+            if (a->free_list[i].len == 0) {
+                memmove(&a->free_list[i], &a->free_list[i + 1], sizeof(VA_span) * (size_t)(a->nfree - i - 1));
+                a->nfree--;
+            }
+            a->used += len;
+            if (a->used > a->peak) a->peak = a->used;
+            return off;
+        }
+    }
+    return (size_t)-1;
+}
+
+static void va_arena_free(VA_arena *a, size_t off, size_t len)
+{
+    int i = 0;
+    len = (len + 255) & ~(size_t)255;
+    a->used -= len;
+    while (i < a->nfree && a->free_list[i].off < off) i++;
+    if (a->nfree == a->cap) {
+        VA_span *t = (VA_span *)realloc(a->free_list, sizeof(VA_span) * (size_t)(2 * a->cap));
+        if (!t) return;   /* the span is lost to this run; the budget only shrinks */
+        a->free_list = t;
+        a->cap *= 2;
+    }
+    memmove(&a->free_list[i + 1], &a->free_list[i], sizeof(VA_span) * (size_t)(a->nfree - i));
+    a->free_list[i].off = off;
+    a->free_list[i].len = len;
+    a->nfree++;
+    if (i + 1 < a->nfree && a->free_list[i].off + a->free_list[i].len == a->free_list[i + 1].off) {
+        a->free_list[i].len += a->free_list[i + 1].len;
+        memmove(&a->free_list[i + 1], &a->free_list[i + 2], sizeof(VA_span) * (size_t)(a->nfree - i - 2));
+        a->nfree--;
+    }
+    if (i > 0 && a->free_list[i - 1].off + a->free_list[i - 1].len == a->free_list[i].off) {
+        a->free_list[i - 1].len += a->free_list[i].len;
+        memmove(&a->free_list[i], &a->free_list[i + 1], sizeof(VA_span) * (size_t)(a->nfree - i - 1));
+        a->nfree--;
+    }
+}
+
+/* ---- one request as launch descriptors ------------------------------------ */
+
+/* twiddle tables e^{-2 pi i m / n} for two-pass lengths, kept for the run */
+typedef struct {
+    long n;
+    void *dev;
+} VA_ptwc;
+
+typedef struct {
+    VA_ptwc tab[64];
+    int n;
+} VA_ptw_cache;
+
+static const void *va_ptw_get(VA_ptw_cache *c, long n)
+{
+    int i;
+    long k;
+    float *host;
+    void *dev = NULL;
+    for (i = 0; i < c->n; i++)
+        if (c->tab[i].n == n) return c->tab[i].dev;
+    if (c->n >= 64) return NULL;
+    host = (float *)malloc(sizeof(float) * 2 * (size_t)n);
+    if (!host) return NULL;
+    for (k = 0; k < n; k++) {
+        double a = -6.283185307179586476925286766559 * (double)k / (double)n;
+        host[2 * k] = (float)cos(a);
+        host[2 * k + 1] = (float)sin(a);
+    }
+    if (vhip_malloc(&dev, sizeof(float) * 2 * (size_t)n) ||
+        vhip_memcpy_h2d(dev, host, sizeof(float) * 2 * (size_t)n)) {
+        free(host);
+        if (dev) vhip_free(dev);
+        return NULL;
+    }
+    free(host);
+    c->tab[c->n].n = n;
+    c->tab[c->n].dev = dev;
+    c->n++;
+    return dev;
+}
+
+static void va_ptw_clear(VA_ptw_cache *c)
+{
+    int i;
+    for (i = 0; i < c->n; i++) vhip_free(c->tab[i].dev);
+    c->n = 0;
+}
+
+/* a dense rows x n matrix at device address p, as a view for va_fftm_descs */
+static void va_fake_view(VA_block *blk, VA_mview *v, void *p, long rows, long n)
+{
+    memset(blk, 0, sizeof(*blk));
+    blk->prec = 4;
+    blk->cplx = 1;
+    blk->kind = VA_LIB_BLOCK;
+    blk->admitted = 1;
+    blk->markings = VA_VALID;
+    blk->size = (vsip_length)(rows * n);
+    blk->dev_r = p;
+    memset(v, 0, sizeof(*v));
+    v->block = blk;
+    v->row_stride = 1;
+    v->row_length = (vsip_length)n;
+    v->col_stride = n;
+    v->col_length = (vsip_length)rows;
+    v->markings = VA_VALID;
+}
+
+static void va_fake_plan(VA_fftm *p, const vsip_amd_request *r)
+{
+    memset(p, 0, sizeof(*p));
+    p->prec = 4;
+    p->type = VA_CCFFTOP;
+    p->M = (vsip_length)r->rows;
+    p->N = (vsip_length)r->n;
+    p->MN = (vsip_length)r->n;
+    p->mN = (vsip_length)r->rows;
+    p->major = VSIP_ROW;
+    p->dir = r->dir;
+    p->scale = r->scale;
+    p->markings = VA_VALID;
+}
+
+/* batched-shape descriptors of request r; *work_bytes gets the two-pass scratch size */
+static int va_req_descs(const vsip_amd_request *r, void *in, void *out, void *work,
+                        VA_ptw_cache *pc, vhip_fftm_desc *d)
+{
+    const VA_sched *sA = NULL, *sB = NULL;
+    const void *ptw = NULL;
+    VA_fftm p;
+    VA_block bi, bo;
+    VA_mview vi, vo;
+    int t = va_tune_get(4, (int)r->n, &sA);
+    if (t == -2) {
+        long dd, rt = 1;
+        while ((rt + 1) * (rt + 1) <= r->n) rt++;
+        for (dd = rt + 1; dd >= 2; dd--) {
+            if (r->n % dd) continue;
+            if (va_tune_get(4, (int)(r->n / dd), &sA)) continue;
+            if (va_tune_get(4, (int)dd, &sB)) continue;
+            break;
+        }
+        if (dd < 2) return -1;
+        ptw = va_ptw_get(pc, r->n);
+        if (!ptw) return -1;
+    } else if (t) {
+        return -1;
+    }
+    va_fake_plan(&p, r);
+    va_fake_view(&bi, &vi, in, r->rows, r->n);
+    va_fake_view(&bo, &vo, out, r->rows, r->n);
+    return va_fftm_descs(&p, &vi, &vo, sA, sB, work, ptw, d);
+}
+
+static int va_req_two_pass(const vsip_amd_request *r)
+{
+    const VA_sched *s = NULL;
+    return va_tune_get(4, (int)r->n, &s) == -2;
+}
+
+/* ---- launch lanes ---------------------------------------------------------- */
+
+typedef struct {
+    void *stream, *event;
+    vhip_fftm_desc *desc_dev, *desc_host;
+    vhip_batch_item *items_dev, *items_host;
+    unsigned *ctl_dev;
+    int cap_desc;
+    long cap_items;
+    long *reqs;
+    int nreqs, busy, seq;
+} VA_lane;
+
+static int va_lane_init(VA_lane *l, int max_jobs)
+{
+    int rc;
+    memset(l, 0, sizeof(*l));
+    l->cap_desc = 2 * max_jobs;
+    l->cap_items = (long)max_jobs * 1200;
+    l->reqs = (long *)malloc(sizeof(long) * (size_t)max_jobs);
+    if (!l->reqs) return -1;
+    rc = vhip_stream_create(&l->stream);
+    if (!rc) rc = vhip_event_create(&l->event);
+    if (!rc) rc = vhip_malloc((void **)&l->desc_dev, sizeof(vhip_fftm_desc) * (size_t)l->cap_desc);
+    if (!rc) rc = vhip_malloc((void **)&l->items_dev, sizeof(vhip_batch_item) * (size_t)l->cap_items);
+    if (!rc) rc = vhip_malloc((void **)&l->ctl_dev, sizeof(unsigned) * (size_t)(l->cap_desc + 1));
+    if (!rc) rc = vhip_host_alloc((void **)&l->desc_host, sizeof(vhip_fftm_desc) * (size_t)l->cap_desc);
+    if (!rc) rc = vhip_host_alloc((void **)&l->items_host, sizeof(vhip_batch_item) * (size_t)l->cap_items);
+    return rc;
+}
+
+static void va_lane_fini(VA_lane *l)
+{
+    if (l->stream) vhip_stream_destroy(l->stream);
+    if (l->event) vhip_event_destroy(l->event);
+    if (l->desc_dev) vhip_free(l->desc_dev);
+    if (l->items_dev) vhip_free(l->items_dev);
+    if (l->ctl_dev) vhip_free(l->ctl_dev);
+    if (l->desc_host) vhip_host_free(l->desc_host);
+    if (l->items_host) vhip_host_free(l->items_host);
+    free(l->reqs);
+}
+
+typedef struct {
+    size_t out_off, work_off, out_len, work_len;
+    int own_out, own_work;
+} VA_reqmem;
+
+static void *va_pool_addr(const vsip_cblock_f *pool, long off)
+{
+    return (char *)pool->b.dev_r + (size_t)off * 8;
+}
+
+int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_f *in_pool,
+                     const vsip_cblock_f *out_pool, size_t hbm_budget, int nstreams, int max_jobs,
+                     vsip_amd_result *res, char *stats, size_t stats_len)
+{
+    VA_arena arena;
+    VA_lane *lanes = NULL;
+    VA_reqmem *mem = NULL;
+    VA_ptw_cache pc;
+    long *fifo = NULL, head = 0, tail = 0, next = 0, done = 0, i;
+    long batches = 0, jobs_launched = 0, max_inflight = 0;
+    int L, rc = 0, grid_max = 0;
+    double t0, tend;
+    if (va_ensure_init()) return -1;
+    if (!in_pool || !in_pool->b.dev_r || nreq <= 0 || nstreams < 1 || max_jobs < 1) return -1;
+    memset(&pc, 0, sizeof(pc));
+    if (va_arena_init(&arena, hbm_budget)) { va_error("serve", "HBM budget allocation failed", -1); return -1; }
+    lanes = (VA_lane *)calloc((size_t)nstreams, sizeof(VA_lane));
+    mem = (VA_reqmem *)calloc((size_t)nreq, sizeof(VA_reqmem));
+    fifo = (long *)malloc(sizeof(long) * (size_t)nreq);
+    if (!lanes || !mem || !fifo) { rc = -1; goto out; }
+    for (L = 0; L < nstreams && !rc; L++) rc = va_lane_init(&lanes[L], max_jobs);
+    if (!rc) rc = vhip_batch_max_grid(VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS, &grid_max);
+    /* warm the schedule cache for every length before the clock starts, and
+     * refuse requests that could never fit the budget */
+    for (i = 0; i < nreq && !rc; i++) {
+        const VA_sched *s;
+        size_t need = (size_t)reqs[i].rows * (size_t)reqs[i].n * 8;
+        if (va_tune_get(4, (int)reqs[i].n, &s) == -1) rc = -1;
+        if (va_req_two_pass(&reqs[i])) need *= 2;
+        if (reqs[i].out_offset >= 0 && !out_pool) rc = -1;
+        if (need + 512 > hbm_budget) rc = -1;
+    }
+    if (rc) { va_error("serve", "setup failed", rc); goto out; }
+    for (i = 0; i < nreq; i++) {
+        res[i].t_admit = res[i].t_done = -1.0;
+        res[i].batch = -1;
+        res[i].stream = -1;
+    }
+
+    t0 = va_now();
+    while (done < nreq) {
+        double now = va_now() - t0;
+        int inflight = 0;
+        while (next < nreq && reqs[next].t_arrive <= now) fifo[tail++] = next++;
+        /* retire */
+        for (L = 0; L < nstreams; L++) {
+            VA_lane *l = &lanes[L];
+            int q;
+            if (!l->busy) continue;
+            q = vhip_event_query(l->event);
+            if (q == 1) { inflight++; continue; }
+            if (q != 0) { rc = q; va_error("serve", vhip_error_string(q), q); goto out; }
+            now = va_now() - t0;
+            for (i = 0; i < l->nreqs; i++) {
+                long r = l->reqs[i];
+                res[r].t_done = now;
+                if (mem[r].own_out) va_arena_free(&arena, mem[r].out_off, mem[r].out_len);
+                if (mem[r].own_work) va_arena_free(&arena, mem[r].work_off, mem[r].work_len);
+                done++;
+            }
+            l->busy = 0;
+            l->nreqs = 0;
+        }
+        if (inflight > max_inflight) max_inflight = inflight;
+        /* admit and launch */
+        for (L = 0; L < nstreams && head < tail; L++) {
+            VA_lane *l = &lanes[L];
+            int nd = 0;
+            long ni = 0, g;
+            if (l->busy) continue;
+            while (head < tail && l->nreqs < max_jobs) {
+                long r = fifo[head];
+                const vsip_amd_request *q = &reqs[r];
+                void *in, *out, *work = NULL;
+                int k, n2;
+                long need_items = 0;
+                mem[r].out_len = (size_t)q->rows * (size_t)q->n * 8;
+                mem[r].own_out = (q->out_offset < 0);
+                mem[r].own_work = va_req_two_pass(q);
+                mem[r].work_len = mem[r].own_work ? mem[r].out_len : 0;
+                if (mem[r].own_out) {
+                    mem[r].out_off = va_arena_alloc(&arena, mem[r].out_len);
+                    if (mem[r].out_off == (size_t)-1) break;
+                }
+                if (mem[r].own_work) {
+                    mem[r].work_off = va_arena_alloc(&arena, mem[r].work_len);
+                    if (mem[r].work_off == (size_t)-1) {
+                        if (mem[r].own_out) va_arena_free(&arena, mem[r].out_off, mem[r].out_len);
+                        break;
+                    }
+                    work = arena.base + mem[r].work_off;
+                }
+                in = va_pool_addr(in_pool, q->in_offset);
+                out = mem[r].own_out ? (void *)(arena.base + mem[r].out_off) : va_pool_addr(out_pool, q->out_offset);
+                n2 = va_req_descs(q, in, out, work, &pc, &l->desc_host[nd]);
+                if (n2 < 0) { rc = -1; va_error("serve", "request has no schedule", (int)q->n); goto out; }
+                for (k = 0; k < n2; k++)
+                    need_items += (l->desc_host[nd + k].nvec + l->desc_host[nd + k].vpb - 1) / l->desc_host[nd + k].vpb;
+                if (ni + need_items > l->cap_items) {
+                    if (mem[r].own_out) va_arena_free(&arena, mem[r].out_off, mem[r].out_len);
+                    if (mem[r].own_work) va_arena_free(&arena, mem[r].work_off, mem[r].work_len);
+                    break;
+                }
+                for (k = 0; k < n2; k++) {
+                    const vhip_fftm_desc *d = &l->desc_host[nd + k];
+                    long ng = (d->nvec + d->vpb - 1) / d->vpb;
+                    for (g = 0; g < ng; g++) {
+                        vhip_batch_item *it = &l->items_host[ni++];
+                        it->desc = nd + k;
+                        it->vbase = g * d->vpb;
+                        it->wait_slot = -1;
+                        it->wait_count = 0;
+                        it->signal_slot = -1;
+                        if (n2 == 2 && k == 0) it->signal_slot = nd;
+                        if (n2 == 2 && k == 1) {
+                            it->wait_slot = nd;
+                            it->wait_count = (int)((l->desc_host[nd].nvec + l->desc_host[nd].vpb - 1) /
+                                                   l->desc_host[nd].vpb);
+                        }
+                    }
+                }
+                nd += n2;
+                res[r].t_admit = va_now() - t0;
+                res[r].batch = (int)batches;
+                res[r].stream = L;
+                l->reqs[l->nreqs++] = r;
+                head++;
+            }
+            if (l->nreqs == 0) break;   /* the FIFO head does not fit the budget yet */
+            /* A two-pass request's second-pass items must follow its first-pass
+             * items, which they do: items are appended in request order. */
+            rc = vhip_memcpy_h2d_async(l->desc_dev, l->desc_host, sizeof(vhip_fftm_desc) * (size_t)nd, l->stream);
+            if (!rc) rc = vhip_memcpy_h2d_async(l->items_dev, l->items_host, sizeof(vhip_batch_item) * (size_t)ni, l->stream);
+            if (!rc) rc = vhip_batch_launch(l->desc_dev, l->items_dev, ni, l->ctl_dev, nd, VA_BATCH_THREADS,
+                                            VA_BATCH_MAXE, VA_BATCH_LDS, (int)(ni < grid_max ? ni : grid_max),
+                                            l->stream);
+            if (!rc) rc = vhip_event_record(l->event, l->stream);
+            if (rc) { va_error("serve", vhip_error_string(rc), rc); goto out; }
+            l->busy = 1;
+            batches++;
+            jobs_launched += l->nreqs;
+        }
+    }
+    tend = va_now() - t0;
+    if (stats && stats_len) {
+        char tmp[512];
+        double bytes = 0.0;
+        for (i = 0; i < nreq; i++) bytes += 2.0 * (double)reqs[i].rows * (double)reqs[i].n * 8.0;
+        sprintf(tmp, "{\"requests\": %ld, \"seconds\": %.6f, \"req_per_s\": %.1f, \"gb_per_s\": %.2f, "
+                "\"batches\": %ld, \"jobs_per_batch\": %.2f, \"lanes\": %d, \"max_jobs\": %d, "
+                "\"max_lanes_in_flight\": %ld, \"hbm_budget\": %lu, \"hbm_peak\": %lu}",
+                nreq, tend, (double)nreq / tend, bytes / tend / 1e9, batches,
+                batches ? (double)jobs_launched / (double)batches : 0.0, nstreams, max_jobs,
+                max_inflight + 1, (unsigned long)hbm_budget, (unsigned long)arena.peak);
+        strncpy(stats, tmp, stats_len - 1);
+        stats[stats_len - 1] = '\0';
+    }
+out:
+    vhip_sync();
+    for (L = 0; lanes && L < nstreams; L++) {
+        if (lanes[L].stream) vhip_stream_destroy(lanes[L].stream);
+        lanes[L].stream = NULL;
+        va_lane_fini(&lanes[L]);
+    }
+    va_ptw_clear(&pc);
+    va_arena_fini(&arena);
+    free(lanes);
+    free(mem);
+    free(fifo);
+    return rc;
+}
+
+/* the same request stream as sequential plain calls, each synchronised */
+int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_f *in_pool,
+                           const vsip_cblock_f *out_pool, size_t hbm_budget, vsip_amd_result *res,
+                           char *stats, size_t stats_len)
+{
+    VA_arena arena;
+    VA_fftm *plans = NULL;
+    int nplans = 0, cap = 64, rc = 0;
+    long i;
+    double t0, tend;
+    if (va_ensure_init()) return -1;
+    if (!in_pool || nreq <= 0) return -1;
+    if (va_arena_init(&arena, hbm_budget)) { va_error("serve", "HBM budget allocation failed", -1); return -1; }
+    plans = (VA_fftm *)calloc((size_t)cap, sizeof(VA_fftm));
+    if (!plans) { va_arena_fini(&arena); return -1; }
+    /* plans are made before the clock starts, as a server would keep them */
+    for (i = 0; i < nreq && !rc; i++) {
+        int k, found = 0;
+        for (k = 0; k < nplans && !found; k++)
+            found = (plans[k].mN == (vsip_length)reqs[i].rows && plans[k].MN == (vsip_length)reqs[i].n &&
+                     plans[k].dir == reqs[i].dir && plans[k].scale == (double)reqs[i].scale);
+        if (found) continue;
+        if (nplans == cap) {
+            VA_fftm *t = (VA_fftm *)realloc(plans, sizeof(VA_fftm) * (size_t)(2 * cap));
+            if (!t) { rc = -1; break; }
+            plans = t;
+            cap *= 2;
+        }
+        rc = va_fftm_init(&plans[nplans], 4, VA_CCFFTOP, (vsip_length)reqs[i].rows, (vsip_length)reqs[i].n,
+                          reqs[i].scale, reqs[i].dir, VSIP_ROW, 0, VSIP_ALG_TIME);
+        if (!rc) nplans++;
+    }
+    if (rc) { va_error("serve_plain", "plan creation failed", rc); goto out; }
+    t0 = va_now();
+    for (i = 0; i < nreq && !rc; i++) {
+        const vsip_amd_request *q = &reqs[i];
+        VA_block bi, bo;
+        VA_mview vi, vo;
+        size_t len = (size_t)q->rows * (size_t)q->n * 8, off = 0;
+        void *out;
+        int k;
+        while (va_now() - t0 < q->t_arrive) { /* wait for the request to arrive */ }
+        res[i].t_admit = va_now() - t0;
+        if (q->out_offset < 0) {
+            off = va_arena_alloc(&arena, len);
+            if (off == (size_t)-1) { rc = -1; va_error("serve_plain", "HBM budget exhausted", -1); break; }
+            out = arena.base + off;
+        } else {
+            out = va_pool_addr(out_pool, q->out_offset);
+        }
+        va_fake_view(&bi, &vi, va_pool_addr(in_pool, q->in_offset), q->rows, q->n);
+        va_fake_view(&bo, &vo, out, q->rows, q->n);
+        for (k = 0; k < nplans; k++)
+            if (plans[k].mN == (vsip_length)q->rows && plans[k].MN == (vsip_length)q->n &&
+                plans[k].dir == q->dir && plans[k].scale == (double)q->scale)
+                break;
+        rc = va_fftm_exec(&plans[k], &vi, &vo, vhip_stream());
+        if (!rc) rc = vhip_sync();
+        res[i].t_done = va_now() - t0;
+        res[i].batch = -1;
+        res[i].stream = 0;
+        if (q->out_offset < 0) va_arena_free(&arena, off, len);
+    }
+    tend = va_now() - t0;
+    if (!rc && stats && stats_len) {
+        char tmp[256];
+        double bytes = 0.0;
+        for (i = 0; i < nreq; i++) bytes += 2.0 * (double)reqs[i].rows * (double)reqs[i].n * 8.0;
+        sprintf(tmp, "{\"requests\": %ld, \"seconds\": %.6f, \"req_per_s\": %.1f, \"gb_per_s\": %.2f, \"plans\": %d}",
+                nreq, tend, (double)nreq / tend, bytes / tend / 1e9, nplans);
+        strncpy(stats, tmp, stats_len - 1);
+        stats[stats_len - 1] = '\0';
+    }
+out:
+    vhip_sync();
+    for (i = 0; i < nplans; i++) va_fftm_release(&plans[i]);
+    free(plans);
+    va_arena_fini(&arena);
+    return rc;
+}

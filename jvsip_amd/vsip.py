@@ -49,6 +49,17 @@ class fftm_attr_d(ctypes.Structure):
                 ("scale", ctypes.c_double), ("dir", c_bl), ("major", c_bl)]
 
 
+class amd_request(ctypes.Structure):
+    _fields_ = [("t_arrive", ctypes.c_double), ("rows", ctypes.c_long), ("n", ctypes.c_long),
+                ("dir", ctypes.c_int), ("scale", ctypes.c_float), ("in_offset", ctypes.c_long),
+                ("out_offset", ctypes.c_long)]
+
+
+class amd_result(ctypes.Structure):
+    _fields_ = [("t_admit", ctypes.c_double), ("t_done", ctypes.c_double), ("batch", ctypes.c_int),
+                ("stream", ctypes.c_int)]
+
+
 class cmattr(ctypes.Structure):
     _fields_ = [("offset", c_vi), ("row_stride", c_st), ("row_length", c_vi),
                 ("col_stride", c_st), ("col_length", c_vi), ("block", P)]
@@ -158,6 +169,12 @@ class AmdLib(VsipLib):
             "vsip_amd_batch_destroy_f": (None, [P]),
             "vsip_amd_batch_info_f": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
             "vsip_amd_tune": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_char_p, ctypes.c_size_t]),
+            "vsip_amd_serve_f": (ctypes.c_int, [ctypes.POINTER(amd_request), ctypes.c_long, P, P, ctypes.c_size_t,
+                                                ctypes.c_int, ctypes.c_int, ctypes.POINTER(amd_result),
+                                                ctypes.c_char_p, ctypes.c_size_t]),
+            "vsip_amd_serve_plain_f": (ctypes.c_int, [ctypes.POINTER(amd_request), ctypes.c_long, P, P,
+                                                      ctypes.c_size_t, ctypes.POINTER(amd_result),
+                                                      ctypes.c_char_p, ctypes.c_size_t]),
         }
         for name, (res, args) in ext.items():
             fn = getattr(d, name)
@@ -192,6 +209,28 @@ class AmdLib(VsipLib):
         buf = ctypes.create_string_buffer(1024)
         self.amd_tune(prec, n, buf, 1024)
         return json.loads(buf.value.decode())
+
+    def serve(self, reqs, in_pool, out_pool=None, budget=64 << 30, lanes=4, max_jobs=64, plain=False):
+        """Run a request stream (list of dicts or tuples in amd_request field
+        order) through the server (or the plain baseline); returns
+        (results as a numpy structured array, stats dict)."""
+        import json
+        import numpy as np
+        n = len(reqs)
+        arr = (amd_request * n)()
+        for i, r in enumerate(reqs):
+            arr[i] = amd_request(*r) if not isinstance(r, dict) else amd_request(**r)
+        res = (amd_result * n)()
+        buf = ctypes.create_string_buffer(1024)
+        if plain:
+            rc = self.amd_serve_plain_f(arr, n, in_pool, out_pool, budget, res, buf, 1024)
+        else:
+            rc = self.amd_serve_f(arr, n, in_pool, out_pool, budget, lanes, max_jobs, res, buf, 1024)
+        if rc:
+            raise RuntimeError("serve failed (%d): %s" % (rc, self.last_error()))
+        out = np.array([(r.t_admit, r.t_done, r.batch, r.stream) for r in res],
+                       dtype=[("t_admit", "f8"), ("t_done", "f8"), ("batch", "i4"), ("stream", "i4")])
+        return out, json.loads(buf.value.decode() or "{}")
 
     def describe(self, plan, t="f"):
         buf = ctypes.create_string_buffer(2048)

@@ -1,0 +1,127 @@
+"""GPU tests of the continuous-batching server (vsip_amd_serve_f) through the C ABI.
+
+A request stream with the serving workload's lengths {256, 1000, 1024, 2187,
+4096, 16384}, mixed directions and scales arrives over time; every request
+reads its input from a device pool and writes its own region of an output
+pool.  Each result must be the same when the request is served alone as in
+the mix (rel L2 <= 1e-6; the same schedule runs it, so in practice the bits
+agree), agree with the plain sequential calls (1e-6) and with the
+double-precision oracle (1e-5).  A tight HBM budget forces admission to wait
+for memory, and must still complete every request with the same results.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from jvsip_amd import datagen, fftm
+from jvsip_amd.vsip import VSIP_ROW
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+LENGTHS = [256, 1000, 1024, 2187, 4096, 16384]
+POOL = 1 << 21            # complex elements in the input pool
+
+
+def _case(name):
+    print("\n[case] %s" % name, flush=True)
+    sys.stdout.flush()
+
+
+def _requests(nreq, rate, seed, max_rows):
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    reqs, out_off = [], 0
+    for i in range(nreq):
+        n = int(rng.choice(LENGTHS))
+        rows = int(rng.integers(1, max_rows + 1))
+        d = -1 if rng.random() < 0.5 else 1
+        scale = 1.0 if d < 0 else 1.0 / n
+        in_off = int(rng.integers(0, POOL - rows * n))
+        reqs.append((float(t[i]), rows, n, d, scale, in_off, out_off))
+        out_off += rows * n
+    return reqs, out_off
+
+
+@pytest.fixture(scope="module")
+def pools(amd):
+    lib = amd
+    x = datagen.complex_data(5, (POOL,))
+    inb = lib.cblockcreate_f(POOL, 0)
+    inv = lib.cmbind_f(inb, 0, POOL, 1, 1, POOL)
+    lib.cmcopyfrom_user_f(x.ctypes.data, None, VSIP_ROW, inv)
+    yield x, inb
+    lib.cmalldestroy_f(inv)
+
+
+def _out_pool(lib, total):
+    b = lib.cblockcreate_f(total, 0)
+    v = lib.cmbind_f(b, 0, total, 1, 1, total)
+    return b, v
+
+
+def _read(lib, v, total):
+    out = np.empty(total, np.complex64)
+    lib.cmcopyto_user_f(v, VSIP_ROW, out.ctypes.data, None)
+    return out
+
+
+def test_serve_mix_equals_alone_plain_and_oracle(amd, pools):
+    lib = amd
+    x, inb = pools
+    reqs, total = _requests(48, rate=20000.0, seed=5, max_rows=48)
+    ob, ov = _out_pool(lib, total)
+    _case("serve mix: %d requests" % len(reqs))
+    res, stats = lib.serve(reqs, inb, ob, budget=1 << 30, lanes=4, max_jobs=16)
+    print("  stats", stats, flush=True)
+    assert np.all(res["t_done"] >= res["t_admit"]) and np.all(res["t_admit"] >= 0)
+    mix = _read(lib, ov, total)
+
+    _case("serve each request alone")
+    alone = np.empty_like(mix)
+    for r in reqs:
+        lib.serve([(0.0,) + tuple(r[1:])], inb, ob, budget=1 << 28, lanes=1, max_jobs=1)
+    alone[:] = _read(lib, ov, total)
+
+    _case("plain sequential calls")
+    res_p, stats_p = lib.serve(reqs, inb, ob, budget=1 << 30, plain=True)
+    print("  plain stats", stats_p, flush=True)
+    plain = _read(lib, ov, total)
+
+    for i, (t, rows, n, d, s, io, oo) in enumerate(reqs):
+        sl = slice(oo, oo + rows * n)
+        want = po.ccfftm(x[io:io + rows * n], fftm.dense_view(rows, n), direction=d, scale=s)
+        e_alone = po.rel_l2(mix[sl], alone[sl])
+        e_plain = po.rel_l2(mix[sl], plain[sl])
+        e_or = po.rel_l2(mix[sl], want)
+        print("  req %2d %4dx%-6d dir %+d: alone %.1e plain %.1e oracle %.1e"
+              % (i, rows, n, d, e_alone, e_plain, e_or), flush=True)
+        assert e_alone <= 1e-6 and e_plain <= 1e-6 and e_or <= 1e-5, i
+    lib.cmalldestroy_f(ov)
+
+
+def test_serve_budget_admission(amd, pools):
+    """A budget smaller than the stream's in-flight demand: admission waits for
+    memory, every request still completes with the right result."""
+    lib = amd
+    x, inb = pools
+    reqs, _ = _requests(40, rate=1e6, seed=7, max_rows=32)
+    reqs = [r[:6] + (-1,) for r in reqs]          # outputs from the server's budget
+    biggest = max(2 * r[1] * r[2] * 8 for r in reqs)
+    budget = 2 * biggest
+    _case("serve with a %d-byte budget" % budget)
+    res, stats = lib.serve(reqs, inb, None, budget=budget, lanes=4, max_jobs=8)
+    print("  stats", stats, flush=True)
+    assert stats["hbm_peak"] <= budget
+    assert np.all(res["t_done"] > 0)
+    assert stats["batches"] > 1
+
+
+def test_serve_error_paths(amd, pools):
+    lib = amd
+    x, inb = pools
+    with pytest.raises(RuntimeError):   # a request that can never fit the budget
+        lib.serve([(0.0, 4, 4096, -1, 1.0, 0, -1)], inb, None, budget=1 << 16)
+    with pytest.raises(RuntimeError):   # no butterfly for 17
+        lib.serve([(0.0, 4, 17, -1, 1.0, 0, -1)], inb, None, budget=1 << 26)
