@@ -142,6 +142,290 @@ def accuracy(lib, view_fn):
             "rows": [int(r) for r in rows]}
 
 
+
+# ---------------------------------------------------------------------------
+# the other configurations of the workload set, each timed on the device and
+# checked for accuracy
+# ---------------------------------------------------------------------------
+
+def _dev_view(lib, x, major=0):
+    """device matrix (vsip_cmcreate_f) holding host array x (ROW-ordered data)"""
+    m, n = x.shape
+    v = lib.cmcreate_f(m, n, major, 0)
+    assert v, lib.last_error()
+    xx = np.ascontiguousarray(x, dtype=np.complex64)
+    lib.cmcopyfrom_user_f(xx.ctypes.data, None, 0, v)
+    return v
+
+
+def _time_calls(lib, hip, fn, reps):
+    """wall and stream-event time per call of fn() over reps back-to-back calls"""
+    stream = lib.amd_stream()
+    fn()
+    lib.amd_sync()
+    e0, e1 = hip.event(), hip.event()
+    t0 = time.perf_counter()
+    hip.record(e0, stream)
+    for _ in range(reps):
+        fn()
+    hip.record(e1, stream)
+    lib.amd_sync()
+    wall = (time.perf_counter() - t0) / reps
+    return wall, hip.elapsed_ms(e0, e1) * 1e-3 / reps
+
+
+def config_c1(lib, hip):
+    """plain vsip_ccfftmip_f forward, ROW, 64 x 1024, scale 1, N(0,1) seed 1"""
+    from jvsip_amd import datagen, fftm
+    from oracle import pyoracle as po
+    x = datagen.complex_data(1, (64, 1024))
+    v = _dev_view(lib, x)
+    p = lib.ccfftmip_create_f(64, 1024, 1.0, -1, 0, 0, 1)
+    wall, gpu = _time_calls(lib, hip, lambda: lib.ccfftmip_f(p, v), 200)
+    # one call from a fresh copy, with synchronisation, for the latency figure
+    lib.cmcopyfrom_user_f(np.ascontiguousarray(x).ctypes.data, None, 0, v)
+    t0 = time.perf_counter()
+    lib.ccfftmip_f(p, v)
+    lib.amd_sync()
+    lat = time.perf_counter() - t0
+    y = np.empty((64, 1024), np.complex64)
+    lib.cmcopyto_user_f(v, 0, y.ctypes.data, None)
+    f = np.load(os.path.join(ROOT, "tests", "golden", "jvsip_c1_64x1024.npz"))
+    out = {"shape": "64x1024", "us_per_call_back_to_back": round(wall * 1e6, 2),
+           "us_per_call_kernel_stream": round(gpu * 1e6, 2), "us_call_plus_sync": round(lat * 1e6, 2),
+           "gb_per_s": round(2 * 64 * 1024 * 8 / gpu / 1e9, 1),
+           "rel_l2_vs_jvsip": po.rel_l2(y, f["out"]),
+           "rel_l2_vs_fp64_oracle": po.rel_l2(y, po.ccfftm(x.ravel(), fftm.dense_view(64, 1024)))}
+    lib.fftm_destroy_f(p)
+    lib.cmalldestroy_f(v)
+    return out
+
+
+def config_c2_real(lib, hip):
+    """vsip_rcfftmop_f 8192 x 2048 real N(0,1) then vsip_crfftmop_f (scale 1/2048) back"""
+    from jvsip_amd import datagen
+    from oracle import pyoracle as po
+    M, N = 8192, 2048
+    x = datagen.real_data(2, (M, N))
+    xv = lib.mcreate_f(M, N, 0, 0)
+    lib.mcopyfrom_user_f(np.ascontiguousarray(x).ctypes.data, 0, xv)
+    zv = lib.cmcreate_f(M, N // 2 + 1, 0, 0)
+    bv = lib.mcreate_f(M, N, 0, 0)
+    rc = lib.rcfftmop_create_f(M, N, 1.0, 0, 0, 1)
+    cr = lib.crfftmop_create_f(M, N, 1.0 / N, 0, 0, 1)
+    _, t_rc = _time_calls(lib, hip, lambda: lib.rcfftmop_f(rc, xv, zv), 20)
+    _, t_cr = _time_calls(lib, hip, lambda: lib.crfftmop_f(cr, zv, bv), 20)
+    lib.rcfftmop_f(rc, xv, zv)
+    lib.crfftmop_f(cr, zv, bv)
+    back = np.empty((M, N), np.float32)
+    lib.mcopyto_user_f(bv, 0, back.ctypes.data)
+    z = np.empty((M, N // 2 + 1), np.complex64)
+    lib.cmcopyto_user_f(zv, 0, z.ctypes.data, None)
+    rows = np.arange(0, M, 1024)
+    zz = np.fft.rfft(x[rows].astype(np.float64), axis=1)
+    bytes_ = M * N * 4 + M * (N // 2 + 1) * 8
+    out = {"shape": "8192x2048 real", "rc_us": round(t_rc * 1e6, 2), "cr_us": round(t_cr * 1e6, 2),
+           "rc_gb_per_s": round(bytes_ / t_rc / 1e9, 1), "cr_gb_per_s": round(bytes_ / t_cr / 1e9, 1),
+           "rc_rel_l2_vs_fp64": po.rel_l2(z[rows], zz), "round_trip_rel_l2": po.rel_l2(back, x)}
+    for h in (rc, cr):
+        lib.fftm_destroy_f(h)
+    for v in (xv, bv):
+        lib.malldestroy_f(v)
+    lib.cmalldestroy_f(zv)
+    return out
+
+
+def config_c3(lib, hip):
+    """strided / mixed-radix ccfftmop_f (parent rows 2x the row length) and the FP64 2048^2 case"""
+    from jvsip_amd import datagen, fftm
+    from oracle import pyoracle as po
+    out = {}
+    cases = [("col_1920x1000", "jvsip_c3_col_1920x1000.npz", 1920, 1000, 1, 2000),
+             ("row_1024x2187", "jvsip_c3_row_1024x2187.npz", 1024, 2187, 0, 4374),
+             ("row_512x1001", "jvsip_c3_row_512x1001.npz", 512, 1001, 0, 2002)]
+    for name, fx, m, n, major, parent in cases:
+        f = np.load(os.path.join(ROOT, "tests", "golden", fx))
+        x = datagen.complex_data(3, (m * parent,))
+        blk = lib.cblockcreate_f(m * parent, 0)
+        pv = lib.cmbind_f(blk, 0, parent, m, 1, parent)
+        lib.cmcopyfrom_user_f(x.ctypes.data, None, 0, pv)
+        xv = lib.cmsubview_f(pv, 0, 0, m, n)              # left half of the parent
+        yv = lib.cmcreate_f(m, n, 0, 0)
+        p = lib.ccfftmop_create_f(m, n, 1.0, -1, major, 0, 1)
+        _, t = _time_calls(lib, hip, lambda: lib.ccfftmop_f(p, xv, yv), 20)
+        y = np.empty((m, n), np.complex64)
+        lib.cmcopyto_user_f(yv, 0, y.ctypes.data, None)
+        vecs = y if major == 0 else y.T
+        v = fftm.View(0, parent, m, 1, n)
+        xs = fftm.gather(x, v)
+        sel = f["rows"]
+        want = po.ccfftm((xs[sel] if major == 0 else xs[:, sel].T).ravel(), fftm.dense_view(len(sel), xs.shape[1] if major == 0 else m))
+        out[name] = {"us": round(t * 1e6, 2), "gb_per_s": round(2 * m * n * 8 / t / 1e9, 1),
+                     "rel_l2_vs_jvsip": po.rel_l2(vecs[sel], f["out_rows"]),
+                     "rel_l2_vs_fp64_oracle": po.rel_l2(vecs[sel], want),
+                     "schedule": json.loads(lib.describe(p))["passes"][0]["radix"]}
+        lib.fftm_destroy_f(p)
+        lib.cmalldestroy_f(yv)
+        lib.cmdestroy_f(xv)
+        lib.cmalldestroy_f(pv)
+    # FP64: vsip_ccfftmip_d 2048 x 2048
+    xd = datagen.complex_data(3, (2048, 2048), dtype=np.complex128)
+    vd = lib.cmcreate_d(2048, 2048, 0, 0)
+    lib.cmcopyfrom_user_d(np.ascontiguousarray(xd).ctypes.data, None, 0, vd)
+    pd = lib.ccfftmip_create_d(2048, 2048, 1.0, -1, 0, 0, 1)
+    pinv = lib.ccfftmip_create_d(2048, 2048, 1.0 / 2048, 1, 0, 0, 1)
+    def fwdinv():
+        lib.ccfftmip_d(pd, vd)
+        lib.ccfftmip_d(pinv, vd)
+    _, t = _time_calls(lib, hip, fwdinv, 10)
+    lib.cmcopyfrom_user_d(np.ascontiguousarray(xd).ctypes.data, None, 0, vd)
+    lib.ccfftmip_d(pd, vd)
+    yd = np.empty((2048, 2048), np.complex128)
+    lib.cmcopyto_user_d(vd, 0, yd.ctypes.data, None)
+    rows = np.arange(0, 2048, 256)
+    want = po.ccfftm(xd[rows].ravel(), fftm.dense_view(len(rows), 2048)).reshape(len(rows), 2048)
+    out["fp64_2048x2048"] = {"us_per_transform": round(t / 2 * 1e6, 2),
+                             "gb_per_s": round(2 * 2048 * 2048 * 16 / (t / 2) / 1e9, 1),
+                             "rel_l2_vs_fp64_oracle": po.rel_l2(yd[rows], want)}
+    lib.fftm_destroy_d(pd)
+    lib.fftm_destroy_d(pinv)
+    lib.cmalldestroy_d(vd)
+    return out
+
+
+def _c4_lengths(count, seed):
+    """lengths log-uniform over {2^a 3^b 5^c} in [64, 65536]"""
+    cands = sorted({2 ** a * 3 ** b * 5 ** c for a in range(17) for b in range(11) for c in range(7)
+                    if 64 <= 2 ** a * 3 ** b * 5 ** c <= 65536})
+    logc = np.log(np.array(cands, dtype=np.float64))
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(np.log(64), np.log(65536), count)
+    return [cands[int(np.argmin(np.abs(logc - v)))] for v in u]
+
+
+def config_c4(lib, hip, njobs=3000):
+    """njobs FFTs of log-uniform 2^a 3^b 5^c lengths in [64, 65536], rows so the
+    total is ~2^27 complex elements: one batched graph launch vs plain per-call"""
+    from jvsip_amd import datagen, fftm
+    from oracle import pyoracle as po
+    lengths = _c4_lengths(njobs, 4)
+    per_job = (1 << 27) / njobs
+    rows = [max(1, int(round(per_job / n))) for n in lengths]
+    total = int(sum(r * n for r, n in zip(rows, lengths)))
+    pool = datagen.complex_data(4, (total,))
+    blk = lib.cblockcreate_f(total, 0)
+    whole = lib.cmbind_f(blk, 0, total, 1, 1, total)
+    lib.cmcopyfrom_user_f(pool.ctypes.data, None, 0, whole)
+    ref_blk = lib.cblockcreate_f(total, 0)
+    ref_whole = lib.cmbind_f(ref_blk, 0, total, 1, 1, total)
+    views, rviews, plans, off = [], [], [], 0
+    for r, n in zip(rows, lengths):
+        views.append(lib.cmbind_f(blk, off, n, r, 1, n))
+        rviews.append(lib.cmbind_f(ref_blk, off, n, r, 1, n))
+        p = lib.ccfftmip_create_f(r, n, 1.0, -1, 0, 0, 1)
+        assert p, lib.last_error()
+        plans.append(p)
+        off += r * n
+    t0 = time.perf_counter()
+    b = lib.batch(plans, views)
+    t_create = time.perf_counter() - t0
+    assert b, lib.last_error()
+    info = lib.batch_info(b)
+    # batched: graph replays (forward in place; the data just keeps transforming)
+    wall_b, gpu_b = _time_calls(lib, hip, lambda: lib.amd_batch_run_f(b), 10)
+    # plain per-call, back to back (no synchronisation between calls)
+    def plain_all():
+        for p, v in zip(plans, views):
+            lib.ccfftmip_f(p, v)
+    wall_p, gpu_p = _time_calls(lib, hip, plain_all, 2)
+    # plain per-call with a synchronisation after each call (sequential calls)
+    t0 = time.perf_counter()
+    for p, v in zip(plans, views):
+        lib.ccfftmip_f(p, v)
+        lib.amd_sync()
+    wall_ps = time.perf_counter() - t0
+    # accuracy: batched vs plain on the same fresh input, and a sample vs the oracle
+    lib.cmcopyfrom_user_f(pool.ctypes.data, None, 0, whole)
+    lib.cmcopyfrom_user_f(pool.ctypes.data, None, 0, ref_whole)
+    lib.amd_batch_run_f(b)
+    for p, v in zip(plans, rviews):
+        lib.ccfftmip_f(p, v)
+    got = np.empty(total, np.complex64)
+    want = np.empty(total, np.complex64)
+    lib.cmcopyto_user_f(whole, 0, got.ctypes.data, None)
+    lib.cmcopyto_user_f(ref_whole, 0, want.ctypes.data, None)
+    errs, off = [], 0
+    sample = set(range(0, njobs, max(1, njobs // 24)))
+    worst_plain = 0.0
+    for j, (r, n) in enumerate(zip(rows, lengths)):
+        sl = slice(off, off + r * n)
+        worst_plain = max(worst_plain, po.rel_l2(got[sl], want[sl]))
+        if j in sample:
+            errs.append(po.rel_l2(got[sl], po.ccfftm(pool[sl], fftm.dense_view(r, n))))
+        off += r * n
+    bytes_ = 2 * total * 8
+    out = {"jobs": njobs, "elements": total, "distinct_lengths": len(set(lengths)), "batch": info,
+           "batch_create_s": round(t_create, 3),
+           "batched_ms": round(gpu_b * 1e3, 3), "batched_gb_per_s": round(bytes_ / gpu_b / 1e9, 1),
+           "plain_back_to_back_ms": round(wall_p * 1e3, 3),
+           "plain_sync_each_ms": round(wall_ps * 1e3, 3),
+           "speedup_vs_plain_back_to_back": round(wall_p / wall_b, 2),
+           "speedup_vs_plain_sync_each": round(wall_ps / wall_b, 2),
+           "max_rel_l2_batched_vs_plain": worst_plain,
+           "max_rel_l2_vs_fp64_oracle_sample": max(errs)}
+    lib.amd_batch_destroy_f(b)
+    for p in plans:
+        lib.fftm_destroy_f(p)
+    for v in views + rviews:
+        lib.cmdestroy_f(v)
+    lib.cmalldestroy_f(whole)
+    lib.cmalldestroy_f(ref_whole)
+    return out
+
+
+def config_c5(lib, nreq=2000, rate=20000.0):
+    """Poisson requests at `rate`/s: rows U[1,512], lengths {256, 1000, 1024,
+    2187, 4096, 16384}, fwd/inv 50/50, N(0,1) inputs from a device pool,
+    64 GB HBM budget; served (continuous batching) vs sequential plain calls"""
+    from jvsip_amd import datagen
+    lengths = [256, 1000, 1024, 2187, 4096, 16384]
+    pool_n = 512 * 16384 * 2
+    x = datagen.complex_data(5, (pool_n,))
+    blk = lib.cblockcreate_f(pool_n, 0)
+    pv = lib.cmbind_f(blk, 0, pool_n, 1, 1, pool_n)
+    lib.cmcopyfrom_user_f(x.ctypes.data, None, 0, pv)
+    rng = np.random.default_rng(5)
+    t = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    reqs = []
+    for i in range(nreq):
+        n = int(rng.choice(lengths))
+        r = int(rng.integers(1, 513))
+        d = -1 if rng.random() < 0.5 else 1
+        reqs.append((float(t[i]), r, n, d, 1.0 if d < 0 else 1.0 / n, int(rng.integers(0, pool_n - r * n)), -1))
+    budget = 64 << 30
+    out = {"requests": nreq, "rate_per_s": rate, "hbm_budget_bytes": budget}
+
+    def lat(res):
+        l = (res["t_done"] - np.array([q[0] for q in reqs])) * 1e6
+        return {"p50_us": round(float(np.percentile(l, 50)), 1), "p99_us": round(float(np.percentile(l, 99)), 1),
+                "mean_us": round(float(l.mean()), 1)}
+
+    lib.serve(reqs[:50], blk, None, budget=1 << 30)            # warm the schedules
+    res_s, st_s = lib.serve(reqs, blk, None, budget=budget, lanes=4, max_jobs=64)
+    res_p, st_p = lib.serve(reqs, blk, None, budget=budget, plain=True)
+    out["served"] = dict(lat(res_s), **st_s)
+    out["plain"] = dict(lat(res_p), **st_p)
+    # saturation: every request present at t = 0
+    sat = [(0.0,) + q[1:] for q in reqs]
+    _, sat_s = lib.serve(sat, blk, None, budget=budget, lanes=4, max_jobs=64)
+    _, sat_p = lib.serve(sat, blk, None, budget=budget, plain=True)
+    out["saturation"] = {"served_req_per_s": sat_s["req_per_s"], "plain_req_per_s": sat_p["req_per_s"],
+                         "served_gb_per_s": sat_s["gb_per_s"], "plain_gb_per_s": sat_p["gb_per_s"],
+                         "speedup": round(sat_s["req_per_s"] / sat_p["req_per_s"], 2)}
+    lib.cmalldestroy_f(pv)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +433,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-configs", action="store_true", help="skip the other workload configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -210,11 +495,27 @@ def main():
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
+    # the device's own in-place read+write and read-only HBM rates (4 x 128 MiB)
+    rmw, rd = ctypes.c_double(), ctypes.c_double()
+    assert lib.amd_hbm_probe(BYTES_PER_CALL // 2, 20, ctypes.byref(rmw), ctypes.byref(rd)) == 0
     acc = accuracy(lib, None)
     for v in views:
         lib.cmalldestroy_f(v)
     lib.fftm_destroy_f(fwd)
     lib.fftm_destroy_f(inv)
+
+    configs = None
+    if not args.no_configs:
+        configs = {}
+        for name, fn in (("c1_plain_64x1024", lambda: config_c1(lib, hip)),
+                         ("c2_real_8192x2048", lambda: config_c2_real(lib, hip)),
+                         ("c3_strided_mixed_fp64", lambda: config_c3(lib, hip)),
+                         ("c4_batched_uneven", lambda: config_c4(lib, hip)),
+                         ("c5_served", lambda: config_c5(lib))):
+            t0 = time.perf_counter()
+            configs[name] = fn()
+            configs[name]["bench_seconds"] = round(time.perf_counter() - t0, 2)
+            print("bench: %s done in %.1f s" % (name, time.perf_counter() - t0), file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -243,10 +544,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel_avg_us": round(avg_launch_ms * 1e3, 3),
-                     "algorithmic_bytes_per_launch": BYTES_PER_CALL},
+                     "algorithmic_bytes_per_launch": BYTES_PER_CALL,
+                     "measured_inplace_rmw_gb_per_s": round(rmw.value, 1),
+                     "measured_read_only_gb_per_s": round(rd.value, 1),
+                     "frac_of_measured_inplace_rmw": round(achieved / rmw.value, 4) if rmw.value else None},
         "cpu_baseline": cpu,
         "accuracy": acc,
         "schedule": json.loads(schedule),
+        "configs": configs,
     }
     print(json.dumps(out))
 

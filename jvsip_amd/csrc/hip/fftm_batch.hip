@@ -1,4 +1,4 @@
-// fftm_batch.hip -- one persistent launch for many multiple-FFTs of uneven
+// fftm_batch.hip -- persistent launches that run many multiple-FFTs of uneven
 // lengths and radices.
 //
 // The host lays the work out as a list of items (a group of vectors of one
@@ -8,15 +8,17 @@
 // that transform, and goes back for more until the list is empty, so the
 // grid never waits for the slowest transform of a batch.
 //
-// Lengths longer than one workgroup's LDS run as two passes (n = n1 * n2);
-// the second-pass items of a transform come after all first-pass items in
-// the list and wait on a per-transform counter that the first-pass items
-// release.  A workgroup only waits after it took a second-pass item, which
-// happens only once every first-pass item has been taken by a running
-// workgroup, so the wait always ends.  The counter handoff uses agent-scope
-// release/acquire atomics, which write back and invalidate the per-XCD L2
-// so the second pass sees the first pass's stores from any XCD.
-#include "fftm_core.hpp"
+// Power-of-two lengths from 256 to 8192 on contiguous rows run the
+// specialised power-of-two stages (8192 / n rows per item); the rest run the
+// generic body.  Each workgroup fetches its next item index while it
+// transforms the current one.
+//
+// Lengths longer than one workgroup's LDS run as two passes (n = n1 * n2).
+// Their second-pass items go to a second list drained by a second persistent
+// launch on the same stream: the kernel boundary makes the first pass's
+// stores visible to every XCD, where an in-kernel handoff would need an
+// agent-scope release (a write-back of the whole L2 of the XCD) per item.
+#include "fftm_pow2.hpp"
 
 namespace vhipk {
 namespace {
@@ -25,41 +27,51 @@ __global__ void batch_reset(unsigned *ctl, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctl[i] = 0u;
 }
 
+__device__ __forceinline__ unsigned next_index(unsigned *head) {
+    return __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ctl[0]: next item; ctl[1]: workgroups finished.  The last workgroup to
+// finish clears both (every workgroup has made its final fetch by then), so
+// the next launch on the stream starts from zero without a clearing kernel.
 template <int MAXE>
 __global__ void __launch_bounds__(512)
     batch_kernel(const vhip_fftm_desc *descs, const vhip_batch_item *items, long nitems,
                  unsigned *ctl) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ long s_item;
+    if (threadIdx.x == 0) s_item = (long)next_index(&ctl[0]);
     for (;;) {
-        __syncthreads();  // the previous item is done with LDS and s_item
-        if (threadIdx.x == 0)
-            s_item = (long)__hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
+        __syncthreads();  // s_item is set; the previous item is done with LDS
         const long it = s_item;
         if (it >= nitems) break;
+        long nxt = 0;
+        if (threadIdx.x == 0) nxt = (long)next_index(&ctl[0]);   // overlaps the item below
         const vhip_batch_item item = items[it];
         const int di = __builtin_amdgcn_readfirstlane(item.desc);
         const vhip_fftm_desc &d = descs[di];
-        if (item.wait_slot >= 0) {
-            if (threadIdx.x == 0) {
-                unsigned *c = &ctl[1 + item.wait_slot];
-                while (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
-                       (unsigned)item.wait_count)
-                    __builtin_amdgcn_s_sleep(2);
-            }
-            __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every wave drops stale lines
+        const int l2 = __builtin_amdgcn_readfirstlane(p2::pow2_batch_log2(d));
+        switch (l2) {
+        case 8: p2::pow2_batch_item<8>(d, item.vbase, smem); break;
+        case 9: p2::pow2_batch_item<9>(d, item.vbase, smem); break;
+        case 10: p2::pow2_batch_item<10>(d, item.vbase, smem); break;
+        case 11: p2::pow2_batch_item<11>(d, item.vbase, smem); break;
+        case 12: p2::pow2_batch_item<12>(d, item.vbase, smem); break;
+        case 13: p2::pow2_batch_item<13>(d, item.vbase, smem); break;
+        default: {
+            const long left = d.nvec - item.vbase;
+            const int nv = left < d.vpb ? (int)left : d.vpb;
+            fft_body<float, MAXE, VHIP_C2C>(d, item.vbase, nv, smem);
         }
-        const long left = d.nvec - item.vbase;
-        const int nv = left < d.vpb ? (int)left : d.vpb;
-        fft_body<float, MAXE, VHIP_C2C>(d, item.vbase, nv, smem);
-        if (item.signal_slot >= 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // every wave's stores are visible
-            __syncthreads();
-            if (threadIdx.x == 0)
-                __hip_atomic_fetch_add(&ctl[1 + item.signal_slot], 1u, __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();  // every lane has read s_item
+        if (threadIdx.x == 0) s_item = nxt;
+    }
+    if (threadIdx.x == 0) {
+        const unsigned fin = __hip_atomic_fetch_add(&ctl[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (fin == gridDim.x - 1) {
+            __hip_atomic_store(&ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -97,15 +109,25 @@ extern "C" int vhip_batch_max_grid(int threads, int maxe, int lds_bytes, int *gr
     return 0;
 }
 
-extern "C" int vhip_batch_launch(const vhip_fftm_desc *descs, const vhip_batch_item *items, long nitems,
-                                 unsigned *ctl, int nslots, int threads, int maxe, int lds_bytes,
+extern "C" int vhip_batch_clear(unsigned *ctl, void *stream) {
+    hipLaunchKernelGGL(vhipk::batch_reset, dim3(1), dim3(64), 0, (hipStream_t)stream, ctl, 4);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vhip_batch_launch(const vhip_fftm_desc *descs, const vhip_batch_item *items, long nitems1,
+                                 long nitems2, unsigned *ctl, int threads, int maxe, int lds_bytes,
                                  int grid, void *stream) {
-    if (threads != 512 || grid < 1 || nitems < 0) return -1;
+    if (threads != 512 || grid < 1 || nitems1 < 0 || nitems2 < 0) return -1;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(vhipk::batch_reset, dim3(1), dim3(256), 0, s, ctl, nslots + 1);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL(vhipk::pick_batch(maxe), dim3(grid), dim3(threads), (size_t)lds_bytes, s,
-                       descs, items, nitems, ctl);
+    hipError_t e = hipSuccess;
+    long g1 = nitems1 < grid ? nitems1 : grid;
+    if (g1 > 0)
+        hipLaunchKernelGGL(vhipk::pick_batch(maxe), dim3((unsigned)g1), dim3(threads), (size_t)lds_bytes, s,
+                           descs, items, nitems1, ctl);
+    e = hipGetLastError();
+    if (e != hipSuccess || nitems2 == 0) return (int)e;
+    long g2 = nitems2 < grid ? nitems2 : grid;
+    hipLaunchKernelGGL(vhipk::pick_batch(maxe), dim3((unsigned)g2), dim3(threads), (size_t)lds_bytes, s,
+                       descs, items + nitems1, nitems2, ctl + 2);
     return (int)hipGetLastError();
 }

@@ -1,0 +1,227 @@
+// fftm_pow2.hpp -- specialised multiple-FFT kernel for power-of-two row
+// transforms of contiguous interleaved complex FP32 data (the common case and
+// the 4096 x 4096 headline).
+//
+// Same Stockham schedule as the generic kernel (radices 16/8/4/2 with the
+// power-of-two bits split evenly over ceil(log2 N / 4) stages, stage twiddles
+// W_{Ns R}^{r k} from the plan's table), but with the length, the radices and
+// the vectors per workgroup fixed at compile time:
+//   - every thread holds 16 elements in every stage (16/R butterflies of
+//     radix R) and all index arithmetic reduces to shifts and masks;
+//   - HBM and twiddle accesses are buffer loads/stores with one VGPR offset per
+//     butterfly and the leg offsets r * (n/R) * 8 in SGPRs, so the 16 loads of
+//     a thread need one address register (the buffer range also bounds every
+//     access to the rows of the current group);
+//   - LDS accesses are a per-thread base plus immediate offsets: the padded
+//     index i + (i >> 5) splits over the disjoint bit fields of the butterfly
+//     index and the leg;
+//   - a workgroup runs `per_wg` consecutive groups of VPB rows (persistent) and
+//     may issue the HBM loads of its next group before transforming the
+//     current one.
+#pragma once
+#include <cstdlib>
+#include "fftm_core.hpp"
+
+namespace vhipk {
+namespace p2 {
+
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+typedef decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc_t(), 0, 0, 0)) raw64_t;
+
+// word 3 of a raw buffer descriptor on gfx950 (32-bit data format, no swizzle)
+constexpr int kBufWord3 = 0x00020000;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kBufWord3);
+}
+__device__ __forceinline__ cpx<float> bload(rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore(rsrc_t r, int voff, int soff, cpx<float> x) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(raw64_t, x), r, voff, soff, 0);
+}
+
+template <int LOG2N>
+struct P2 {
+    static constexpr int N = 1 << LOG2N;
+    static constexpr int NST = (LOG2N + 3) / 4;
+    static constexpr int bits(int s) { return LOG2N / NST + (s < LOG2N % NST ? 1 : 0); }
+    static constexpr int log2ns(int s) { return s == 0 ? 0 : log2ns(s - 1) + bits(s - 1); }
+    // padded LDS vector stride (odd)
+    static constexpr int S0 = N + (N >> 5);
+    static constexpr int LDSV = (S0 % 2 == 0) ? S0 + 1 : S0;
+};
+
+__host__ __device__ constexpr int padc(int i) { return i + (i >> 5); }
+
+// Stage S: 16 elements per thread in x (already loaded for S == 0), twiddle,
+// DFT, write to LDS (or HBM for the last stage).
+template <int LOG2N, int VPB, int S, bool PLAIN>
+__device__ __forceinline__ void p2_stage(cpx<float> (&x)[16], cpx<float> *lds, rsrc_t tw,
+                                         const int *tw_off, rsrc_t out, long out_vs, float osgn,
+                                         float scale) {
+    using P = P2<LOG2N>;
+    constexpr int N = P::N;
+    constexpr int LR = P::bits(S);
+    constexpr int R = 1 << LR;
+    constexpr int B = 16 / R;                 // butterflies per thread
+    constexpr int LNB = LOG2N - LR;           // log2 of butterflies per vector
+    constexpr int NB = 1 << LNB;
+    constexpr int LNS = P::log2ns(S);
+    constexpr int NS = 1 << LNS;
+    constexpr bool LAST = (S == P::NST - 1);
+    constexpr int T = VPB * N / 16;
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+
+    if (S > 0) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int b = t + i * T;
+            const int v = b >> LNB, j = b & (NB - 1);
+            const cpx<float> *src = lds + v * P::LDSV + padc(j);
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[i * R + r] = src[padc(r * NB)];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        const int b = t + i * T;
+        const int v = b >> LNB, j = b & (NB - 1);
+        const int k = j & (NS - 1);
+        if (S > 0) {
+            const int toff = tw_off[S] * 8;
+#pragma unroll
+            for (int r = 1; r < R; ++r)
+                x[i * R + r] = cmul(x[i * R + r], bload(tw, k * 8, toff + (r - 1) * NS * 8));
+        }
+        Dft<float, R>::run(&x[i * R]);
+        if (LAST) {
+            const int vo = (int)(v * out_vs + j) * 8;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const cpx<float> y = x[i * R + r];
+                if (PLAIN) bstore(out, vo, r * NB * 8, y);
+                else bstore(out, vo, r * NB * 8, {y.x * scale, y.y * (scale * osgn)});
+            }
+        } else {
+            const int base = ((j >> LNS) << (LNS + LR)) + k;
+            cpx<float> *dst = lds + v * P::LDSV + padc(base);
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[padc(r * NS)] = x[i * R + r];
+        }
+    }
+    if (!LAST) __syncthreads();
+}
+
+// HBM loads of one group of VPB rows in first-stage order
+template <int LOG2N, int VPB, bool PLAIN>
+__device__ __forceinline__ void p2_load(cpx<float> (&x)[16], rsrc_t in, long in_vs, float isgn) {
+    constexpr int N = 1 << LOG2N;
+    constexpr int LR = P2<LOG2N>::bits(0);
+    constexpr int R = 1 << LR;
+    constexpr int B = 16 / R;
+    constexpr int LNB = LOG2N - LR;
+    constexpr int NB = 1 << LNB;
+    constexpr int T = VPB * N / 16;
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        const int b = t + i * T;
+        const int v = b >> LNB, j = b & (NB - 1);
+        const int vo = (int)(v * in_vs + j) * 8;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const cpx<float> y = bload(in, vo, r * NB * 8);   // rows past the group end read 0
+            if (PLAIN) x[i * R + r] = y;
+            else x[i * R + r] = {y.x, y.y * isgn};
+        }
+    }
+}
+
+// buffer range of the group starting at row g*VPB: its rows, clipped to nvec
+template <int LOG2N, int VPB>
+__device__ __forceinline__ unsigned group_bytes(long nvec, long g, long vs) {
+    long nv = nvec - g * VPB;
+    if (nv > VPB) nv = VPB;
+    return (unsigned)(((nv - 1) * vs + (1 << LOG2N)) * 8);
+}
+
+template <int LOG2N, int VPB, bool PF, bool PLAIN>
+__device__ __forceinline__ void pow2_body(const vhip_fftm_desc &d, int per_wg, cpx<float> *lds) {
+    using P = P2<LOG2N>;
+    const cpx<float> *in = (const cpx<float> *)d.in.re;
+    cpx<float> *out = (cpx<float> *)d.out.re;
+    const rsrc_t tw = make_rsrc(d.tw, (unsigned)(d.tw_off[P::NST - 1] + 15 * (P::N / 16) + 16) * 8);
+    const float isgn = d.conj_in ? -1.f : 1.f;
+    const float osgn = d.conj_out ? -1.f : 1.f;
+    const float scale = (float)d.scale;
+    const long ivs = d.in.vs0, ovs = d.out.vs0;
+    const long ngroups = (d.nvec + VPB - 1) / VPB;
+    long g = (long)blockIdx.x * per_wg;
+    long gend = g + per_wg;
+    if (gend > ngroups) gend = ngroups;
+    if (g >= gend) return;
+
+    cpx<float> x[16];
+    p2_load<LOG2N, VPB, PLAIN>(x, make_rsrc(in + g * VPB * ivs, group_bytes<LOG2N, VPB>(d.nvec, g, ivs)), ivs, isgn);
+    for (; g < gend; ++g) {
+        cpx<float> nx[PF ? 16 : 1];
+        const long gn = g + 1;
+        if (PF && gn < gend)
+            p2_load<LOG2N, VPB, PLAIN>(*reinterpret_cast<cpx<float>(*)[16]>(nx),
+                                make_rsrc(in + gn * VPB * ivs, group_bytes<LOG2N, VPB>(d.nvec, gn, ivs)),
+                                ivs, isgn);
+        const rsrc_t o = make_rsrc(out + g * VPB * ovs, group_bytes<LOG2N, VPB>(d.nvec, g, ovs));
+        p2_stage<LOG2N, VPB, 0, PLAIN>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+        if constexpr (P::NST > 1) p2_stage<LOG2N, VPB, (P::NST > 1 ? 1 : 0), PLAIN>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+        if constexpr (P::NST > 2) p2_stage<LOG2N, VPB, (P::NST > 2 ? 2 : 0), PLAIN>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+        if constexpr (P::NST > 3) p2_stage<LOG2N, VPB, (P::NST > 3 ? 3 : 0), PLAIN>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+        if (PF) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) x[e] = nx[PF ? e : 0];
+        } else if (gn < gend) {
+            p2_load<LOG2N, VPB, PLAIN>(x, make_rsrc(in + gn * VPB * ivs, group_bytes<LOG2N, VPB>(d.nvec, gn, ivs)),
+                                ivs, isgn);
+        }
+    }
+}
+
+// One work item of the batched kernel: VPB = 8192 / N rows starting at
+// vector vbase of the transform d, with the batched workgroup of 512 lanes.
+template <int LOG2N>
+__device__ __forceinline__ void pow2_batch_item(const vhip_fftm_desc &d, long vbase, char *smem) {
+    constexpr int VPB = 8192 >> LOG2N;
+    using P = P2<LOG2N>;
+    const cpx<float> *in = (const cpx<float> *)d.in.re;
+    cpx<float> *out = (cpx<float> *)d.out.re;
+    cpx<float> *lds = (cpx<float> *)smem;
+    const rsrc_t tw = make_rsrc(d.tw, (unsigned)(d.tw_off[P::NST - 1] + 15 * (P::N / 16) + 16) * 8);
+    const float isgn = d.conj_in ? -1.f : 1.f;
+    const float osgn = d.conj_out ? -1.f : 1.f;
+    const float scale = (float)d.scale;
+    const long ivs = d.in.vs0, ovs = d.out.vs0;
+    const long g = vbase / VPB;
+    cpx<float> x[16];
+    p2_load<LOG2N, VPB, false>(x, make_rsrc(in + vbase * ivs, group_bytes<LOG2N, VPB>(d.nvec, g, ivs)), ivs, isgn);
+    const rsrc_t o = make_rsrc(out + vbase * ovs, group_bytes<LOG2N, VPB>(d.nvec, g, ovs));
+    p2_stage<LOG2N, VPB, 0, false>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 1) p2_stage<LOG2N, VPB, (P::NST > 1 ? 1 : 0), false>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 2) p2_stage<LOG2N, VPB, (P::NST > 2 ? 2 : 0), false>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 3) p2_stage<LOG2N, VPB, (P::NST > 3 ? 3 : 0), false>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+}
+
+// Does descriptor d run on the power-of-two path of the batched kernel?
+__device__ __forceinline__ int pow2_batch_log2(const vhip_fftm_desc &d) {
+    const int n = d.n;
+    if ((n & (n - 1)) || n < 256 || n > 8192) return 0;
+    if (!d.in.contig || !d.out.contig || d.ptw || d.map_in != VHIP_MAP_ALONG || d.map_out != VHIP_MAP_ALONG)
+        return 0;
+    if (d.vpb != 8192 / n || d.in.nv0 < d.nvec || d.out.nv0 < d.nvec) return 0;
+    return 31 - __builtin_clz(n);
+}
+
+}  // namespace p2
+}  // namespace vhipk

@@ -108,14 +108,10 @@ typedef struct {
 } vhip_mat;
 
 /* One work item of a batched launch: vectors [vbase, vbase + vpb) of the
- * transform described by descs[desc].  Items of the second pass of a
- * two-pass transform wait until ctl[1 + wait_slot] reaches wait_count; items
- * of its first pass add one to ctl[1 + signal_slot] when done. */
+ * transform described by descs[desc]. */
 typedef struct {
     int desc;
-    int wait_slot;      /* -1: no wait */
-    int wait_count;
-    int signal_slot;    /* -1: no signal */
+    int pad_;
     long vbase;
 } vhip_batch_item;
 
@@ -124,12 +120,15 @@ int vhip_mat_copy(int prec, int cplx, const vhip_mat *src, const vhip_mat *dst,
                   long m, long n, void *stream);
 int vhip_fftm_launch(const vhip_fftm_desc *d, void *stream);
 
-/* Batched launch: one persistent kernel of `threads` threads per workgroup
- * and `lds_bytes` of LDS per workgroup drains the item list in order; ctl
- * holds 1 + nslots counters and is cleared by a fill kernel first.  Both
+/* Batched launch: a persistent kernel of `threads` lanes per workgroup and
+ * `lds_bytes` of LDS drains items[0 .. nitems1), then (when nitems2 > 0) a
+ * second one drains items[nitems1 .. nitems1 + nitems2), the second passes of
+ * two-pass transforms.  ctl holds 4 counters that must be zero before the
+ * first launch (vhip_batch_clear); every launch leaves them zero again.  All
  * launches go to `stream` (which may be capturing a graph). */
-int vhip_batch_launch(const vhip_fftm_desc *descs, const vhip_batch_item *items, long nitems,
-                      unsigned *ctl, int nslots, int threads, int maxe, int lds_bytes,
+int vhip_batch_clear(unsigned *ctl, void *stream);
+int vhip_batch_launch(const vhip_fftm_desc *descs, const vhip_batch_item *items, long nitems1,
+                      long nitems2, unsigned *ctl, int threads, int maxe, int lds_bytes,
                       int grid, void *stream);
 int vhip_batch_max_grid(int threads, int maxe, int lds_bytes, int *grid);
 

@@ -9,11 +9,12 @@
  *     lengths that do not fit one workgroup run as two passes
  *     (n = n1 * n2, twiddle e^{-2 pi i n2 k1 / n} between them, scratch owned
  *     by the batch);
- *   - the work-item list: first-pass items of two-pass transforms first, then
- *     the single-pass items, largest transforms first, then the second-pass
- *     items (which wait on their transform's first-pass counter);
- *   - a graph of [counter clear, persistent kernel] captured on a private
- *     stream.
+ *   - two work-item lists: the single-pass items and the first-pass items of
+ *     two-pass transforms (longest transforms first, items of one length
+ *     together so its twiddles stay in cache), then the second-pass items;
+ *   - a graph of [persistent kernel over list 1, persistent kernel over
+ *     list 2] captured on a private stream (each launch leaves its work
+ *     counters cleared for the next replay).
  * vsip_amd_batch_run_f replays the graph on the library stream, so it orders
  * with the plain vsip_* calls and costs one graph launch per batch.  The
  * views' blocks must stay admitted and their device data must not move while
@@ -39,7 +40,7 @@ struct vsip_amd_batch_f {
     int njobs;
     int ndesc;
     long nitems;
-    int nslots;
+    long nitems2;
     int grid;
     vhip_fftm_desc *desc_dev;
     vhip_batch_item *items_dev;
@@ -95,16 +96,16 @@ static int va_batch_split(int prec, long n, const VA_sched **a, const VA_sched *
 }
 
 typedef struct {
-    long cost;      /* vectors * length of the item's transform */
+    long len;       /* length of the item's transform pass */
     vhip_batch_item it;
-    int phase;      /* 0 first pass, 1 single pass, 2 second pass */
+    int phase;      /* 0 single or first pass, 1 second pass */
 } VA_item_sort;
 
 static int va_item_cmp(const void *x, const void *y)
 {
     const VA_item_sort *a = (const VA_item_sort *)x, *b = (const VA_item_sort *)y;
     if (a->phase != b->phase) return a->phase - b->phase;
-    if (a->cost != b->cost) return a->cost > b->cost ? -1 : 1;
+    if (a->len != b->len) return a->len > b->len ? -1 : 1;
     if (a->it.desc != b->it.desc) return a->it.desc - b->it.desc;
     return a->it.vbase < b->it.vbase ? -1 : (a->it.vbase > b->it.vbase);
 }
@@ -180,44 +181,32 @@ vsip_amd_batch_f *vsip_amd_batch_create_f(vsip_length njobs, vsip_fftm_f *const 
             for (g = 0; g < ng; g++) {
                 VA_item_sort *it = &items[ni++];
                 it->it.desc = b->ndesc + k;
+                it->it.pad_ = 0;
                 it->it.vbase = g * d->vpb;
-                it->it.wait_slot = -1;
-                it->it.wait_count = 0;
-                it->it.signal_slot = -1;
-                it->cost = (long)(p->MN * p->mN);
-                if (nd == 1) {
-                    it->phase = 1;
-                } else if (k == 0) {
-                    it->phase = 0;
-                    it->it.signal_slot = b->nslots;
-                } else {
-                    long ngA = (descs[b->ndesc].nvec + descs[b->ndesc].vpb - 1) / descs[b->ndesc].vpb;
-                    it->phase = 2;
-                    it->it.wait_slot = b->nslots;
-                    it->it.wait_count = (int)ngA;
-                }
+                it->len = d->n;
+                it->phase = (k == 1);
+                if (k == 1) b->nitems2++;
             }
         }
-        if (nd == 2) b->nslots++;
         b->ndesc += nd;
     }
     qsort(items, (size_t)ni, sizeof(VA_item_sort), va_item_cmp);
     flat = (vhip_batch_item *)malloc(sizeof(vhip_batch_item) * (size_t)(ni ? ni : 1));
     if (!flat) goto fail;
     for (j = 0; j < ni; j++) flat[j] = items[j].it;
-    b->nitems = ni;
+    b->nitems = ni - b->nitems2;
 
     rc = vhip_malloc((void **)&b->desc_dev, sizeof(vhip_fftm_desc) * (size_t)b->ndesc);
     if (!rc) rc = vhip_malloc((void **)&b->items_dev, sizeof(vhip_batch_item) * (size_t)(ni ? ni : 1));
-    if (!rc) rc = vhip_malloc((void **)&b->ctl_dev, sizeof(unsigned) * (size_t)(b->nslots + 1));
+    if (!rc) rc = vhip_malloc((void **)&b->ctl_dev, sizeof(unsigned) * 4);
+    if (!rc) rc = vhip_memset(b->ctl_dev, 0, sizeof(unsigned) * 4);
     if (!rc) rc = vhip_memcpy_h2d(b->desc_dev, descs, sizeof(vhip_fftm_desc) * (size_t)b->ndesc);
     if (!rc && ni) rc = vhip_memcpy_h2d(b->items_dev, flat, sizeof(vhip_batch_item) * (size_t)ni);
     if (!rc) rc = vhip_batch_max_grid(VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS, &b->grid);
-    if (!rc && b->grid > ni && ni > 0) b->grid = (int)ni;
     if (!rc) rc = vhip_stream_create(&b->cap_stream);
     if (!rc) rc = vhip_graph_begin(b->cap_stream);
     if (!rc) {
-        int lr = vhip_batch_launch(b->desc_dev, b->items_dev, b->nitems, b->ctl_dev, b->nslots,
+        int lr = vhip_batch_launch(b->desc_dev, b->items_dev, b->nitems, b->nitems2, b->ctl_dev,
                                    VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS, b->grid, b->cap_stream);
         rc = vhip_graph_end(b->cap_stream, &b->graph);
         if (!rc) rc = lr;
@@ -265,10 +254,10 @@ int vsip_amd_batch_info_f(const vsip_amd_batch_f *b, char *buf, size_t len)
     char tmp[512];
     int pos;
     if (!b) return -1;
-    pos = sprintf(tmp, "{\"jobs\": %d, \"descriptors\": %d, \"items\": %ld, \"two_pass_jobs\": %d, "
-                  "\"grid\": %d, \"threads\": %d, \"lds_bytes\": %d, \"elements\": %ld}",
-                  b->njobs, b->ndesc, b->nitems, b->two_pass_jobs, b->grid, VA_BATCH_THREADS,
-                  VA_BATCH_LDS, b->total_elems);
+    pos = sprintf(tmp, "{\"jobs\": %d, \"descriptors\": %d, \"items\": %ld, \"second_pass_items\": %ld, "
+                  "\"two_pass_jobs\": %d, \"grid\": %d, \"threads\": %d, \"lds_bytes\": %d, \"elements\": %ld}",
+                  b->njobs, b->ndesc, b->nitems + b->nitems2, b->nitems2, b->two_pass_jobs, b->grid,
+                  VA_BATCH_THREADS, VA_BATCH_LDS, b->total_elems);
     if (buf && len) {
         strncpy(buf, tmp, len - 1);
         buf[len - 1] = '\0';

@@ -118,10 +118,12 @@ struct vsip_fftmattributes_d { VA_fftm p; };
 
 /* Workgroup shape of the batched (persistent) kernel: every transform of a
  * batch runs with these lanes, register class and LDS bytes per workgroup;
- * single-pass lengths up to 8192 fit, longer ones run in two passes. */
+ * single-pass lengths up to 8192 fit, longer ones run in two passes.  The
+ * LDS size is what 8192/n rows of every power-of-two n from 256 to 8192 need
+ * (the largest is 32 rows of 256, padded), two workgroups per CU. */
 #define VA_BATCH_THREADS 512
 #define VA_BATCH_MAXE 16
-#define VA_BATCH_LDS 67712
+#define VA_BATCH_LDS 68480
 
 /* ---- shared helpers ------------------------------------------------------ */
 int va_ensure_init(void);

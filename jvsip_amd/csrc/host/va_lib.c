@@ -103,6 +103,52 @@ static int va_describe(const VA_fftm *p, char *buf, size_t len)
     return pos;
 }
 
+typedef struct {
+    void *buf[4];
+    size_t bytes;
+    int kind, i;
+    void *sink;
+} VA_probe;
+
+static int va_probe_step(void *arg)
+{
+    VA_probe *p = (VA_probe *)arg;
+    int rc = vhip_probe_launch(p->kind, p->buf[p->i & 3], p->bytes, p->sink, vhip_stream());
+    p->i++;
+    return rc;
+}
+
+int vsip_amd_hbm_probe(size_t bytes, int reps, double *rmw_gbs, double *read_gbs)
+{
+    VA_probe p;
+    float ms = 0.f;
+    int k, rc = 0;
+    *rmw_gbs = *read_gbs = 0.0;
+    if (va_ensure_init()) return va_init_rc;
+    memset(&p, 0, sizeof(p));
+    p.bytes = bytes;
+    for (k = 0; k < 4 && !rc; k++) {
+        rc = vhip_malloc(&p.buf[k], bytes);
+        if (!rc) rc = vhip_memset(p.buf[k], 0, bytes);
+    }
+    if (!rc) rc = vhip_malloc(&p.sink, 256);
+    if (!rc) {
+        p.kind = 0;
+        rc = vhip_event_time(vhip_stream(), va_probe_step, &p, reps, &ms);
+        if (!rc && ms > 0.f) *rmw_gbs = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
+    }
+    if (!rc) {
+        p.kind = 1;
+        rc = vhip_event_time(vhip_stream(), va_probe_step, &p, reps, &ms);
+        if (!rc && ms > 0.f) *read_gbs = (double)bytes / (ms * 1e-3) / 1e9;
+    }
+    vhip_sync();
+    for (k = 0; k < 4; k++)
+        if (p.buf[k]) vhip_free(p.buf[k]);
+    if (p.sink) vhip_free(p.sink);
+    return rc;
+}
+
 int vsip_amd_schedule(int prec, long n, long nvec, int lds_max, char *buf, size_t len)
 {
     VA_sched sc;

@@ -11,7 +11,10 @@
  *   3. while a launch lane (a stream with its own descriptor tables) is free,
  *      requests are admitted from the FIFO head as long as their output and
  *      scratch fit the HBM budget and the batch has room, and the lane
- *      launches them as one persistent batched kernel.
+ *      launches them as one persistent batched kernel (a batch of one or two
+ *      requests launches their own kernels directly instead, which needs no
+ *      descriptor upload and keeps the latency of a lightly loaded server at
+ *      that of a single call).
  * Requests that arrive while others run are admitted as lanes and memory
  * free up; batches on different lanes overlap on the device.  Inputs are read
  * from a device-resident input pool (the request names an offset); outputs go
@@ -33,6 +36,9 @@
 #include <time.h>
 #include "va_internal.h"
 #include "vsip_amd.h"
+
+/* batches of at most this many requests launch each request's own kernel */
+#define VA_DIRECT_MAX 2
 
 static double va_now(void)
 {
@@ -255,7 +261,7 @@ static int va_req_two_pass(const vsip_amd_request *r)
 typedef struct {
     void *stream, *event;
     vhip_fftm_desc *desc_dev, *desc_host;
-    vhip_batch_item *items_dev, *items_host;
+    vhip_batch_item *items_host;
     unsigned *ctl_dev;
     int cap_desc;
     long cap_items;
@@ -273,10 +279,13 @@ static int va_lane_init(VA_lane *l, int max_jobs)
     if (!l->reqs) return -1;
     rc = vhip_stream_create(&l->stream);
     if (!rc) rc = vhip_event_create(&l->event);
-    if (!rc) rc = vhip_malloc((void **)&l->desc_dev, sizeof(vhip_fftm_desc) * (size_t)l->cap_desc);
-    if (!rc) rc = vhip_malloc((void **)&l->items_dev, sizeof(vhip_batch_item) * (size_t)l->cap_items);
-    if (!rc) rc = vhip_malloc((void **)&l->ctl_dev, sizeof(unsigned) * (size_t)(l->cap_desc + 1));
-    if (!rc) rc = vhip_host_alloc((void **)&l->desc_host, sizeof(vhip_fftm_desc) * (size_t)l->cap_desc);
+    /* descriptors and items travel in one upload: desc_host/desc_dev hold both */
+    if (!rc) rc = vhip_malloc((void **)&l->desc_dev, sizeof(vhip_fftm_desc) * (size_t)l->cap_desc +
+                              sizeof(vhip_batch_item) * (size_t)l->cap_items);
+    if (!rc) rc = vhip_malloc((void **)&l->ctl_dev, sizeof(unsigned) * 4);
+    if (!rc) rc = vhip_memset(l->ctl_dev, 0, sizeof(unsigned) * 4);
+    if (!rc) rc = vhip_host_alloc((void **)&l->desc_host, sizeof(vhip_fftm_desc) * (size_t)l->cap_desc +
+                                  sizeof(vhip_batch_item) * (size_t)l->cap_items);
     if (!rc) rc = vhip_host_alloc((void **)&l->items_host, sizeof(vhip_batch_item) * (size_t)l->cap_items);
     return rc;
 }
@@ -286,7 +295,6 @@ static void va_lane_fini(VA_lane *l)
     if (l->stream) vhip_stream_destroy(l->stream);
     if (l->event) vhip_event_destroy(l->event);
     if (l->desc_dev) vhip_free(l->desc_dev);
-    if (l->items_dev) vhip_free(l->items_dev);
     if (l->ctl_dev) vhip_free(l->ctl_dev);
     if (l->desc_host) vhip_host_free(l->desc_host);
     if (l->items_host) vhip_host_free(l->items_host);
@@ -303,6 +311,70 @@ static void *va_pool_addr(const vsip_cblock_f *pool, long off)
     return (char *)pool->b.dev_r + (size_t)off * 8;
 }
 
+/* plain schedules by length, for batches small enough to launch directly */
+typedef struct {
+    VA_fftm *p;
+    int n;
+} VA_plan_cache;
+
+static const VA_fftm *va_plan_get(VA_plan_cache *c, long n)
+{
+    int i;
+    for (i = 0; i < c->n; i++)
+        if ((long)c->p[i].MN == n) return &c->p[i];
+    return NULL;
+}
+
+static int va_plan_cache_fill(VA_plan_cache *c, const vsip_amd_request *reqs, long nreq)
+{
+    long i;
+    c->p = (VA_fftm *)calloc(64, sizeof(VA_fftm));
+    c->n = 0;
+    if (!c->p) return -1;
+    for (i = 0; i < nreq; i++) {
+        if (va_plan_get(c, reqs[i].n)) continue;
+        if (c->n == 64) return -1;
+        if (va_fftm_init(&c->p[c->n], 4, VA_CCFFTOP, 512, (vsip_length)reqs[i].n, 1.0, -1, VSIP_ROW, 0,
+                         VSIP_ALG_TIME))
+            return -1;
+        c->n++;
+    }
+    return 0;
+}
+
+static void va_plan_cache_free(VA_plan_cache *c)
+{
+    int i;
+    for (i = 0; i < c->n; i++) va_fftm_release(&c->p[i]);
+    free(c->p);
+    c->p = NULL;
+    c->n = 0;
+}
+
+/* launch request r directly with its length's plain schedule */
+static int va_direct_launch(const VA_plan_cache *pc, const vsip_amd_request *r, void *in, void *out,
+                            void *stream)
+{
+    const VA_fftm *base = va_plan_get((VA_plan_cache *)pc, r->n);
+    VA_fftm p;
+    VA_block bi, bo;
+    VA_mview vi, vo;
+    vhip_fftm_desc d[2];
+    int nd, k, rc = 0;
+    if (!base) return -1;
+    p = *base;
+    p.mN = (vsip_length)r->rows;
+    p.M = (vsip_length)r->rows;
+    p.dir = r->dir;
+    p.scale = r->scale;
+    va_fake_view(&bi, &vi, in, r->rows, r->n);
+    va_fake_view(&bo, &vo, out, r->rows, r->n);
+    nd = va_fftm_descs(&p, &vi, &vo, NULL, NULL, NULL, NULL, d);
+    if (nd < 0) return -1;
+    for (k = 0; k < nd && !rc; k++) rc = vhip_fftm_launch(&d[k], stream);
+    return rc;
+}
+
 int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_f *in_pool,
                      const vsip_cblock_f *out_pool, size_t hbm_budget, int nstreams, int max_jobs,
                      vsip_amd_result *res, char *stats, size_t stats_len)
@@ -311,18 +383,23 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
     VA_lane *lanes = NULL;
     VA_reqmem *mem = NULL;
     VA_ptw_cache pc;
-    long *fifo = NULL, head = 0, tail = 0, next = 0, done = 0, i;
+    VA_plan_cache plc;
+    void **req_in = NULL, **req_out = NULL;
+    long *fifo = NULL, head = 0, tail = 0, next = 0, done = 0, i, direct = 0;
     long batches = 0, jobs_launched = 0, max_inflight = 0;
     int L, rc = 0, grid_max = 0;
     double t0, tend;
     if (va_ensure_init()) return -1;
     if (!in_pool || !in_pool->b.dev_r || nreq <= 0 || nstreams < 1 || max_jobs < 1) return -1;
     memset(&pc, 0, sizeof(pc));
+    memset(&plc, 0, sizeof(plc));
     if (va_arena_init(&arena, hbm_budget)) { va_error("serve", "HBM budget allocation failed", -1); return -1; }
     lanes = (VA_lane *)calloc((size_t)nstreams, sizeof(VA_lane));
     mem = (VA_reqmem *)calloc((size_t)nreq, sizeof(VA_reqmem));
     fifo = (long *)malloc(sizeof(long) * (size_t)nreq);
-    if (!lanes || !mem || !fifo) { rc = -1; goto out; }
+    req_in = (void **)malloc(sizeof(void *) * (size_t)nreq);
+    req_out = (void **)malloc(sizeof(void *) * (size_t)nreq);
+    if (!lanes || !mem || !fifo || !req_in || !req_out) { rc = -1; goto out; }
     for (L = 0; L < nstreams && !rc; L++) rc = va_lane_init(&lanes[L], max_jobs);
     if (!rc) rc = vhip_batch_max_grid(VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS, &grid_max);
     /* warm the schedule cache for every length before the clock starts, and
@@ -335,6 +412,7 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
         if (reqs[i].out_offset >= 0 && !out_pool) rc = -1;
         if (need + 512 > hbm_budget) rc = -1;
     }
+    if (!rc && va_plan_cache_fill(&plc, reqs, nreq)) rc = -1;
     if (rc) { va_error("serve", "setup failed", rc); goto out; }
     for (i = 0; i < nreq; i++) {
         res[i].t_admit = res[i].t_done = -1.0;
@@ -353,7 +431,7 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
             int q;
             if (!l->busy) continue;
             q = vhip_event_query(l->event);
-            if (q == 1) { inflight++; continue; }
+            if (q == 1) continue;
             if (q != 0) { rc = q; va_error("serve", vhip_error_string(q), q); goto out; }
             now = va_now() - t0;
             for (i = 0; i < l->nreqs; i++) {
@@ -366,12 +444,11 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
             l->busy = 0;
             l->nreqs = 0;
         }
-        if (inflight > max_inflight) max_inflight = inflight;
         /* admit and launch */
         for (L = 0; L < nstreams && head < tail; L++) {
             VA_lane *l = &lanes[L];
             int nd = 0;
-            long ni = 0, g;
+            long ni = 0, n1 = 0, n2b = 0, g;
             if (l->busy) continue;
             while (head < tail && l->nreqs < max_jobs) {
                 long r = fifo[head];
@@ -409,22 +486,19 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
                 for (k = 0; k < n2; k++) {
                     const vhip_fftm_desc *d = &l->desc_host[nd + k];
                     long ng = (d->nvec + d->vpb - 1) / d->vpb;
+                    /* first lists fill from the front, second passes from the back */
                     for (g = 0; g < ng; g++) {
-                        vhip_batch_item *it = &l->items_host[ni++];
+                        vhip_batch_item *it = (k == 0) ? &l->items_host[n1++]
+                                                       : &l->items_host[l->cap_items - 1 - (n2b++)];
                         it->desc = nd + k;
+                        it->pad_ = 0;
                         it->vbase = g * d->vpb;
-                        it->wait_slot = -1;
-                        it->wait_count = 0;
-                        it->signal_slot = -1;
-                        if (n2 == 2 && k == 0) it->signal_slot = nd;
-                        if (n2 == 2 && k == 1) {
-                            it->wait_slot = nd;
-                            it->wait_count = (int)((l->desc_host[nd].nvec + l->desc_host[nd].vpb - 1) /
-                                                   l->desc_host[nd].vpb);
-                        }
                     }
+                    ni += ng;
                 }
                 nd += n2;
+                req_in[r] = in;
+                req_out[r] = out;
                 res[r].t_admit = va_now() - t0;
                 res[r].batch = (int)batches;
                 res[r].stream = L;
@@ -432,19 +506,34 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
                 head++;
             }
             if (l->nreqs == 0) break;   /* the FIFO head does not fit the budget yet */
-            /* A two-pass request's second-pass items must follow its first-pass
-             * items, which they do: items are appended in request order. */
-            rc = vhip_memcpy_h2d_async(l->desc_dev, l->desc_host, sizeof(vhip_fftm_desc) * (size_t)nd, l->stream);
-            if (!rc) rc = vhip_memcpy_h2d_async(l->items_dev, l->items_host, sizeof(vhip_batch_item) * (size_t)ni, l->stream);
-            if (!rc) rc = vhip_batch_launch(l->desc_dev, l->items_dev, ni, l->ctl_dev, nd, VA_BATCH_THREADS,
-                                            VA_BATCH_MAXE, VA_BATCH_LDS, (int)(ni < grid_max ? ni : grid_max),
-                                            l->stream);
+            if (l->nreqs <= VA_DIRECT_MAX) {
+                /* a small batch: each request's own plain kernel, no upload */
+                for (g = 0; g < l->nreqs && !rc; g++) {
+                    long r = l->reqs[g];
+                    rc = va_direct_launch(&plc, &reqs[r], req_in[r], req_out[r], l->stream);
+                }
+                direct += l->nreqs;
+            } else {
+                /* one upload: the descriptors, then the first list, then the second
+                 * list (stacked from the back of the item area while building) */
+                char *hb = (char *)l->desc_host, *db = (char *)l->desc_dev;
+                size_t dbytes = sizeof(vhip_fftm_desc) * (size_t)nd;
+                vhip_batch_item *hi = (vhip_batch_item *)(hb + dbytes);
+                memmove(hi, l->items_host, sizeof(vhip_batch_item) * (size_t)n1);
+                for (g = 0; g < n2b; g++) hi[n1 + g] = l->items_host[l->cap_items - n2b + g];
+                rc = vhip_memcpy_h2d_async(db, hb, dbytes + sizeof(vhip_batch_item) * (size_t)ni, l->stream);
+                if (!rc) rc = vhip_batch_launch(l->desc_dev, (const vhip_batch_item *)(db + dbytes), n1, n2b,
+                                                l->ctl_dev, VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS,
+                                                grid_max, l->stream);
+            }
             if (!rc) rc = vhip_event_record(l->event, l->stream);
             if (rc) { va_error("serve", vhip_error_string(rc), rc); goto out; }
             l->busy = 1;
             batches++;
             jobs_launched += l->nreqs;
         }
+        for (inflight = 0, L = 0; L < nstreams; L++) inflight += lanes[L].busy;
+        if (inflight > max_inflight) max_inflight = inflight;
     }
     tend = va_now() - t0;
     if (stats && stats_len) {
@@ -453,10 +542,11 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
         for (i = 0; i < nreq; i++) bytes += 2.0 * (double)reqs[i].rows * (double)reqs[i].n * 8.0;
         sprintf(tmp, "{\"requests\": %ld, \"seconds\": %.6f, \"req_per_s\": %.1f, \"gb_per_s\": %.2f, "
                 "\"batches\": %ld, \"jobs_per_batch\": %.2f, \"lanes\": %d, \"max_jobs\": %d, "
-                "\"max_lanes_in_flight\": %ld, \"hbm_budget\": %lu, \"hbm_peak\": %lu}",
+                "\"max_lanes_in_flight\": %ld, \"hbm_budget\": %lu, \"hbm_peak\": %lu, "
+                "\"direct_launched\": %ld}",
                 nreq, tend, (double)nreq / tend, bytes / tend / 1e9, batches,
                 batches ? (double)jobs_launched / (double)batches : 0.0, nstreams, max_jobs,
-                max_inflight + 1, (unsigned long)hbm_budget, (unsigned long)arena.peak);
+                max_inflight, (unsigned long)hbm_budget, (unsigned long)arena.peak, direct);
         strncpy(stats, tmp, stats_len - 1);
         stats[stats_len - 1] = '\0';
     }
@@ -468,10 +558,13 @@ out:
         va_lane_fini(&lanes[L]);
     }
     va_ptw_clear(&pc);
+    va_plan_cache_free(&plc);
     va_arena_fini(&arena);
     free(lanes);
     free(mem);
     free(fifo);
+    free(req_in);
+    free(req_out);
     return rc;
 }
 

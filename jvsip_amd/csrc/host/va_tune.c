@@ -46,7 +46,7 @@ typedef struct {
 static int va_trial_run(void *arg)
 {
     VA_trial *t = (VA_trial *)arg;
-    return vhip_batch_launch(t->desc_dev, t->items_dev, t->nitems, t->ctl, 0, VA_BATCH_THREADS,
+    return vhip_batch_launch(t->desc_dev, t->items_dev, t->nitems, 0, t->ctl, VA_BATCH_THREADS,
                              VA_BATCH_MAXE, VA_BATCH_LDS, t->grid, t->stream);
 }
 
@@ -84,14 +84,13 @@ static float va_time_sched(const VA_sched *s, int prec, void *scratch, long nvec
     if (!items) return -1.f;
     for (k = 0; k < ni; k++) {
         items[k].desc = 0;
-        items[k].wait_slot = -1;
-        items[k].wait_count = 0;
-        items[k].signal_slot = -1;
+        items[k].pad_ = 0;
         items[k].vbase = k * d.vpb;
     }
     rc = vhip_malloc((void **)&tr.desc_dev, sizeof(d));
     if (!rc) rc = vhip_malloc((void **)&tr.items_dev, sizeof(vhip_batch_item) * (size_t)ni);
     if (!rc) rc = vhip_malloc((void **)&tr.ctl, 64);
+    if (!rc) rc = vhip_memset(tr.ctl, 0, 64);
     if (!rc) rc = vhip_memcpy_h2d(tr.desc_dev, &d, sizeof(d));
     if (!rc) rc = vhip_memcpy_h2d(tr.items_dev, items, sizeof(vhip_batch_item) * (size_t)ni);
     if (!rc) rc = vhip_batch_max_grid(VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS, &tr.grid);
@@ -178,6 +177,21 @@ int va_tune_get(int prec, int n, const VA_sched **out)
     memset(t, 0, sizeof(*t));
     t->prec = prec;
     t->n = n;
+
+    /* power-of-two lengths 256..8192 take the batched kernel's specialised
+     * path, whose schedule is fixed: even radix-16/8/4/2 split, 8192/n rows */
+    if (prec == 4 && n >= 256 && n <= 8192 && (n & (n - 1)) == 0) {
+        int r2[VHIP_MAX_STAGES], k2 = va_factor_variant(n, prec, 4, 0, r2);
+        if (va_sched_shape_fixed(&t->s, prec, n, r2, k2, VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS,
+                                 8192 / n) == 0 && t->s.along.vpb == 8192 / n && va_sched_upload(&t->s, prec) == 0) {
+            t->rc = 0;
+            va_tuned_n++;
+            *out = &t->s;
+            return 0;
+        }
+        va_sched_free(&t->s);
+        memset(&t->s, 0, sizeof(t->s));
+    }
 
     /* candidates */
     vmax = (VA_BATCH_THREADS * VA_BATCH_MAXE) / n;

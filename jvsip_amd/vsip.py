@@ -97,6 +97,9 @@ def _sig(t):
         "mcreate": (P, [c_vi, c_vi, c_bl, c_bl]),
         "mdestroy": (P, [P]),
         "malldestroy": (None, [P]),
+        "mcopyfrom_user": (None, [P, c_bl, P]),
+        "mcopyto_user": (None, [P, c_bl, P]),
+        "msubview": (P, [P, c_vi, c_vi, c_vi, c_vi]),
         "mget": (sc, [P, c_vi, c_vi]),
         "mput": (None, [P, c_vi, c_vi, sc]),
         "ccfftmip_create": (P, [c_vi, c_vi, sc, c_bl, c_bl, ctypes.c_uint, c_bl]),
@@ -168,6 +171,8 @@ class AmdLib(VsipLib):
             "vsip_amd_batch_run_f": (ctypes.c_int, [P]),
             "vsip_amd_batch_destroy_f": (None, [P]),
             "vsip_amd_batch_info_f": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
+            "vsip_amd_hbm_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                                  ctypes.POINTER(ctypes.c_double)]),
             "vsip_amd_tune": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_char_p, ctypes.c_size_t]),
             "vsip_amd_serve_f": (ctypes.c_int, [ctypes.POINTER(amd_request), ctypes.c_long, P, P, ctypes.c_size_t,
                                                 ctypes.c_int, ctypes.c_int, ctypes.POINTER(amd_result),
