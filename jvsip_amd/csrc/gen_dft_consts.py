@@ -9,6 +9,8 @@ from decimal import Decimal, getcontext
 
 getcontext().prec = 50
 PRIMES = (3, 5, 7, 11, 13, 17, 19, 23, 29, 31)
+# composite radices built from two smaller DFTs; their internal twiddles
+COMPOSITES = (6, 9, 10, 12, 14, 15)
 
 
 def _pi():
@@ -59,6 +61,18 @@ def main():
         out.append("  /* cos(2 pi m / %d), sin(2 pi m / %d), m = 0..%d */" % (p, p, p - 1))
         out.append("  static constexpr double c[%d] = {%s};" % (p, ", ".join(cs)))
         out.append("  static constexpr double s[%d] = {%s};" % (p, ", ".join(ss)))
+        out.append("};")
+    out.append("template <int R> struct CompTrig;")
+    for r in COMPOSITES:
+        cs, ss = [], []
+        for m in range(r):
+            c, s_ = _cos_sin(2 * pi * m / r)
+            cs.append("%.20e" % c)
+            ss.append("%.20e" % s_)
+        out.append("template <> struct CompTrig<%d> {" % r)
+        out.append("  /* cos(2 pi m / %d), sin(2 pi m / %d), m = 0..%d */" % (r, r, r - 1))
+        out.append("  static constexpr double c[%d] = {%s};" % (r, ", ".join(cs)))
+        out.append("  static constexpr double s[%d] = {%s};" % (r, ", ".join(ss)))
         out.append("};")
     print("\n".join(out))
 

@@ -178,11 +178,50 @@ struct DftOdd {
         x[0] = y0;
     }
 };
+// Composite R = R1 * R2 from two smaller DFTs: n = R2 n1 + n2, k = k1 + R1 k2,
+// y[k1 + R1 k2] = sum_n2 W_R2^{n2 k2} [W_R^{n2 k1} sum_n1 x[R2 n1 + n2] W_R1^{n1 k1}].
+template <typename T, int R1, int R2, template <typename, int> class D>
+struct DftComp {
+    __device__ __forceinline__ static void run(cpx<T> *x) {
+        constexpr int R = R1 * R2;
+        cpx<T> t[R1], u[R2], y[R];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) t[n1] = x[R2 * n1 + n2];
+            D<T, R1>::run(t);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) {
+                const int m = (n2 * k1) % R;
+                const T c = (T)CompTrig<R>::c[m];
+                const T s = -(T)CompTrig<R>::s[m];   // forward: e^{-2 pi i m / R}
+                x[R2 * k1 + n2] = (m == 0) ? t[k1] : cmulc(t[k1], c, s);
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) u[n2] = x[R2 * k1 + n2];
+            D<T, R2>::run(u);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) y[k1 + R1 * k2] = u[k2];
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) x[k] = y[k];
+    }
+};
+
 template <typename T> struct Dft<T, 3> : DftOdd<T, 3> {};
 template <typename T> struct Dft<T, 5> : DftOdd<T, 5> {};
 template <typename T> struct Dft<T, 7> : DftOdd<T, 7> {};
 template <typename T> struct Dft<T, 11> : DftOdd<T, 11> {};
 template <typename T> struct Dft<T, 13> : DftOdd<T, 13> {};
+template <typename T> struct Dft<T, 6> : DftComp<T, 2, 3, Dft> {};
+template <typename T> struct Dft<T, 9> : DftComp<T, 3, 3, Dft> {};
+template <typename T> struct Dft<T, 10> : DftComp<T, 2, 5, Dft> {};
+template <typename T> struct Dft<T, 12> : DftComp<T, 4, 3, Dft> {};
+template <typename T> struct Dft<T, 14> : DftComp<T, 2, 7, Dft> {};
+template <typename T> struct Dft<T, 15> : DftComp<T, 3, 5, Dft> {};
 
 // --------------------------------------------------------------------------
 // indexing helpers
@@ -290,10 +329,19 @@ __device__ __forceinline__ void store_out(const vhip_fftm_desc &d, long voff, in
 // --------------------------------------------------------------------------
 // one Stockham stage
 // --------------------------------------------------------------------------
+
+// butterflies one thread holds in a stage of radix R (the host planner uses the
+// same rule): MAXE / R, at least one; radices other than 1-5, 7, 8 and 16
+// (composites and the primes 11, 13) hold one
+constexpr int stage_maxb(int R, int maxe) {
+    return (R == 6 || R >= 9) && R != 16 ? 1 : ((maxe / R) > 0 ? maxe / R : 1);
+}
 template <typename T, int R, int MAXE, int KIND, bool SRC_G, bool DST_G>
 __device__ __forceinline__ void stage(const vhip_fftm_desc &d, const Shared &sh, cpx<T> *lds,
                                       int s, int Ns, int nv, T isgn, T osgn, T scale) {
-    constexpr int MAXB = (MAXE / R) > 0 ? (MAXE / R) : 1;
+    // butterflies per thread: the register class allows MAXE elements, except
+    // that composite and large prime radices stay at one butterfly per thread
+    constexpr int MAXB = stage_maxb(R, MAXE);
     const int n = d.n;
     const int nbv = n / R;          // butterflies per vector
     const int nbt = nv * nbv;       // butterflies in this workgroup
@@ -394,6 +442,13 @@ __device__ __forceinline__ void run_stage(const vhip_fftm_desc &d, const Shared 
     case 11: stage_dispatch_pos<T, 11, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
     case 13: stage_dispatch_pos<T, 13, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
     case 1: stage_dispatch_pos<T, 1, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    // composite radices (the planner emits them for float only)
+    case 6: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 6, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 9: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 9, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 10: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 10, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 12: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 12, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 14: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 14, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 15: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 15, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
     default: break;  // the planner only emits the radices above
     }
 }
@@ -489,8 +544,10 @@ __device__ __forceinline__ void fft_body(const vhip_fftm_desc &d, long vbase, in
     if (KIND == VHIP_R2C) r2c_post<T>(d, sh, lds, nv, scale);
 }
 
+// The 32-element register class runs at most 512 lanes per workgroup, which
+// leaves it 256 registers per lane; the smaller classes allow 1024 lanes.
 template <typename T, int MAXE, int KIND>
-__global__ void __launch_bounds__(1024) fftm_kernel(const vhip_fftm_desc d) {
+__global__ void __launch_bounds__(MAXE > 16 ? 512 : 1024) fftm_kernel(const vhip_fftm_desc d) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const long vbase = (long)blockIdx.x * d.vpb;
     long left = d.nvec - vbase;

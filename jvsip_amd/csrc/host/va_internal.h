@@ -150,6 +150,7 @@ int va_sched_shape_fixed(VA_sched *s, int prec, int n, const int *radix, int nst
                          int threads, int maxe, int lds_max, int vpb_max);
 int va_sched_upload(VA_sched *s, int prec);
 int va_factor_variant(int n, int prec, int maxbits, int odd_first, int *radix);
+int va_factor_pack(int n, int *radix);
 int va_tune_get(int prec, int n, const VA_sched **out);
 void va_sched_free(VA_sched *s);
 int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, double scale,

@@ -76,14 +76,49 @@ int va_factor_variant(int n, int prec, int maxbits, int odd_first, int *radix)
     return nst;
 }
 
-/* the default schedule's factorisation */
+/*
+ * Pack the prime factors of n (all <= 13) into as few stages as possible with
+ * every radix <= 16 (float kernels: 2..16 including the composite radices
+ * 6, 9, 10, 12, 14, 15): first-fit over the factors in decreasing order.
+ * Returns the stage count or -1.
+ */
+int va_factor_pack(int n, int *radix)
+{
+    int f[64], nf = 0, m = n, p, i, j, nst = 0;
+    static const int primes[6] = {13, 11, 7, 5, 3, 2};
+    if (n < 1) return -1;
+    for (i = 0; i < 6; i++)
+        while (m % primes[i] == 0 && nf < 64) { f[nf++] = primes[i]; m /= primes[i]; }
+    if (m != 1) return -1;
+    for (i = 0; i < nf; i++) {
+        p = f[i];
+        for (j = 0; j < nst; j++)
+            if (radix[j] * p <= 16) { radix[j] *= p; break; }
+        if (j == nst) {
+            if (nst >= VHIP_MAX_STAGES) return -1;
+            radix[nst++] = p;
+        }
+    }
+    if (n == 1) radix[nst++] = 1;
+    /* larger radices first */
+    for (i = 1; i < nst; i++)
+        for (j = i; j > 0 && radix[j] > radix[j - 1]; j--) { p = radix[j]; radix[j] = radix[j - 1]; radix[j - 1] = p; }
+    return nst;
+}
+
+/* the default schedule's factorisation: powers of two split evenly (the
+ * specialised power-of-two kernels expect that split), other float lengths
+ * packed into radices <= 16, double lengths in radix <= 8 and prime stages */
 static int va_factor(int n, int prec, int *radix)
 {
+    if (prec == 4 && (n & (n - 1)) != 0) return va_factor_pack(n, radix);
     return va_factor_variant(n, prec, prec == 4 ? 4 : 3, 0, radix);
 }
 
+/* butterflies one thread holds in a stage of radix r (the kernels' rule) */
 static int va_maxb(int r, int maxe)
 {
+    if ((r == 6 || r >= 9) && r != 16) return 1;
     return (maxe / r) > 0 ? (maxe / r) : 1;
 }
 
@@ -104,13 +139,14 @@ static long va_lds_bytes(const VA_sched *s, int prec, int vpb)
     return (long)vpb * s->lds_stride * 2 * prec + (long)vpb * 20;
 }
 
-/* smallest thread count (multiple of 64, <= 1024) that fits; 0 if none */
+/* smallest thread count (multiple of 64, <= 1024; <= 512 for the 32-element
+ * register class) that fits; 0 if none */
 static int va_fit_threads(const VA_sched *s, int vpb, int maxe)
 {
     long want = ((long)vpb * s->n + maxe - 1) / maxe;
-    int thr = (int)(((want + 63) / 64) * 64);
+    int thr = (int)(((want + 63) / 64) * 64), tmax = maxe > 16 ? 512 : 1024;
     if (thr < 64) thr = 64;
-    for (; thr <= 1024; thr += 64)
+    for (; thr <= tmax; thr += 64)
         if (va_shape_ok(s, vpb, thr, maxe)) return thr;
     return 0;
 }

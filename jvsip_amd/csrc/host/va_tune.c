@@ -5,8 +5,9 @@
  * workgroup shape (VA_BATCH_THREADS lanes, VA_BATCH_MAXE register class,
  * VA_BATCH_LDS bytes of LDS).  For each transform length the tuner times the
  * candidate schedules on the device and keeps the fastest:
- *   - factorisations: the power-of-two part in stages of at most 16, 8 or 4
- *     points, odd stages before or after it;
+ *   - factorisations: all factors packed into radices <= 16; the power-of-two
+ *     part in stages of at most 16, 8 or 4 points with the odd stages before
+ *     or after it;
  *   - vectors per workgroup: the most that fit, and half of that.
  * Each candidate runs as a one-transform batch over about 64 MiB of scratch
  * data, timed with HIP events.  Results live in a per-process table and, when
@@ -154,8 +155,8 @@ int va_tune_get(int prec, int n, const VA_sched **out)
 {
     int i, mb, of, h, nc = 0, rad[VHIP_MAX_STAGES], nst, best = -1, vmax, rc;
     static const int bits_f[3] = {4, 3, 2};
-    VA_sched cand[12];
-    float ms[12];
+    VA_sched cand[16];
+    float ms[16];
     void *scratch = NULL, *stream = NULL;
     long nvec;
     VA_tuned *t;
@@ -196,10 +197,11 @@ int va_tune_get(int prec, int n, const VA_sched **out)
     /* candidates */
     vmax = (VA_BATCH_THREADS * VA_BATCH_MAXE) / n;
     if (vmax < 1) vmax = 1;
-    for (mb = 0; mb < 3; mb++) {
+    for (mb = -1; mb < 3; mb++) {
         for (of = 0; of < 2; of++) {
             int dup = 0, c;
-            nst = va_factor_variant(n, prec, bits_f[mb], of, rad);
+            if (mb < 0 && (of || prec != 4)) continue;
+            nst = (mb < 0) ? va_factor_pack(n, rad) : va_factor_variant(n, prec, bits_f[mb], of, rad);
             if (nst < 0) { t->rc = -1; va_tuned_n++; return -1; }
             for (c = 0; c < nc && !dup; c++) {
                 int same = (cand[c].nst == nst), q;

@@ -55,17 +55,18 @@ def _kernel_rules_hold(s, prec, lds_max):
     n = s["n"]
     prod = 1
     for r in s["radix"]:
-        assert r in (1, 2, 3, 4, 5, 7, 8, 11, 13) + ((16,) if prec == 4 else ())
+        assert r in (1, 2, 3, 4, 5, 7, 8, 11, 13) + ((6, 9, 10, 12, 14, 15, 16) if prec == 4 else ())
         prod *= r
     assert prod == n
     S = s["lds_stride"]
     assert S >= n + (n >> (5 if prec == 4 else 4)) and S % 2 == 1
     for vpb, thr, maxe in (s["along"], s["across"]):
-        assert maxe in maxe_ok and 64 <= thr <= 1024 and thr % 64 == 0 and vpb >= 1
+        assert maxe in maxe_ok and 64 <= thr <= (512 if maxe > 16 else 1024) and thr % 64 == 0 and vpb >= 1
         assert vpb * S * 2 * prec + vpb * 20 <= lds_max
         for r in s["radix"]:
             per_thread = -(-(vpb * (n // r)) // thr)
-            assert per_thread <= max(1, maxe // r), (n, r, vpb, thr, maxe)
+            cap = 1 if (r == 6 or r >= 9) and r != 16 else max(1, maxe // r)
+            assert per_thread <= cap, (n, r, vpb, thr, maxe)
 
 
 @pytest.mark.parametrize("prec", [4, 8])

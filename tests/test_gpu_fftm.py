@@ -46,7 +46,7 @@ def test_ccfftmip_row_pow2(amd, ref, m, n):
     _check("vs jvsip", y, fftm.ccfftm(ref, x.ravel(), v), TOL_F)
 
 
-@pytest.mark.parametrize("n", [3, 5, 7, 11, 13, 6, 9, 10, 12, 15, 24, 60, 100, 143, 1000, 1001,
+@pytest.mark.parametrize("n", [3, 5, 7, 11, 13, 6, 9, 10, 12, 14, 15, 24, 60, 96, 100, 143, 243, 1000, 1001,
                                1920, 2187, 3125, 6561, 1152, 4000])
 def test_ccfftmip_row_mixed(amd, ref, n):
     m = 6
