@@ -115,7 +115,49 @@ typedef struct {
     long vbase;
 } vhip_batch_item;
 
+/*
+ * Element passes of a staged transform, which runs its c2c part on
+ * contiguous scratch rows (row v of the launch at work + v * m complex
+ * elements) between a PRE pass from the caller's view and a POST pass back:
+ *   PRE  C2C  w_k = chirp_k * (conj?) x_k, k < n; 0 for n <= k < m
+ *        R2C  w_k = chirp_k * (x_2k + i x_2k+1)
+ *        C2R  w_k = chirp_k * conj(Y_k), Y_k = (Z_k + conj Z_{n-k})
+ *                   + i e_k (Z_k - conj Z_{n-k}), e_k = euler[k]
+ *   MUL  w_k *= table[k], k < m (every row)
+ *   POST with Y_k = chirp_k * w_{rev(k)}, rev(k) = (m - k) mod m when
+ *        reverse, else k:
+ *        C2C  y_k = scale * (conj?) Y_k, k < n
+ *        R2C  z_k = scale / 2 * [(Y_k + conj Y_{n-k}) - i e_k (Y_k - conj Y_{n-k})],
+ *             k = 0..n, Y_n = Y_0
+ *        C2R  x_2k + i x_2k+1 = scale * conj(Y_k)
+ * chirp == NULL means chirp_k = 1.  The view's vector v of a launch is
+ * vector vbase + v of the descriptor's view.
+ */
+#define VHIP_XF_PRE 0
+#define VHIP_XF_MUL 1
+#define VHIP_XF_POST 2
+
+typedef struct {
+    int prec;
+    int kind;              /* VHIP_C2C / VHIP_R2C / VHIP_C2R */
+    int step;              /* VHIP_XF_PRE / VHIP_XF_MUL / VHIP_XF_POST */
+    int conj;              /* C2C: conjugate on the way in (PRE) or out (POST) */
+    int reverse;           /* POST: read w at (m - k) mod m */
+    int across;            /* lanes walk across vectors (unit vector stride) */
+    long n;                /* complex transform length */
+    long m;                /* scratch row length, m >= n */
+    long nvec;             /* vectors in this launch */
+    long vbase;            /* first vector of the view */
+    double scale;          /* POST */
+    const void *chirp;     /* n entries or NULL */
+    const void *table;     /* MUL: m entries */
+    const void *euler;     /* R2C: e^{-i pi k/n}; C2R: e^{+i pi k/n}; k = 0..n */
+    vhip_view view;
+    void *work;
+} vhip_xform_desc;
+
 /* kernels */
+int vhip_xform_launch(const vhip_xform_desc *d, void *stream);
 int vhip_mat_copy(int prec, int cplx, const vhip_mat *src, const vhip_mat *dst,
                   long m, long n, void *stream);
 int vhip_fftm_launch(const vhip_fftm_desc *d, void *stream);

@@ -68,6 +68,7 @@ void va_fftm_release(VA_fftm *p)
     vhip_sync();
     va_sched_free(&p->pass[0]);
     va_sched_free(&p->pass[1]);
+    va_staged_release(p);
     if (p->dev_euler) vhip_free(p->dev_euler);
     if (p->dev_ptw) vhip_free(p->dev_ptw);
     if (p->dev_work) vhip_free(p->dev_work);
@@ -100,8 +101,11 @@ static int va_plan_split(VA_fftm *p)
     return best ? 0 : -1;
 }
 
-int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, double scale,
-                 int dir, vsip_major major, unsigned int ntimes, vsip_alg_hint hint)
+/* allow_staged == 0 gives up where a staged transform would be needed (the
+ * inner plans of staged transforms, whose lengths are always smooth) */
+static int va_fftm_init2(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, double scale,
+                         int dir, vsip_major major, unsigned int ntimes, vsip_alg_hint hint,
+                         int allow_staged)
 {
     int rc, real = (type == VA_RCFFTOP || type == VA_CRFFTOP);
     memset(p, 0, sizeof(*p));
@@ -129,12 +133,20 @@ int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, d
     if (p->MN > 0x7fffffffUL) return -1;
     p->npass = 1;
     rc = va_sched_plan(&p->pass[0], prec, (int)p->MN, p->mN);
-    if (rc == -2 && !real) {
-        va_sched_free(&p->pass[0]);
-        if (va_plan_split(p)) return -1;
+    if (rc == -2 && !real && !va_plan_split(p)) {
         p->npass = 2;
         if (va_upload_table(&p->dev_ptw, prec, (long)p->MN, -1, (double)p->MN, 2.0)) goto fail;
         if (vhip_malloc(&p->dev_work, (size_t)p->mN * p->MN * 2 * (size_t)prec)) goto fail;
+    } else if (allow_staged && (rc == -1 || rc == -2)) {
+        /* a prime factor without a butterfly (-1), or a length beyond one
+         * workgroup that the split cannot serve: staged transform; Bluestein
+         * unless only the real transform's length is the obstacle */
+        int blue = (rc == -1) || !real;
+        va_sched_free(&p->pass[0]);
+        memset(&p->pass[0], 0, sizeof(p->pass[0]));
+        p->npass = 0;
+        if (blue && va_staged_init(p, 1)) goto fail;
+        if (!blue && va_staged_init(p, 0)) goto fail;
     } else if (rc) {
         va_sched_free(&p->pass[0]);
         return -1;
@@ -148,6 +160,17 @@ int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, d
 fail:
     va_fftm_release(p);
     return -1;
+}
+
+int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, double scale,
+                 int dir, vsip_major major, unsigned int ntimes, vsip_alg_hint hint)
+{
+    return va_fftm_init2(p, prec, type, M, N, scale, dir, major, ntimes, hint, 1);
+}
+
+int va_fftm_init_inner(VA_fftm *p, int prec, vsip_length M, vsip_length N)
+{
+    return va_fftm_init2(p, prec, VA_CCFFTIP, M, N, 1.0, VSIP_FFT_FWD, VSIP_ROW, 0, VSIP_ALG_TIME, 0);
 }
 
 /* transform axis of a view: element stride, vector stride, length, count */
@@ -221,21 +244,24 @@ static void va_desc_common(vhip_fftm_desc *d, const VA_fftm *p, const VA_sched *
  * selects the split with n1 = sA->n, n2 = sB->n.  Returns the descriptor
  * count, or -1 after reporting an error.
  */
-int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, const VA_sched *sA,
-                  const VA_sched *sB, void *work, const void *ptw, vhip_fftm_desc *d)
+/*
+ * Check that (in, out) fit plan p and give their device views with the
+ * transform axis' element and vector strides.  Returns the transform kind,
+ * or -1 after reporting an error.
+ */
+int va_exec_views(const VA_fftm *p, const VA_mview *in, const VA_mview *out, vhip_view *gi,
+                  vhip_view *go, long *es_i, long *vs_i, long *es_o, long *vs_o)
 {
-    long es_i, vs_i, es_o, vs_o;
     vsip_length n_i, nv_i, n_o, nv_o, want_in, want_out;
-    vhip_view gi, go;
-    int kind, inv, two;
+    int kind;
     if (!p || p->markings != VA_VALID) { va_error("fftm", "invalid plan", -1); return -1; }
     if (!in || !out) { va_error("fftm", "NULL view", -1); return -1; }
     if (in->block->prec != p->prec || out->block->prec != p->prec) {
         va_error("fftm", "view precision does not match the plan", -1);
         return -1;
     }
-    va_axis(in, p->major, &es_i, &vs_i, &n_i, &nv_i);
-    va_axis(out, p->major, &es_o, &vs_o, &n_o, &nv_o);
+    va_axis(in, p->major, es_i, vs_i, &n_i, &nv_i);
+    va_axis(out, p->major, es_o, vs_o, &n_o, &nv_o);
     kind = (p->type == VA_RCFFTOP) ? VHIP_R2C : (p->type == VA_CRFFTOP) ? VHIP_C2R : VHIP_C2C;
     want_in = (kind == VHIP_R2C) ? 2 * p->MN : (kind == VHIP_C2R) ? p->MN + 1 : p->MN;
     want_out = (kind == VHIP_R2C) ? p->MN + 1 : (kind == VHIP_C2R) ? 2 * p->MN : p->MN;
@@ -243,10 +269,22 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
         va_error("fftm", "view shape does not match the plan", -1);
         return -1;
     }
-    if (va_view_dev(in, &gi) || va_view_dev(out, &go)) {
+    if (va_view_dev(in, gi) || va_view_dev(out, go)) {
         va_error("fftm", "block is not admitted", -1);
         return -1;
     }
+    return kind;
+}
+
+int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, const VA_sched *sA,
+                  const VA_sched *sB, void *work, const void *ptw, vhip_fftm_desc *d)
+{
+    long es_i, vs_i, es_o, vs_o;
+    vhip_view gi, go;
+    int kind, inv, two;
+    kind = va_exec_views(p, in, out, &gi, &go, &es_i, &vs_i, &es_o, &vs_o);
+    if (kind < 0) return -1;
+    if (p->staged) { va_error("fftm", "a staged plan has no single launch descriptor", -1); return -1; }
     inv = (kind == VHIP_C2C && p->dir > 0);
     if (!sA) {
         sA = &p->pass[0];
@@ -258,7 +296,7 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
     if (two && kind != VHIP_C2C) { va_error("fftm", "two-pass real transform", -1); return -1; }
 
     if (!two) {
-        int map_in = va_map(es_i, vs_i, nv_i), map_out = va_map(es_o, vs_o, nv_o);
+        int map_in = va_map(es_i, vs_i, p->mN), map_out = va_map(es_o, vs_o, p->mN);
         gi.es = es_i; gi.vs0 = vs_i; gi.nv0 = (long)p->mN; gi.vs1 = 0;
         go.es = es_o; go.vs0 = vs_o; go.nv0 = (long)p->mN; go.vs1 = 0;
         if (kind == VHIP_R2C)
@@ -317,6 +355,7 @@ int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void
     vhip_fftm_desc d[2];
     int n, k, rc = 0;
     if (p && p->mN == 0) return 0;
+    if (p && p->markings == VA_VALID && p->staged) return va_staged_exec(p, in, out, stream);
     n = va_fftm_descs(p, in, out, NULL, NULL, NULL, NULL, d);
     if (n < 0) return -1;
     for (k = 0; k < n && !rc; k++) {

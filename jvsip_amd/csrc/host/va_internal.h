@@ -110,6 +110,17 @@ typedef struct VA_fftm {
     void *dev_euler;         /* real transforms: e^{-/+ i pi k/MN}, k = 0..MN */
     void *dev_ptw;           /* four-step: e^{-2 pi i m / MN}, m = 0..MN-1 */
     void *dev_work;          /* four-step: mN * MN complex scratch */
+    /* staged transforms (va_stage.c): PRE pass view -> scratch rows of length
+     * sm, a forward c2c plan on them (twice, with the chirp filter product
+     * between, for Bluestein), POST pass scratch -> view; `chunk` vectors per
+     * slab */
+    int staged;              /* 0 none, 1 real transform around a long c2c, 2 Bluestein */
+    vsip_length sm;
+    vsip_length chunk;
+    struct VA_fftm *inner;   /* forward ccfftmip, ROW, chunk x sm */
+    void *dev_chirp;         /* Bluestein: e^{-i pi k^2 / MN}, k < MN */
+    void *dev_filt;          /* Bluestein: sm-point DFT of the conjugate chirp, / sm */
+    void *dev_stage;         /* chunk * sm complex scratch */
     int markings;
 } VA_fftm;
 
@@ -155,8 +166,16 @@ int va_tune_get(int prec, int n, const VA_sched **out);
 void va_sched_free(VA_sched *s);
 int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, double scale,
                  int dir, vsip_major major, unsigned int ntimes, vsip_alg_hint hint);
+int va_fftm_init_inner(VA_fftm *p, int prec, vsip_length M, vsip_length N);
 void va_fftm_release(VA_fftm *p);
 int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void *stream);
+int va_staged_init(VA_fftm *p, int bluestein);
+void va_staged_release(VA_fftm *p);
+int va_staged_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void *stream);
+int va_exec_views(const VA_fftm *p, const VA_mview *in, const VA_mview *out, vhip_view *gi,
+                  vhip_view *go, long *es_i, long *vs_i, long *es_o, long *vs_o);
+long va_bluestein_length(long n);
+void va_host_fft(long n, double *x);
 int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, const VA_sched *sA,
                   const VA_sched *sB, void *work, const void *ptw, vhip_fftm_desc *d);
 

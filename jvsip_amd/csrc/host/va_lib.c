@@ -78,24 +78,40 @@ void *vsip_amd_cblock_device_d(const vsip_cblock_d *b, void **im)
 void *vsip_amd_block_device_f(const vsip_block_f *b) { return b ? b->b.dev_r : NULL; }
 void *vsip_amd_block_device_d(const vsip_block_d *b) { return b ? b->b.dev_r : NULL; }
 
-static int va_describe(const VA_fftm *p, char *buf, size_t len)
+static int va_describe_passes(const VA_fftm *p, char *tmp)
 {
-    char tmp[1024];
     int pos = 0, k, s;
-    static const char *types[4] = {"ccfftmip", "ccfftmop", "rcfftmop", "crfftmop"};
-    pos += sprintf(tmp + pos, "{\"type\": \"%s\", \"prec\": %d, \"MN\": %lu, \"mN\": %lu, "
-                   "\"npass\": %d, \"passes\": [", types[p->type & 3], p->prec,
-                   (unsigned long)p->MN, (unsigned long)p->mN, p->npass);
+    pos += sprintf(tmp + pos, "\"npass\": %d, \"passes\": [", p->npass);
     for (k = 0; k < p->npass; k++) {
         const VA_sched *sc = &p->pass[k];
         pos += sprintf(tmp + pos, "%s{\"n\": %d, \"radix\": [", k ? ", " : "", sc->n);
-        for (s = 0; s < sc->nst && pos < 900; s++)
+        for (s = 0; s < sc->nst; s++)
             pos += sprintf(tmp + pos, "%s%d", s ? ", " : "", sc->radix[s]);
         pos += sprintf(tmp + pos, "], \"along\": [%d, %d, %d], \"across\": [%d, %d, %d], "
                        "\"lds_stride\": %d}", sc->along.vpb, sc->along.threads, sc->along.maxe,
                        sc->across.vpb, sc->across.threads, sc->across.maxe, sc->lds_stride);
     }
-    pos += sprintf(tmp + pos, "]}");
+    pos += sprintf(tmp + pos, "]");
+    return pos;
+}
+
+static int va_describe(const VA_fftm *p, char *buf, size_t len)
+{
+    /* at most 2 + 2 passes of <= 24 stages each: well under 4096 bytes */
+    char tmp[4096];
+    int pos = 0;
+    static const char *types[4] = {"ccfftmip", "ccfftmop", "rcfftmop", "crfftmop"};
+    pos += sprintf(tmp + pos, "{\"type\": \"%s\", \"prec\": %d, \"MN\": %lu, \"mN\": %lu, ",
+                   types[p->type & 3], p->prec, (unsigned long)p->MN, (unsigned long)p->mN);
+    pos += va_describe_passes(p, tmp + pos);
+    if (p->staged && p->inner) {
+        pos += sprintf(tmp + pos, ", \"staged\": \"%s\", \"scratch_length\": %lu, \"chunk\": %lu, "
+                       "\"inner\": {", p->staged == 2 ? "bluestein" : "real", (unsigned long)p->sm,
+                       (unsigned long)p->chunk);
+        pos += va_describe_passes(p->inner, tmp + pos);
+        pos += sprintf(tmp + pos, "}");
+    }
+    pos += sprintf(tmp + pos, "}");
     if (buf && len) {
         strncpy(buf, tmp, len - 1);
         buf[len - 1] = '\0';

@@ -238,8 +238,8 @@ class AmdLib(VsipLib):
         return out, json.loads(buf.value.decode() or "{}")
 
     def describe(self, plan, t="f"):
-        buf = ctypes.create_string_buffer(2048)
-        getattr(self, "amd_fftm_describe_" + t)(plan, buf, 2048)
+        buf = ctypes.create_string_buffer(4096)
+        getattr(self, "amd_fftm_describe_" + t)(plan, buf, 4096)
         return buf.value.decode()
 
 
