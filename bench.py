@@ -468,8 +468,10 @@ def config_c5(lib, nreq=2000, rate=20000.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=4)
+    # the device reaches its sustained clocks only after some milliseconds of
+    # work: the default warm-up runs ~25 ms of the same steps before timing
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-configs", action="store_true", help="skip the other workload configurations")

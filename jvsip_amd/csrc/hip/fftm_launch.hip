@@ -34,6 +34,7 @@ extern "C" int vhip_fftm_set_lds_limit(int bytes) {
                 if (e != hipSuccess) return (int)e;
             }
     int rc = vhipk::pow2_set_lds_limit(bytes);
+    rc = rc ? rc : vhipk::pow2_real_set_lds_limit(bytes);
     return rc ? rc : vhipk::batch_set_lds_limit(bytes);
 }
 

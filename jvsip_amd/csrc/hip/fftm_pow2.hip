@@ -29,18 +29,6 @@ __global__ void __launch_bounds__(VPB * (1 << LOG2N) / 16)
 }
 
 template <int LOG2N>
-constexpr int p2_vpb() { return LOG2N >= 12 ? 1 : (1 << (12 - LOG2N)); }
-
-template <int LOG2N>
-bool p2_radices_match(const vhip_fftm_desc *d) {
-    using P = P2<LOG2N>;
-    if (d->nst != P::NST) return false;
-    for (int s = 0; s < P::NST; ++s)
-        if (d->radix[s] != (1 << P::bits(s))) return false;
-    return true;
-}
-
-template <int LOG2N>
 int p2_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     constexpr int VPB = p2_vpb<LOG2N>();
     constexpr int T = VPB * (1 << LOG2N) / 16;
@@ -102,7 +90,8 @@ int p2_set_lds(int bytes) {
 // Launch the power-of-two row kernel when it applies; returns -100 when it
 // does not (the caller then uses the generic kernel).
 int pow2_try_launch(const vhip_fftm_desc *d, void *stream, int cus) {
-    if (d->prec != 4 || d->kind != VHIP_C2C || d->ptw) return -100;
+    if (d->prec != 4 || d->ptw) return -100;
+    if (d->kind != VHIP_C2C) return pow2_real_try_launch(d, stream);
     if (!d->in.contig || !d->out.contig || d->in.split || d->out.split) return -100;
     if (d->map_in != VHIP_MAP_ALONG || d->map_out != VHIP_MAP_ALONG) return -100;
     if (d->in.nv0 < d->nvec || d->out.nv0 < d->nvec) return -100;   // single-level vectors only

@@ -56,7 +56,7 @@ __host__ __device__ constexpr int padc(int i) { return i + (i >> 5); }
 
 // Stage S: 16 elements per thread in x (already loaded for S == 0), twiddle,
 // DFT, write to LDS (or HBM for the last stage).
-template <int LOG2N, int VPB, int S, bool PLAIN>
+template <int LOG2N, int VPB, int S, bool PLAIN, bool LAST_TO_LDS = false>
 __device__ __forceinline__ void p2_stage(cpx<float> (&x)[16], cpx<float> *lds, rsrc_t tw,
                                          const int *tw_off, rsrc_t out, long out_vs, float osgn,
                                          float scale) {
@@ -69,7 +69,7 @@ __device__ __forceinline__ void p2_stage(cpx<float> (&x)[16], cpx<float> *lds, r
     constexpr int NB = 1 << LNB;
     constexpr int LNS = P::log2ns(S);
     constexpr int NS = 1 << LNS;
-    constexpr bool LAST = (S == P::NST - 1);
+    constexpr bool LAST = (S == P::NST - 1) && !LAST_TO_LDS;   // the last stage stores to HBM
     constexpr int T = VPB * N / 16;
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
@@ -113,6 +113,19 @@ __device__ __forceinline__ void p2_stage(cpx<float> (&x)[16], cpx<float> *lds, r
         }
     }
     if (!LAST) __syncthreads();
+}
+
+// rows per workgroup of the power-of-two kernels: 4096 elements per group
+template <int LOG2N>
+constexpr int p2_vpb() { return LOG2N >= 12 ? 1 : (1 << (12 - LOG2N)); }
+
+template <int LOG2N>
+bool p2_radices_match(const vhip_fftm_desc *d) {
+    using P = P2<LOG2N>;
+    if (d->nst != P::NST) return false;
+    for (int s = 0; s < P::NST; ++s)
+        if (d->radix[s] != (1 << P::bits(s))) return false;
+    return true;
 }
 
 // HBM loads of one group of VPB rows in first-stage order
@@ -185,6 +198,94 @@ __device__ __forceinline__ void pow2_body(const vhip_fftm_desc &d, int per_wg, c
         } else if (gn < gend) {
             p2_load<LOG2N, VPB, PLAIN>(x, make_rsrc(in + gn * VPB * ivs, group_bytes<LOG2N, VPB>(d.nvec, gn, ivs)),
                                 ivs, isgn);
+        }
+    }
+}
+
+// buffer range of `len` complex elements per row for the group at row g*VPB
+template <int VPB>
+__device__ __forceinline__ unsigned group_bytes_len(long nvec, long g, long vs, long len) {
+    long nv = nvec - g * VPB;
+    if (nv > VPB) nv = VPB;
+    return (unsigned)(((nv - 1) * vs + len) * 8);
+}
+
+// Real transforms of contiguous FP32 rows whose half length n = 2^LOG2N runs
+// on the power-of-two stages (behaviour of VI_rcfftmop_f_def.h:53-115 and
+// VI_crfftmop_f_def.h:74-124, as in the generic kernel's r2c_post/c2r_pre):
+//   R2C: the real row read as n complex values x[2m] + i x[2m+1] (its own
+//        interleaved layout), the n-point stages with the last one writing
+//        LDS, then z_k = scale/2 [(Y_k + conj Y_{n-k}) - i e_k (Y_k - conj Y_{n-k})],
+//        k = 0..n, e_k = e^{-i pi k/n}, stored to the complex output row;
+//   C2R: conj(Y_k), Y_k = (Z_k + conj Z_{n-k}) + i e_k (Z_k - conj Z_{n-k}),
+//        e_k = e^{+i pi k/n}, formed in registers from the complex input row,
+//        the n-point stages, and scale * conj(result) stored as the real
+//        row's interleaved pairs.
+// Strides d.in.vs0 / d.out.vs0 of a real row are counted in reals (even, so
+// rows stay 8-byte aligned); they are halved to complex strides here.
+template <int LOG2N, int VPB, int KIND>
+__device__ __forceinline__ void pow2_real_body(const vhip_fftm_desc &d, cpx<float> *lds) {
+    using P = P2<LOG2N>;
+    constexpr int N = P::N;
+    constexpr int T = VPB * N / 16;
+    const rsrc_t tw = make_rsrc(d.tw, (unsigned)(d.tw_off[P::NST - 1] + 15 * (N / 16) + 16) * 8);
+    const cpx<float> *eu = (const cpx<float> *)d.euler;
+    const float scale = (float)d.scale;
+    const long ivs = KIND == VHIP_R2C ? d.in.vs0 / 2 : d.in.vs0;
+    const long ovs = KIND == VHIP_C2R ? d.out.vs0 / 2 : d.out.vs0;
+    const long g = blockIdx.x;
+    const cpx<float> *in = (const cpx<float> *)d.in.re + g * VPB * ivs;
+    cpx<float> *out = (cpx<float> *)d.out.re + g * VPB * ovs;
+    long nvl = d.nvec - g * VPB;
+    const int nv = nvl > VPB ? VPB : (int)nvl;
+    cpx<float> x[16];
+    if (KIND == VHIP_R2C) {
+        p2_load<LOG2N, VPB, true>(x, make_rsrc(in, group_bytes<LOG2N, VPB>(d.nvec, g, ivs)), ivs, 1.f);
+    } else {
+        constexpr int LR = P::bits(0);
+        constexpr int R = 1 << LR;
+        constexpr int B = 16 / R;
+        constexpr int LNB = LOG2N - LR;
+        constexpr int NB = 1 << LNB;
+        const rsrc_t ri = make_rsrc(in, group_bytes_len<VPB>(d.nvec, g, ivs, N + 1));
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int b = t + i * T;
+            const int v = b >> LNB, j = b & (NB - 1);
+            const int vo = (int)(v * ivs) * 8;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int e = j + r * NB;
+                const cpx<float> F = bload(ri, vo + e * 8, 0);
+                const cpx<float> B2 = bload(ri, vo + (N - e) * 8, 0);
+                const cpx<float> Bc = {B2.x, -B2.y};
+                const cpx<float> A = F + Bc;
+                const cpx<float> WD = cmul(eu[e], F - Bc);
+                x[i * R + r] = {A.x - WD.y, -(A.y + WD.x)};       // conj(A + i WD)
+            }
+        }
+    }
+    const rsrc_t o = make_rsrc(out, group_bytes<LOG2N, VPB>(d.nvec, g, ovs));
+    constexpr bool TOL = (KIND == VHIP_R2C);
+    const float osgn = KIND == VHIP_C2R ? -1.f : 1.f;
+    p2_stage<LOG2N, VPB, 0, false, TOL>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 1) p2_stage<LOG2N, VPB, (P::NST > 1 ? 1 : 0), false, TOL>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 2) p2_stage<LOG2N, VPB, (P::NST > 2 ? 2 : 0), false, TOL>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 3) p2_stage<LOG2N, VPB, (P::NST > 3 ? 3 : 0), false, TOL>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if (KIND == VHIP_R2C) {
+        // the n + 1 outputs of each row from Y in LDS (natural order)
+        const float half = 0.5f * scale;
+        for (int t = threadIdx.x; t < nv * (N + 1); t += T) {
+            const int v = t / (N + 1), k = t - v * (N + 1);
+            const cpx<float> *y = lds + v * P::LDSV;
+            const cpx<float> F = y[padc(k == N ? 0 : k)];
+            const cpx<float> B2 = y[padc(k == 0 ? 0 : N - k)];
+            const cpx<float> Bc = {B2.x, -B2.y};
+            const cpx<float> A = F + Bc;
+            const cpx<float> WD = cmul(eu[k], F - Bc);
+            out[v * ovs + k] = {half * (A.x + WD.y), half * (A.y - WD.x)};
         }
     }
 }

@@ -186,3 +186,42 @@ def test_four_step(amd, n):
     for d in (VSIP_FFT_FWD, VSIP_FFT_INV):
         y = fftm.ccfftm(amd, x.ravel(), v, direction=d, scale=0.5)
         _check("vs oracle dir %d" % d, y, po.ccfftm(x.ravel(), v, direction=d, scale=0.5), TOL_F)
+
+
+@pytest.mark.parametrize("m,n", [(5, 512), (37, 1024), (3, 4096), (6, 8192), (2, 16384), (3, 32768)])
+def test_real_pow2_rows(amd, ref, m, n):
+    """rcfftmop/crfftmop on the power-of-two real-row kernel (half lengths
+    256..16384), row counts that leave a partial last group."""
+    _case("rcfftmop_f / crfftmop_f ROW %dx%d (power-of-two rows)" % (m, n))
+    x = datagen.real_data(23, (m, n))
+    xv, zv = fftm.dense_view(m, n), fftm.dense_view(m, n // 2 + 1)
+    z0 = np.zeros(m * (n // 2 + 1), np.complex64)
+    z = fftm.rcfftm(amd, x.ravel(), xv, z0, zv, scale=0.5)
+    _check("rc vs oracle", z, po.rcfftm(x.ravel(), xv, z0, zv, scale=0.5), TOL_F)
+    if n <= 4096:
+        _check("rc vs jvsip", z, fftm.rcfftm(ref, x.ravel(), xv, z0, zv, scale=0.5), TOL_F)
+    zz = datagen.complex_data(24, (m * (n // 2 + 1),))       # not Hermitian
+    x0 = np.zeros(m * n, np.float32)
+    xb = fftm.crfftm(amd, zz, zv, x0, xv, scale=1.0 / n)
+    _check("cr vs oracle", xb, po.crfftm(zz, zv, x0, xv, scale=1.0 / n), TOL_F)
+    if n <= 4096:
+        _check("cr vs jvsip", xb, fftm.crfftm(ref, zz, zv, x0, xv, scale=1.0 / n), TOL_F)
+    back = fftm.crfftm(amd, 2 * z, zv, x0, xv, scale=1.0 / n)
+    _check("round trip", back, x.ravel(), 1e-6)
+
+
+def test_real_pow2_strided_rows(amd):
+    """Real rows 2x apart in their parents (rows of a 2N-wide real matrix,
+    complex rows of a 2(N/2+1)-wide one) on the power-of-two real kernel."""
+    m, n = 9, 2048
+    h = n // 2 + 1
+    _case("rcfftmop_f / crfftmop_f ROW %dx%d, parent rows 2x" % (m, n))
+    x = datagen.real_data(25, (m * 2 * n,))
+    xv = fftm.View(0, 2 * n, m, 1, n)
+    zv = fftm.View(2, 2 * h, m, 1, h)
+    z0 = np.zeros(m * 2 * h, np.complex64)
+    z = fftm.rcfftm(amd, x, xv, z0, zv)
+    _check("rc vs oracle", z, po.rcfftm(x, xv, z0, zv), TOL_F)
+    x0 = np.zeros(m * 2 * n, np.float32)
+    xb = fftm.crfftm(amd, z, zv, x0, xv, scale=1.0 / n)
+    _check("cr vs oracle", xb, po.crfftm(z, zv, x0, xv, scale=1.0 / n), TOL_F)
