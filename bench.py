@@ -158,10 +158,16 @@ def _dev_view(lib, x, major=0):
     return v
 
 
-def _time_calls(lib, hip, fn, reps):
-    """wall and stream-event time per call of fn() over reps back-to-back calls"""
+def _time_calls(lib, hip, fn, reps, warm_s=0.03):
+    """wall and stream-event time per call of fn() over reps back-to-back
+    calls, after at least warm_s seconds of the same calls (the device reaches
+    its sustained clocks only under load)"""
     stream = lib.amd_stream()
+    t0 = time.perf_counter()
     fn()
+    while time.perf_counter() - t0 < warm_s:
+        fn()
+        lib.amd_sync()
     lib.amd_sync()
     e0, e1 = hip.event(), hip.event()
     t0 = time.perf_counter()

@@ -151,6 +151,14 @@ static int va_fit_threads(const VA_sched *s, int vpb, int maxe)
     return 0;
 }
 
+/* min(v, the positive integer in environment variable `name`), a tuning cap */
+static int va_env_cap(const char *name, int v)
+{
+    const char *e = getenv(name);
+    int c = e ? atoi(e) : 0;
+    return (c > 0 && c < v) ? c : v;
+}
+
 static int va_pick_launch(const VA_sched *s, int prec, int vpb_want, int lds_soft, int lds_max,
                           VA_launch *out)
 {
@@ -193,15 +201,25 @@ int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max)
     s->lds_stride = va_pad(prec, n);
     if (s->lds_stride % 2 == 0) s->lds_stride += 1;
 
-    /* along: enough elements per workgroup to keep ~256 lanes busy */
+    /* along: enough elements per workgroup to keep ~256 lanes busy, but no
+     * more vectors per workgroup than leave every CU two workgroups */
     want = (prec == 4) ? 4096 : 2048;
     vpb_along = (int)(want / n);
+    if (nvec > 0) {
+        int cus = 256;
+        long fill;
+        vhip_cu_count(&cus);
+        if (cus < 1) cus = 256;
+        fill = ((long)nvec + 2L * cus - 1) / (2L * cus);
+        if (vpb_along > fill) vpb_along = (int)fill;
+    }
+    vpb_along = va_env_cap("VSIP_AMD_VPB_ALONG", vpb_along);
     if (vpb_along < 1) vpb_along = 1;
     if ((vsip_length)vpb_along > nvec && nvec > 0) vpb_along = (int)nvec;
     if (va_pick_launch(s, prec, vpb_along, 65536, lds_max, &s->along)) return -2;
 
     /* across: 16 vectors give 128-byte (float) row segments per load */
-    vpb_across = 16;
+    vpb_across = va_env_cap("VSIP_AMD_VPB_ACROSS", 16);
     if ((vsip_length)vpb_across > nvec && nvec > 0) vpb_across = (int)nvec;
     if (va_pick_launch(s, prec, vpb_across, lds_max, lds_max, &s->across)) return -2;
 

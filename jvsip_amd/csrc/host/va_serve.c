@@ -10,11 +10,13 @@
  *      time and their output/scratch memory returns to the HBM arena;
  *   3. while a launch lane (a stream with its own descriptor tables) is free,
  *      requests are admitted from the FIFO head as long as their output and
- *      scratch fit the HBM budget and the batch has room, and the lane
- *      launches them as one persistent batched kernel (a batch of one or two
- *      requests launches their own kernels directly instead, which needs no
- *      descriptor upload and keeps the latency of a lightly loaded server at
- *      that of a single call).
+ *      scratch fit the HBM budget and the batch has room; on the lane's
+ *      stream, requests of at least VA_DIRECT_ELEMS elements launch their
+ *      own length-specialised kernel, and the smaller ones share one
+ *      persistent batched kernel (when there are more than VA_DIRECT_MAX of
+ *      them: fewer launch directly too, which needs no descriptor upload and
+ *      keeps the latency of a lightly loaded server at that of a single
+ *      call).
  * Requests that arrive while others run are admitted as lanes and memory
  * free up; batches on different lanes overlap on the device.  Inputs are read
  * from a device-resident input pool (the request names an offset); outputs go
@@ -39,6 +41,17 @@
 
 /* batches of at most this many requests launch each request's own kernel */
 #define VA_DIRECT_MAX 2
+/* requests of at least this many elements launch their own (plain,
+ * length-specialised) kernel on the lane's stream; smaller ones share the
+ * lane's persistent batched launch.  VSIP_AMD_SERVE_DIRECT overrides. */
+#define VA_DIRECT_ELEMS 32768L
+
+static long va_direct_elems(void)
+{
+    const char *e = getenv("VSIP_AMD_SERVE_DIRECT");
+    if (e) return atol(e);
+    return VA_DIRECT_ELEMS;
+}
 
 static double va_now(void)
 {
@@ -266,6 +279,7 @@ typedef struct {
     int cap_desc;
     long cap_items;
     long *reqs;
+    char *direct;          /* per admitted request: 1 = own kernel, 0 = batched */
     int nreqs, busy, seq;
 } VA_lane;
 
@@ -276,7 +290,8 @@ static int va_lane_init(VA_lane *l, int max_jobs)
     l->cap_desc = 2 * max_jobs;
     l->cap_items = (long)max_jobs * 1200;
     l->reqs = (long *)malloc(sizeof(long) * (size_t)max_jobs);
-    if (!l->reqs) return -1;
+    l->direct = (char *)malloc((size_t)max_jobs);
+    if (!l->reqs || !l->direct) return -1;
     rc = vhip_stream_create(&l->stream);
     if (!rc) rc = vhip_event_create(&l->event);
     /* descriptors and items travel in one upload: desc_host/desc_dev hold both */
@@ -299,6 +314,7 @@ static void va_lane_fini(VA_lane *l)
     if (l->desc_host) vhip_host_free(l->desc_host);
     if (l->items_host) vhip_host_free(l->items_host);
     free(l->reqs);
+    free(l->direct);
 }
 
 typedef struct {
@@ -388,6 +404,7 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
     long *fifo = NULL, head = 0, tail = 0, next = 0, done = 0, i, direct = 0;
     long batches = 0, jobs_launched = 0, max_inflight = 0;
     int L, rc = 0, grid_max = 0;
+    long direct_elems = va_direct_elems();
     double t0, tend;
     if (va_ensure_init()) return -1;
     if (!in_pool || !in_pool->b.dev_r || nreq <= 0 || nstreams < 1 || max_jobs < 1) return -1;
@@ -447,18 +464,18 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
         /* admit and launch */
         for (L = 0; L < nstreams && head < tail; L++) {
             VA_lane *l = &lanes[L];
-            int nd = 0;
+            int nd = 0, nbatched = 0;
             long ni = 0, n1 = 0, n2b = 0, g;
             if (l->busy) continue;
             while (head < tail && l->nreqs < max_jobs) {
                 long r = fifo[head];
                 const vsip_amd_request *q = &reqs[r];
                 void *in, *out, *work = NULL;
-                int k, n2;
+                int k, n2, dir_launch = ((long)q->rows * q->n >= direct_elems);
                 long need_items = 0;
                 mem[r].out_len = (size_t)q->rows * (size_t)q->n * 8;
                 mem[r].own_out = (q->out_offset < 0);
-                mem[r].own_work = va_req_two_pass(q);
+                mem[r].own_work = !dir_launch && va_req_two_pass(q);
                 mem[r].work_len = mem[r].own_work ? mem[r].out_len : 0;
                 if (mem[r].own_out) {
                     mem[r].out_off = va_arena_alloc(&arena, mem[r].out_len);
@@ -474,6 +491,17 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
                 }
                 in = va_pool_addr(in_pool, q->in_offset);
                 out = mem[r].own_out ? (void *)(arena.base + mem[r].out_off) : va_pool_addr(out_pool, q->out_offset);
+                if (dir_launch) {
+                    req_in[r] = in;
+                    req_out[r] = out;
+                    res[r].t_admit = va_now() - t0;
+                    res[r].batch = (int)batches;
+                    res[r].stream = L;
+                    l->direct[l->nreqs] = 1;
+                    l->reqs[l->nreqs++] = r;
+                    head++;
+                    continue;
+                }
                 n2 = va_req_descs(q, in, out, work, &pc, &l->desc_host[nd]);
                 if (n2 < 0) { rc = -1; va_error("serve", "request has no schedule", (int)q->n); goto out; }
                 for (k = 0; k < n2; k++)
@@ -502,18 +530,21 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
                 res[r].t_admit = va_now() - t0;
                 res[r].batch = (int)batches;
                 res[r].stream = L;
+                l->direct[l->nreqs] = 0;
                 l->reqs[l->nreqs++] = r;
+                nbatched++;
                 head++;
             }
             if (l->nreqs == 0) break;   /* the FIFO head does not fit the budget yet */
-            if (l->nreqs <= VA_DIRECT_MAX) {
-                /* a small batch: each request's own plain kernel, no upload */
-                for (g = 0; g < l->nreqs && !rc; g++) {
-                    long r = l->reqs[g];
-                    rc = va_direct_launch(&plc, &reqs[r], req_in[r], req_out[r], l->stream);
-                }
-                direct += l->nreqs;
-            } else {
+            /* requests of their own kernel, and a batch of at most VA_DIRECT_MAX
+             * requests (plain kernels need no descriptor upload) */
+            for (g = 0; g < l->nreqs && !rc; g++) {
+                long r = l->reqs[g];
+                if (!l->direct[g] && nbatched > VA_DIRECT_MAX) continue;
+                rc = va_direct_launch(&plc, &reqs[r], req_in[r], req_out[r], l->stream);
+                direct++;
+            }
+            if (!rc && nbatched > VA_DIRECT_MAX) {
                 /* one upload: the descriptors, then the first list, then the second
                  * list (stacked from the back of the item area while building) */
                 char *hb = (char *)l->desc_host, *db = (char *)l->desc_dev;
