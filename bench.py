@@ -481,6 +481,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-configs", action="store_true", help="skip the other workload configurations")
+    ap.add_argument("--only", default="", help="comma-separated configuration names (or prefixes, "
+                    "e.g. c3,c4) to run besides the headline; default all")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -560,6 +562,8 @@ def main():
                          ("c3_strided_mixed_fp64", lambda: config_c3(lib, hip)),
                          ("c4_batched_uneven", lambda: config_c4(lib, hip)),
                          ("c5_served", lambda: config_c5(lib))):
+            if args.only and not any(name.startswith(o.strip()) for o in args.only.split(",") if o.strip()):
+                continue
             t0 = time.perf_counter()
             configs[name] = fn()
             configs[name]["bench_seconds"] = round(time.perf_counter() - t0, 2)

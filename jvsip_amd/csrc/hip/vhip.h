@@ -48,6 +48,10 @@ typedef struct {
     long nv0;
     int split;
     int contig;   /* set by the host: interleaved/real with es == 1 and aligned */
+    long span;    /* set by the host: bytes from re (and im) that cover every
+                     element a launch touches, when all strides are >= 0 and
+                     that is below 2 GiB (kernels then use 32-bit buffer
+                     offsets); 0 otherwise */
 } vhip_view;
 
 /* One launch of the multiple-FFT kernel: nvec independent transforms. */

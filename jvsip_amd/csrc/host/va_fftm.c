@@ -205,6 +205,24 @@ static int va_view_dev(const VA_mview *v, vhip_view *g)
 
 static long va_labs(long x) { return x < 0 ? -x : x; }
 
+/* bytes from the view base covering `nvec` vectors of `len` elements (of
+ * `esz` bytes); 0 when a stride is negative or the span reaches 2 GiB, so the
+ * kernels fall back to 64-bit addressing */
+static long va_span(const vhip_view *g, long nvec, long len, long esz)
+{
+    long nv0 = g->nv0 > 0 ? g->nv0 : nvec, last;
+    if (g->es < 0 || g->vs0 < 0 || g->vs1 < 0 || nvec < 1 || len < 1) return 0;
+    last = (nvec > nv0 ? nv0 - 1 : nvec - 1) * g->vs0 + ((nvec - 1) / nv0) * g->vs1 + (len - 1) * g->es;
+    if (last < 0 || (last + 1) > ((1L << 31) - 64) / esz) return 0;
+    return (last + 1) * esz;
+}
+
+/* element size of a view's arrays: complex interleaved 2*prec, split or real prec */
+static long va_esz(const vhip_view *g, int prec, int cplx)
+{
+    return (cplx && !g->split) ? 2L * prec : (long)prec;
+}
+
 static int va_map(long es, long vs, vsip_length nvec)
 {
     return (nvec > 1 && va_labs(vs) < va_labs(es)) ? VHIP_MAP_ACROSS : VHIP_MAP_ALONG;
@@ -307,6 +325,10 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
             go.contig = (es_o == 1 && (out->offset % 2) == 0 && (p->mN <= 1 || vs_o % 2 == 0));
         else
             go.contig = (!go.split && es_o == 1);
+        gi.span = va_span(&gi, (long)p->mN, (long)(kind == VHIP_R2C ? 2 * p->MN : kind == VHIP_C2R ? p->MN + 1 : p->MN),
+                          va_esz(&gi, p->prec, kind != VHIP_R2C));
+        go.span = va_span(&go, (long)p->mN, (long)(kind == VHIP_C2R ? 2 * p->MN : kind == VHIP_R2C ? p->MN + 1 : p->MN),
+                          va_esz(&go, p->prec, kind != VHIP_C2R));
         va_desc_common(&d[0], p, sA, kind, (long)p->mN, map_in, map_out);
         d[0].in = gi;
         d[0].out = go;
@@ -327,7 +349,9 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
         /* pass A: vectors (row, j2), elements j1 at stride n2*es; out T[row][k1][j2] */
         gi.es = n2 * es_i; gi.vs0 = es_i; gi.nv0 = n2; gi.vs1 = vs_i;
         gi.contig = 0;
+        gi.span = va_span(&gi, mN * n2, n1, va_esz(&gi, p->prec, 1));
         t.es = n2; t.vs0 = 1; t.nv0 = n2; t.vs1 = MN;
+        t.span = va_span(&t, mN * n2, n1, va_esz(&t, p->prec, 1));
         va_desc_common(&d[0], p, sA, VHIP_C2C, mN * n2, VHIP_MAP_ACROSS, VHIP_MAP_ACROSS);
         d[0].in = gi;
         d[0].out = t;
@@ -338,8 +362,10 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
         d[0].ptw_n = MN;
         /* pass B: vectors (row, k1), elements j2 contiguous; out X[row][k1 + n1*k2] */
         t.es = 1; t.vs0 = n2; t.nv0 = n1; t.vs1 = MN; t.contig = 1;
+        t.span = va_span(&t, mN * n1, n2, va_esz(&t, p->prec, 1));
         go.es = n1 * es_o; go.vs0 = es_o; go.nv0 = n1; go.vs1 = vs_o;
         go.contig = 0;
+        go.span = va_span(&go, mN * n1, n2, va_esz(&go, p->prec, 1));
         va_desc_common(&d[1], p, sB, VHIP_C2C, mN * n1, VHIP_MAP_ALONG, VHIP_MAP_ACROSS);
         d[1].in = t;
         d[1].out = go;

@@ -80,6 +80,7 @@ static float va_time_sched(const VA_sched *s, int prec, void *scratch, long nvec
     d.in.vs0 = d.out.vs0 = s->n;
     d.in.nv0 = d.out.nv0 = nvec;
     d.in.contig = d.out.contig = 1;
+    if (nvec * s->n * 2L * prec < (1L << 31) - 64) d.in.span = d.out.span = nvec * s->n * 2L * prec;
     ni = (nvec + d.vpb - 1) / d.vpb;
     items = (vhip_batch_item *)malloc(sizeof(vhip_batch_item) * (size_t)ni);
     if (!items) return -1.f;
