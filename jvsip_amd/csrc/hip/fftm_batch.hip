@@ -36,8 +36,11 @@ __device__ __forceinline__ unsigned next_index(unsigned *head) {
 // the next launch on the stream starts from zero without a clearing kernel.
 template <int MAXE>
 __global__ void __launch_bounds__(512)
-    batch_kernel(const vhip_fftm_desc *descs, const vhip_batch_item *items, long nitems,
-                 unsigned *ctl) {
+    batch_kernel(const vhip_fftm_desc *__restrict__ descs, const vhip_batch_item *__restrict__ items,
+                 long nitems, unsigned *ctl) {
+    // descriptors and items are read-only for the launch (restrict): their
+    // fields load as scalars, so buffer descriptors built from them stay in
+    // SGPRs
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ long s_item;
     if (threadIdx.x == 0) s_item = (long)next_index(&ctl[0]);
@@ -61,7 +64,7 @@ __global__ void __launch_bounds__(512)
         default: {
             const long left = d.nvec - item.vbase;
             const int nv = left < d.vpb ? (int)left : d.vpb;
-            fft_body<float, MAXE, VHIP_C2C>(d, item.vbase, nv, smem);
+            fft_body<float, MAXE, VHIP_C2C, true>(d, item.vbase, nv, smem);
         }
         }
         __syncthreads();  // every lane has read s_item

@@ -250,6 +250,49 @@ __device__ __forceinline__ long vec_off(const vhip_view &g, long v) {
     return (v - q * g.nv0) * g.vs0 + q * g.vs1;
 }
 
+// --------------------------------------------------------------------------
+// raw buffer access: a descriptor over [base, base + bytes) and 32-bit
+// offsets, one VGPR per access plus a wave-uniform SGPR offset
+// --------------------------------------------------------------------------
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+// word 3 of a raw buffer descriptor on gfx950 (32-bit data format, no swizzle)
+constexpr int kBufWord3 = 0x00020000;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kBufWord3);
+}
+typedef decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc_t(), 0, 0, 0)) raw64_t;
+typedef decltype(__builtin_amdgcn_raw_buffer_load_b128(rsrc_t(), 0, 0, 0)) raw128_t;
+template <typename T> struct Buf;
+template <> struct Buf<float> {
+    static __device__ __forceinline__ cpx<float> ldc(rsrc_t r, int v, int s) {
+        return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+    }
+    static __device__ __forceinline__ float lds(rsrc_t r, int v, int s) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+    }
+    static __device__ __forceinline__ void stc(rsrc_t r, int v, int s, cpx<float> x) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(raw64_t, x), r, v, s, 0);
+    }
+    static __device__ __forceinline__ void sts(rsrc_t r, int v, int s, float x) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s, 0);
+    }
+};
+template <> struct Buf<double> {
+    static __device__ __forceinline__ cpx<double> ldc(rsrc_t r, int v, int s) {
+        return __builtin_bit_cast(cpx<double>, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+    }
+    static __device__ __forceinline__ double lds(rsrc_t r, int v, int s) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+    }
+    static __device__ __forceinline__ void stc(rsrc_t r, int v, int s, cpx<double> x) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(raw128_t, x), r, v, s, 0);
+    }
+    static __device__ __forceinline__ void sts(rsrc_t r, int v, int s, double x) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(raw64_t, x), r, v, s, 0);
+    }
+};
+
 struct Shared {
     long *voff_in;
     long *voff_out;
@@ -259,30 +302,39 @@ struct Shared {
 // --------------------------------------------------------------------------
 // global loads / stores
 // --------------------------------------------------------------------------
-template <typename T, int KIND>
-__device__ __forceinline__ cpx<T> load_in(const vhip_fftm_desc &d, long voff, int n, T isgn) {
+// Global loads and stores of the first and last stage.  Every view of a
+// launch spans less than 2 GiB from its base with non-negative strides (the
+// host guarantees it: vhip_view.span), so all addressing is 32-bit buffer
+// offsets: vb is the vector's element offset and element j + leg of the
+// vector is accessed (leg: the butterfly leg's offset r * n / R, the same in
+// every lane).  SOFF puts leg in the scalar offset (one VGPR offset per
+// butterfly; suits kernels that read their descriptor from memory) or in the
+// vector offset (no scalar registers per leg; suits kernels whose descriptor
+// arrives as an argument and already occupies many scalar registers).
+template <typename T, int KIND, bool SOFF = false>
+__device__ __forceinline__ cpx<T> load_in(const vhip_fftm_desc &d, int vb, int j, int leg, T isgn) {
+    constexpr int ES = (int)sizeof(T);
+    const int es = (int)d.in.es;
+    const int lv = SOFF ? 0 : leg;     // leg in the vector offset, or
+    const int ls = SOFF ? leg : 0;     // in the scalar offset
     cpx<T> v;
     if (KIND == VHIP_R2C) {
-        const T *re = (const T *)d.in.re;
+        const rsrc_t r = make_rsrc(d.in.re, d.in.span);
         if (d.in.contig) {
-            v = ((const cpx<T> *)(re + voff))[n];
+            v = Buf<T>::ldc(r, vb * ES + (j + lv) * 2 * ES, ls * 2 * ES);
         } else {
-            const long o = voff + (long)(2 * n) * d.in.es;
-            v.x = re[o];
-            v.y = re[o + d.in.es];
+            const int o = (vb + 2 * (j + lv) * es) * ES, so = 2 * ls * es * ES;
+            v.x = Buf<T>::lds(r, o, so);
+            v.y = Buf<T>::lds(r, o + es * ES, so);
         }
         return v;
     }
-    if (d.in.contig) {
-        v = ((const cpx<T> *)d.in.re)[voff + n];
+    if (d.in.split) {
+        const int o = (vb + (j + lv) * es) * ES, so = ls * es * ES;
+        v.x = Buf<T>::lds(make_rsrc(d.in.re, d.in.span), o, so);
+        v.y = Buf<T>::lds(make_rsrc(d.in.im, d.in.span), o, so);
     } else {
-        const long o = voff + (long)n * d.in.es;
-        if (d.in.split) {
-            v.x = ((const T *)d.in.re)[o];
-            v.y = ((const T *)d.in.im)[o];
-        } else {
-            v = ((const cpx<T> *)d.in.re)[o];
-        }
+        v = Buf<T>::ldc(make_rsrc(d.in.re, d.in.span), (vb + (j + lv) * es) * 2 * ES, ls * es * 2 * ES);
     }
     v.y *= isgn;
     return v;
@@ -303,27 +355,37 @@ __device__ __forceinline__ void store_c(const vhip_view &g, long voff, int n, cp
     }
 }
 
-template <typename T, int KIND>
-__device__ __forceinline__ void store_out(const vhip_fftm_desc &d, long voff, int vi0, int n,
+template <typename T, int KIND, bool SOFF = false>
+__device__ __forceinline__ void store_out(const vhip_fftm_desc &d, int vb, int vi0, int j, int leg,
                                           cpx<T> v, T osgn, T scale) {
+    constexpr int ES = (int)sizeof(T);
+    const int es = (int)d.out.es;
+    const int lv = SOFF ? 0 : leg;
+    const int ls = SOFF ? leg : 0;
     if (d.ptw) {
-        const cpx<T> w = ((const cpx<T> *)d.ptw)[(long)vi0 * n];
+        const cpx<T> w = ((const cpx<T> *)d.ptw)[vi0 * (j + leg)];
         v = cmul(v, w);
     }
     v.x *= scale;
     v.y *= scale * osgn;
     if (KIND == VHIP_C2R) {
-        T *re = (T *)d.out.re;
+        const rsrc_t r = make_rsrc(d.out.re, d.out.span);
         if (d.out.contig) {
-            ((cpx<T> *)(re + voff))[n] = v;
+            Buf<T>::stc(r, vb * ES + (j + lv) * 2 * ES, ls * 2 * ES, v);
         } else {
-            const long o = voff + (long)(2 * n) * d.out.es;
-            re[o] = v.x;
-            re[o + d.out.es] = v.y;
+            const int o = (vb + 2 * (j + lv) * es) * ES, so = 2 * ls * es * ES;
+            Buf<T>::sts(r, o, so, v.x);
+            Buf<T>::sts(r, o + es * ES, so, v.y);
         }
         return;
     }
-    store_c(d.out, voff, n, v);
+    if (d.out.split) {
+        const int o = (vb + (j + lv) * es) * ES, so = ls * es * ES;
+        Buf<T>::sts(make_rsrc(d.out.re, d.out.span), o, so, v.x);
+        Buf<T>::sts(make_rsrc(d.out.im, d.out.span), o, so, v.y);
+    } else {
+        Buf<T>::stc(make_rsrc(d.out.re, d.out.span), (vb + (j + lv) * es) * 2 * ES, ls * es * 2 * ES, v);
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -336,7 +398,7 @@ __device__ __forceinline__ void store_out(const vhip_fftm_desc &d, long voff, in
 constexpr int stage_maxb(int R, int maxe) {
     return (R == 6 || R >= 9) && R != 16 ? 1 : ((maxe / R) > 0 ? maxe / R : 1);
 }
-template <typename T, int R, int MAXE, int KIND, bool SRC_G, bool DST_G>
+template <typename T, int R, int MAXE, int KIND, bool SRC_G, bool DST_G, bool SOFF>
 __device__ __forceinline__ void stage(const vhip_fftm_desc &d, const Shared &sh, cpx<T> *lds,
                                       int s, int Ns, int nv, T isgn, T osgn, T scale) {
     // butterflies per thread: the register class allows MAXE elements, except
@@ -349,7 +411,8 @@ __device__ __forceinline__ void stage(const vhip_fftm_desc &d, const Shared &sh,
     const int S = d.lds_stride;
     const int across = SRC_G ? (d.map_in == VHIP_MAP_ACROSS)
                              : (DST_G ? (d.map_out == VHIP_MAP_ACROSS) : 0);
-    const cpx<T> *tw = (const cpx<T> *)d.tw + d.tw_off[s];
+    // this stage's twiddles: (R - 1) * Ns entries from tw_off[s]
+    const rsrc_t twr = make_rsrc((const cpx<T> *)d.tw + d.tw_off[s], (long)n * (long)sizeof(cpx<T>));
 
     cpx<T> x[MAXB][R];
     int bv[MAXB], bj[MAXB];
@@ -371,9 +434,9 @@ __device__ __forceinline__ void stage(const vhip_fftm_desc &d, const Shared &sh,
             bv[i] = v;
             bj[i] = j;
             if (SRC_G) {
-                const long vo = sh.voff_in[v];
+                const int vo = (int)sh.voff_in[v];
 #pragma unroll
-                for (int r = 0; r < R; ++r) x[i][r] = load_in<T, KIND>(d, vo, j + r * nbv, isgn);
+                for (int r = 0; r < R; ++r) x[i][r] = load_in<T, KIND, SOFF>(d, vo, j, r * nbv, isgn);
             } else {
                 const cpx<T> *src = lds + v * S;
 #pragma unroll
@@ -392,16 +455,17 @@ __device__ __forceinline__ void stage(const vhip_fftm_desc &d, const Shared &sh,
                 int q;
                 divmod(j, Ns, q, k);
 #pragma unroll
-                for (int r = 1; r < R; ++r) x[i][r] = cmul(x[i][r], tw[(r - 1) * Ns + k]);
+                for (int r = 1; r < R; ++r)
+                    x[i][r] = cmul(x[i][r], Buf<T>::ldc(twr, (k + (r - 1) * Ns) * (int)sizeof(cpx<T>), 0));
             }
             Dft<T, R>::run(x[i]);
             if (DST_G) {
                 // last stage: Ns * R == n, outputs land at j + r * nbv
-                const long vo = sh.voff_out[v];
+                const int vo = (int)sh.voff_out[v];
                 const int vi0 = d.ptw ? sh.vidx0[v] : 0;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    store_out<T, KIND>(d, vo, vi0, j + r * nbv, x[i][r], osgn, scale);
+                    store_out<T, KIND, SOFF>(d, vo, vi0, j, r * nbv, x[i][r], osgn, scale);
             } else {
                 cpx<T> *dst = lds + v * S;
                 const int base = (j - k) * R + k;
@@ -413,17 +477,17 @@ __device__ __forceinline__ void stage(const vhip_fftm_desc &d, const Shared &sh,
     if (!DST_G) __syncthreads();
 }
 
-template <typename T, int R, int MAXE, int KIND>
+template <typename T, int R, int MAXE, int KIND, bool SOFF>
 __device__ __forceinline__ void stage_dispatch_pos(const vhip_fftm_desc &d, const Shared &sh,
                                                    cpx<T> *lds, int s, int Ns, int nv, bool first,
                                                    bool last, T isgn, T osgn, T scale) {
-    if (first && last) stage<T, R, MAXE, KIND, true, true>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
-    else if (first) stage<T, R, MAXE, KIND, true, false>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
-    else if (last) stage<T, R, MAXE, KIND, false, true>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
-    else stage<T, R, MAXE, KIND, false, false>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
+    if (first && last) stage<T, R, MAXE, KIND, true, true, SOFF>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
+    else if (first) stage<T, R, MAXE, KIND, true, false, SOFF>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
+    else if (last) stage<T, R, MAXE, KIND, false, true, SOFF>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
+    else stage<T, R, MAXE, KIND, false, false, SOFF>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
 }
 
-template <typename T, int MAXE, int KIND>
+template <typename T, int MAXE, int KIND, bool SOFF>
 __device__ __forceinline__ void run_stage(const vhip_fftm_desc &d, const Shared &sh, cpx<T> *lds,
                                           int s, int Ns, int nv, bool first, bool last, T isgn,
                                           T osgn, T scale) {
@@ -431,24 +495,24 @@ __device__ __forceinline__ void run_stage(const vhip_fftm_desc &d, const Shared 
     case 16:
         // radix 16 only in the float kernels (the planner never emits it for double)
         if constexpr (sizeof(T) == 4)
-            stage_dispatch_pos<T, 16, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale);
+            stage_dispatch_pos<T, 16, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale);
         break;
-    case 8: stage_dispatch_pos<T, 8, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 4: stage_dispatch_pos<T, 4, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 2: stage_dispatch_pos<T, 2, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 3: stage_dispatch_pos<T, 3, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 5: stage_dispatch_pos<T, 5, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 7: stage_dispatch_pos<T, 7, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 11: stage_dispatch_pos<T, 11, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 13: stage_dispatch_pos<T, 13, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 1: stage_dispatch_pos<T, 1, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 8: stage_dispatch_pos<T, 8, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 4: stage_dispatch_pos<T, 4, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 2: stage_dispatch_pos<T, 2, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 3: stage_dispatch_pos<T, 3, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 5: stage_dispatch_pos<T, 5, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 7: stage_dispatch_pos<T, 7, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 11: stage_dispatch_pos<T, 11, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 13: stage_dispatch_pos<T, 13, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 1: stage_dispatch_pos<T, 1, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
     // composite radices (the planner emits them for float only)
-    case 6: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 6, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 9: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 9, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 10: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 10, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 12: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 12, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 14: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 14, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 15: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 15, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 6: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 6, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 9: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 9, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 10: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 10, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 12: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 12, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 14: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 14, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 15: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 15, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
     default: break;  // the planner only emits the radices above
     }
 }
@@ -496,8 +560,8 @@ __device__ __forceinline__ void c2r_pre(const vhip_fftm_desc &d, const Shared &s
         else if (d.map_in == VHIP_MAP_ACROSS) divmod(t, nv, k, v);
         else divmod(t, n, v, k);
         const long vo = sh.voff_in[v];
-        const cpx<T> F = load_in<T, VHIP_C2C>(d, vo, k, (T)1);
-        const cpx<T> B = load_in<T, VHIP_C2C>(d, vo, n - k, (T)1);
+        const cpx<T> F = load_in<T, VHIP_C2C>(d, (int)vo, k, 0, (T)1);
+        const cpx<T> B = load_in<T, VHIP_C2C>(d, (int)vo, n - k, 0, (T)1);
         const cpx<T> Bc = {B.x, -B.y};
         const cpx<T> A = F + Bc;
         const cpx<T> WD = cmul(eu[k], F - Bc);
@@ -507,7 +571,7 @@ __device__ __forceinline__ void c2r_pre(const vhip_fftm_desc &d, const Shared &s
     }
 }
 
-template <typename T, int MAXE, int KIND>
+template <typename T, int MAXE, int KIND, bool SOFF = false>
 __device__ __forceinline__ void fft_body(const vhip_fftm_desc &d, long vbase, int nv,
                                          char *smem) {
     cpx<T> *lds = (cpx<T> *)smem;
@@ -538,7 +602,7 @@ __device__ __forceinline__ void fft_body(const vhip_fftm_desc &d, long vbase, in
     for (int s = 0; s < d.nst; ++s) {
         const bool first = (s == 0) && KIND != VHIP_C2R;
         const bool last = (s == d.nst - 1) && KIND != VHIP_R2C;
-        run_stage<T, MAXE, KIND>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale);
+        run_stage<T, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale);
         Ns *= d.radix[s];
     }
     if (KIND == VHIP_R2C) r2c_post<T>(d, sh, lds, nv, scale);

@@ -102,7 +102,8 @@ static int va_plan_split(VA_fftm *p)
 }
 
 /* allow_staged == 0 gives up where a staged transform would be needed (the
- * inner plans of staged transforms, whose lengths are always smooth) */
+ * inner plans of staged transforms, whose lengths are always smooth); 2 makes
+ * a staged plan for every length (views the kernels cannot address directly) */
 static int va_fftm_init2(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, double scale,
                          int dir, vsip_major major, unsigned int ntimes, vsip_alg_hint hint,
                          int allow_staged)
@@ -133,15 +134,20 @@ static int va_fftm_init2(VA_fftm *p, int prec, int type, vsip_length M, vsip_len
     if (p->MN > 0x7fffffffUL) return -1;
     p->npass = 1;
     rc = va_sched_plan(&p->pass[0], prec, (int)p->MN, p->mN);
+    if (allow_staged == 2 && (rc == 0 || rc == -2)) {
+        va_sched_free(&p->pass[0]);
+        rc = -4;                      /* staged around the plain schedule */
+    }
     if (rc == -2 && !real && !va_plan_split(p)) {
         p->npass = 2;
         if (va_upload_table(&p->dev_ptw, prec, (long)p->MN, -1, (double)p->MN, 2.0)) goto fail;
         if (vhip_malloc(&p->dev_work, (size_t)p->mN * p->MN * 2 * (size_t)prec)) goto fail;
-    } else if (allow_staged && (rc == -1 || rc == -2)) {
+    } else if (allow_staged && (rc == -1 || rc == -2 || rc == -4)) {
         /* a prime factor without a butterfly (-1), or a length beyond one
          * workgroup that the split cannot serve: staged transform; Bluestein
-         * unless only the real transform's length is the obstacle */
-        int blue = (rc == -1) || !real;
+         * unless only the real transform's length is the obstacle (or the
+         * staging is asked for, -4) */
+        int blue = (rc == -1) || (rc == -2 && !real);
         va_sched_free(&p->pass[0]);
         memset(&p->pass[0], 0, sizeof(p->pass[0]));
         p->npass = 0;
@@ -329,6 +335,10 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
                           va_esz(&gi, p->prec, kind != VHIP_R2C));
         go.span = va_span(&go, (long)p->mN, (long)(kind == VHIP_C2R ? 2 * p->MN : kind == VHIP_R2C ? p->MN + 1 : p->MN),
                           va_esz(&go, p->prec, kind != VHIP_C2R));
+        if (!gi.span || !go.span) {
+            va_error("fftm", "a view of one launch spans 2 GiB or more or has a negative stride", -1);
+            return -1;
+        }
         va_desc_common(&d[0], p, sA, kind, (long)p->mN, map_in, map_out);
         d[0].in = gi;
         d[0].out = go;
@@ -366,6 +376,10 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
         go.es = n1 * es_o; go.vs0 = es_o; go.nv0 = n1; go.vs1 = vs_o;
         go.contig = 0;
         go.span = va_span(&go, mN * n1, n2, va_esz(&go, p->prec, 1));
+        if (!gi.span || !go.span || !t.span) {
+            va_error("fftm", "a view of one launch spans 2 GiB or more or has a negative stride", -1);
+            return -1;
+        }
         va_desc_common(&d[1], p, sB, VHIP_C2C, mN * n1, VHIP_MAP_ALONG, VHIP_MAP_ACROSS);
         d[1].in = t;
         d[1].out = go;
@@ -376,17 +390,89 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
     }
 }
 
+/* vectors per launch that keep every view of the launch within 2 GiB of its
+ * base (32-bit buffer offsets in the kernels); 0 when not even one fits */
+static long va_exec_chunk(const VA_fftm *p, const VA_mview *in, const VA_mview *out)
+{
+    const long lim = (1L << 31) - 64;
+    long chunk = (long)p->mN, es, vs, len, esz, c;
+    vsip_length n, nv;
+    int k;
+    for (k = 0; k < 2; k++) {
+        const VA_mview *v = k ? out : in;
+        int real_side = (p->type == VA_RCFFTOP && k == 0) || (p->type == VA_CRFFTOP && k == 1);
+        va_axis(v, p->major, &es, &vs, &n, &nv);
+        len = real_side ? 2L * (long)p->MN : (p->type == VA_RCFFTOP || p->type == VA_CRFFTOP) ? (long)p->MN + 1
+                                                                                              : (long)p->MN;
+        esz = (!real_side && !v->block->split) ? 2L * p->prec : (long)p->prec;
+        if ((len - 1) * es + 1 > lim / esz) return 0;
+        c = vs > 0 ? (lim / esz - 1 - (len - 1) * es) / vs + 1 : chunk;
+        if (c < chunk) chunk = c;
+    }
+    if (p->npass == 2) {                        /* the split's scratch rows */
+        c = lim / ((long)p->MN * 2L * p->prec);
+        if (c < chunk) chunk = c;
+    }
+    return chunk;
+}
+
+/* a view with a negative stride */
+static int va_view_negative(const VA_mview *v)
+{
+    return v->row_stride < 0 || v->col_stride < 0;
+}
+
+/* views the launches cannot address from one base (a negative stride, or a
+ * single vector spanning 2 GiB): the plan's transform staged through
+ * contiguous scratch rows, with a temporary staged plan of the same shape */
+static int va_exec_via_copy(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void *stream)
+{
+    VA_fftm t;
+    int rc;
+    if (va_fftm_init2(&t, p->prec, p->type, p->M, p->N, p->scale, p->dir, p->major, p->ntimes, p->hint, 2)) {
+        va_error("fftm", "no staged plan for this view", -1);
+        return -1;
+    }
+    rc = va_staged_exec(&t, in, out, stream);
+    va_fftm_release(&t);             /* synchronises before freeing */
+    return rc;
+}
+
 int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void *stream)
 {
     vhip_fftm_desc d[2];
     int n, k, rc = 0;
+    long chunk, v0, mN;
     if (p && p->mN == 0) return 0;
-    if (p && p->markings == VA_VALID && p->staged) return va_staged_exec(p, in, out, stream);
-    n = va_fftm_descs(p, in, out, NULL, NULL, NULL, NULL, d);
-    if (n < 0) return -1;
-    for (k = 0; k < n && !rc; k++) {
-        rc = vhip_fftm_launch(&d[k], stream);
-        if (rc) va_error("fftm", vhip_error_string(rc), rc);
+    if (!p || p->markings != VA_VALID || !in || !out || !in->block || !out->block) {
+        va_error("fftm", "invalid plan or view", -1);
+        return -1;
+    }
+    if (p->staged) return va_staged_exec(p, in, out, stream);
+    if (va_view_negative(in) || va_view_negative(out)) return va_exec_via_copy(p, in, out, stream);
+    mN = (long)p->mN;
+    chunk = va_exec_chunk(p, in, out);
+    if (chunk < 1) return va_exec_via_copy(p, in, out, stream);
+    for (v0 = 0; v0 < mN && !rc; v0 += chunk) {
+        long cnt = mN - v0 < chunk ? mN - v0 : chunk;
+        VA_fftm q = *p;
+        VA_mview vi = *in, vo = *out;
+        q.mN = (vsip_length)cnt;
+        if (p->major == VSIP_ROW) {
+            vi.offset += (vsip_offset)(v0 * in->col_stride);
+            vo.offset += (vsip_offset)(v0 * out->col_stride);
+            vi.col_length = vo.col_length = (vsip_length)cnt;
+        } else {
+            vi.offset += (vsip_offset)(v0 * in->row_stride);
+            vo.offset += (vsip_offset)(v0 * out->row_stride);
+            vi.row_length = vo.row_length = (vsip_length)cnt;
+        }
+        n = va_fftm_descs(&q, &vi, &vo, NULL, NULL, NULL, NULL, d);
+        if (n < 0) return -1;
+        for (k = 0; k < n && !rc; k++) {
+            rc = vhip_fftm_launch(&d[k], stream);
+            if (rc) va_error("fftm", vhip_error_string(rc), rc);
+        }
     }
     return rc;
 }
