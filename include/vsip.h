@@ -2,7 +2,9 @@
  * vsip.h -- VSIPL 1.3 C API, multiple-FFT subset, for the MI355X implementation.
  *
  * Types, enum values and prototypes keep the VSIPL C API surface of the
- * reference header c_VSIP_src/vsip.h so that callers recompile unchanged:
+ * reference header c_VSIP_src/vsip.h so that callers recompile unchanged
+ * (the multiple FFT, its storage calls, and the 1-D FFT with its vector
+ * views):
  *   scalar typedefs            vsip.h:124-135
  *   enums                      vsip.h:138-153  (vsip_memory_hint, vsip_major,
  *                                               vsip_fft_dir, vsip_fft_place,
@@ -67,6 +69,12 @@ typedef struct vsip_cmviewattributes_d vsip_cmview_d;
 typedef struct vsip_cmviewattributes_f vsip_cmview_f;
 typedef struct vsip_fftmattributes_f vsip_fftm_f;
 typedef struct vsip_fftmattributes_d vsip_fftm_d;
+typedef struct vsip_vviewattributes_d vsip_vview_d;
+typedef struct vsip_vviewattributes_f vsip_vview_f;
+typedef struct vsip_cvviewattributes_d vsip_cvview_d;
+typedef struct vsip_cvviewattributes_f vsip_cvview_f;
+typedef struct vsip_fftattributes_f vsip_fft_f;
+typedef struct vsip_fftattributes_d vsip_fft_d;
 
 /* ---- attribute structs (vsip.h:242-245, 277-278) -------------------- */
 typedef struct { vsip_offset offset; vsip_stride row_stride; vsip_length row_length;
@@ -81,6 +89,15 @@ typedef struct { vsip_scalar_mi input; vsip_scalar_mi output; vsip_fft_place pla
                  vsip_scalar_d scale; vsip_fft_dir dir; vsip_major major; } vsip_fftm_attr_d;
 typedef struct { vsip_scalar_mi input; vsip_scalar_mi output; vsip_fft_place place;
                  vsip_scalar_f scale; vsip_fft_dir dir; vsip_major major; } vsip_fftm_attr_f;
+/* vsip.h:231-234, 275-276 */
+typedef struct { vsip_offset offset; vsip_stride stride; vsip_length length; vsip_block_d* block; } vsip_vattr_d;
+typedef struct { vsip_offset offset; vsip_stride stride; vsip_length length; vsip_block_f* block; } vsip_vattr_f;
+typedef struct { vsip_offset offset; vsip_stride stride; vsip_length length; vsip_cblock_d* block; } vsip_cvattr_d;
+typedef struct { vsip_offset offset; vsip_stride stride; vsip_length length; vsip_cblock_f* block; } vsip_cvattr_f;
+typedef struct { vsip_scalar_vi input; vsip_scalar_vi output; vsip_fft_place place;
+                 vsip_scalar_d scale; vsip_fft_dir dir; } vsip_fft_attr_d;
+typedef struct { vsip_scalar_vi input; vsip_scalar_vi output; vsip_fft_place place;
+                 vsip_scalar_f scale; vsip_fft_dir dir; } vsip_fft_attr_f;
 
 /* ---- library lifecycle ---------------------------------------------- */
 int vsip_init(void *ptr);                                         /* vsip.h:432 */
@@ -222,6 +239,79 @@ int vsip_fftm_destroy_f(vsip_fftm_f*);                                          
 int vsip_fftm_destroy_d(vsip_fftm_d*);                                                /* vsip.h:425 */
 void vsip_fftm_getattr_f(const vsip_fftm_f*, vsip_fftm_attr_f*);                      /* vsip.h:947 */
 void vsip_fftm_getattr_d(const vsip_fftm_d*, vsip_fftm_attr_d*);                      /* vsip.h:946 */
+
+/* ---- vector views (the 1-D FFT's operands) ----------------------------- */
+vsip_vview_f* vsip_vbind_f(const vsip_block_f*, vsip_offset, vsip_stride, vsip_length); /* vsip.h:491 */
+vsip_vview_d* vsip_vbind_d(const vsip_block_d*, vsip_offset, vsip_stride, vsip_length); /* vsip.h:503 */
+vsip_vview_f* vsip_vcreate_f(vsip_length, vsip_memory_hint);                          /* vsip.h:488 */
+vsip_vview_d* vsip_vcreate_d(vsip_length, vsip_memory_hint);                          /* vsip.h:501 */
+vsip_block_f* vsip_vdestroy_f(vsip_vview_f*);                                         /* vsip.h:304 */
+vsip_block_d* vsip_vdestroy_d(vsip_vview_d*);                                         /* vsip.h:296 */
+void vsip_valldestroy_f(vsip_vview_f*);                                               /* vsip.h:1292 */
+void vsip_valldestroy_d(vsip_vview_d*);                                               /* vsip.h:1291 */
+vsip_scalar_f vsip_vget_f(const vsip_vview_f*, vsip_index);                           /* vsip.h:638 */
+vsip_scalar_d vsip_vget_d(const vsip_vview_d*, vsip_index);                           /* vsip.h:605 */
+void vsip_vput_f(const vsip_vview_f*, vsip_index, vsip_scalar_f);                     /* vsip.h:1556 */
+void vsip_vput_d(const vsip_vview_d*, vsip_index, vsip_scalar_d);                     /* vsip.h:1555 */
+void vsip_vgetattrib_f(const vsip_vview_f*, vsip_vattr_f*);                           /* vsip.h:1422 */
+void vsip_vgetattrib_d(const vsip_vview_d*, vsip_vattr_d*);                           /* vsip.h:1421 */
+void vsip_vcopyfrom_user_f(vsip_scalar_f* const, const vsip_vview_f*);                /* vsip.h:1378 */
+void vsip_vcopyfrom_user_d(vsip_scalar_d* const, const vsip_vview_d*);                /* vsip.h:1377 */
+void vsip_vcopyto_user_f(const vsip_vview_f*, vsip_scalar_f* const);                  /* vsip.h:1384 */
+void vsip_vcopyto_user_d(const vsip_vview_d*, vsip_scalar_d* const);                  /* vsip.h:1383 */
+
+vsip_cvview_f* vsip_cvbind_f(const vsip_cblock_f*, vsip_offset, vsip_stride, vsip_length); /* vsip.h:514 */
+vsip_cvview_d* vsip_cvbind_d(const vsip_cblock_d*, vsip_offset, vsip_stride, vsip_length); /* vsip.h:517 */
+vsip_cvview_f* vsip_cvcreate_f(vsip_length, vsip_memory_hint);                        /* vsip.h:512 */
+vsip_cvview_d* vsip_cvcreate_d(vsip_length, vsip_memory_hint);                        /* vsip.h:515 */
+vsip_cblock_f* vsip_cvdestroy_f(vsip_cvview_f*);                                      /* vsip.h:360 */
+vsip_cblock_d* vsip_cvdestroy_d(vsip_cvview_d*);                                      /* vsip.h:352 */
+void vsip_cvalldestroy_f(vsip_cvview_f*);                                             /* vsip.h:865 */
+void vsip_cvalldestroy_d(vsip_cvview_d*);                                             /* vsip.h:864 */
+vsip_cscalar_f vsip_cvget_f(const vsip_cvview_f*, vsip_index);                        /* vsip.h:568 */
+vsip_cscalar_d vsip_cvget_d(const vsip_cvview_d*, vsip_index);                        /* vsip.h:539 */
+void vsip_cvput_f(const vsip_cvview_f*, vsip_index, vsip_cscalar_f);                  /* vsip.h:915 */
+void vsip_cvput_d(const vsip_cvview_d*, vsip_index, vsip_cscalar_d);                  /* vsip.h:914 */
+void vsip_cvgetattrib_f(const vsip_cvview_f*, vsip_cvattr_f*);                        /* vsip.h:889 */
+void vsip_cvgetattrib_d(const vsip_cvview_d*, vsip_cvattr_d*);                        /* vsip.h:888 */
+void vsip_cvcopyfrom_user_f(vsip_scalar_f* const, vsip_scalar_f* const,
+                            const vsip_cvview_f*);                                    /* vsip.h:875 */
+void vsip_cvcopyfrom_user_d(vsip_scalar_d* const, vsip_scalar_d* const,
+                            const vsip_cvview_d*);                                    /* vsip.h:874 */
+void vsip_cvcopyto_user_f(const vsip_cvview_f*, vsip_scalar_f* const,
+                          vsip_scalar_f* const);                                      /* vsip.h:877 */
+void vsip_cvcopyto_user_d(const vsip_cvview_d*, vsip_scalar_d* const,
+                          vsip_scalar_d* const);                                      /* vsip.h:876 */
+
+/* ---- 1-D FFT ------------------------------------------------------------ */
+vsip_fft_f* vsip_ccfftip_create_f(vsip_length N, vsip_scalar_f scale, vsip_fft_dir dir,
+                                  unsigned int ntimes, vsip_alg_hint hint);           /* vsip.h:1805 */
+vsip_fft_d* vsip_ccfftip_create_d(vsip_length N, vsip_scalar_d scale, vsip_fft_dir dir,
+                                  unsigned int ntimes, vsip_alg_hint hint);           /* vsip.h:1801 */
+vsip_fft_f* vsip_ccfftop_create_f(vsip_length N, vsip_scalar_f scale, vsip_fft_dir dir,
+                                  unsigned int ntimes, vsip_alg_hint hint);           /* vsip.h:1806 */
+vsip_fft_d* vsip_ccfftop_create_d(vsip_length N, vsip_scalar_d scale, vsip_fft_dir dir,
+                                  unsigned int ntimes, vsip_alg_hint hint);           /* vsip.h:1802 */
+vsip_fft_f* vsip_rcfftop_create_f(vsip_length N, vsip_scalar_f scale, unsigned int ntimes,
+                                  vsip_alg_hint hint);                                /* vsip.h:1808 */
+vsip_fft_d* vsip_rcfftop_create_d(vsip_length N, vsip_scalar_d scale, unsigned int ntimes,
+                                  vsip_alg_hint hint);                                /* vsip.h:1804 */
+vsip_fft_f* vsip_crfftop_create_f(vsip_length N, vsip_scalar_f scale, unsigned int ntimes,
+                                  vsip_alg_hint hint);                                /* vsip.h:1807 */
+vsip_fft_d* vsip_crfftop_create_d(vsip_length N, vsip_scalar_d scale, unsigned int ntimes,
+                                  vsip_alg_hint hint);                                /* vsip.h:1803 */
+void vsip_ccfftip_f(const vsip_fft_f*, const vsip_cvview_f*);                         /* vsip.h:713 */
+void vsip_ccfftip_d(const vsip_fft_d*, const vsip_cvview_d*);                         /* vsip.h:712 */
+void vsip_ccfftop_f(const vsip_fft_f*, const vsip_cvview_f*, const vsip_cvview_f*);   /* vsip.h:719 */
+void vsip_ccfftop_d(const vsip_fft_d*, const vsip_cvview_d*, const vsip_cvview_d*);   /* vsip.h:718 */
+void vsip_rcfftop_f(const vsip_fft_f*, const vsip_vview_f*, const vsip_cvview_f*);    /* vsip.h:1186 */
+void vsip_rcfftop_d(const vsip_fft_d*, const vsip_vview_d*, const vsip_cvview_d*);    /* vsip.h:1185 */
+void vsip_crfftop_f(const vsip_fft_f*, const vsip_cvview_f*, const vsip_vview_f*);    /* vsip.h:829 */
+void vsip_crfftop_d(const vsip_fft_d*, const vsip_cvview_d*, const vsip_vview_d*);    /* vsip.h:828 */
+int vsip_fft_destroy_f(vsip_fft_f*);                                                  /* vsip.h:424 */
+int vsip_fft_destroy_d(vsip_fft_d*);                                                  /* vsip.h:423 */
+void vsip_fft_getattr_f(const vsip_fft_f*, vsip_fft_attr_f*);                         /* vsip.h:945 */
+void vsip_fft_getattr_d(const vsip_fft_d*, vsip_fft_attr_d*);                         /* vsip.h:944 */
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,11 @@ struct vsip_mviewattributes_f { VA_mview v; };
 struct vsip_mviewattributes_d { VA_mview v; };
 struct vsip_cmviewattributes_f { VA_mview v; };
 struct vsip_cmviewattributes_d { VA_mview v; };
+/* a vector view is the 1 x length matrix view (offset, stride, length) */
+struct vsip_vviewattributes_f { VA_mview v; };
+struct vsip_vviewattributes_d { VA_mview v; };
+struct vsip_cvviewattributes_f { VA_mview v; };
+struct vsip_cvviewattributes_d { VA_mview v; };
 
 /* ---- FFT schedules ------------------------------------------------------ */
 
@@ -126,6 +131,9 @@ typedef struct VA_fftm {
 
 struct vsip_fftmattributes_f { VA_fftm p; };
 struct vsip_fftmattributes_d { VA_fftm p; };
+/* a 1-D FFT is the multiple FFT of one ROW vector */
+struct vsip_fftattributes_f { VA_fftm p; };
+struct vsip_fftattributes_d { VA_fftm p; };
 
 /* Workgroup shape of the batched (persistent) kernel: every transform of a
  * batch runs with these lanes, register class and LDS bytes per workgroup;
@@ -153,6 +161,8 @@ VA_mview *va_mview_bind(VA_block *b, vsip_offset o, vsip_stride cs, vsip_length 
 void va_mview_destroy(VA_mview *v);
 int va_elem_dev(const VA_mview *v, vsip_index i, vsip_index j, void **r, void **im);
 int va_mview_copy_host(const VA_mview *v, int to_device, void *r, void *im, vsip_major major);
+void va_get(const VA_mview *v, vsip_index i, vsip_index j, void *re, void *im);
+void va_put(const VA_mview *v, vsip_index i, vsip_index j, const void *re, const void *im);
 
 /* planning and execution */
 int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max);

@@ -88,7 +88,7 @@ int va_elem_dev(const VA_mview *v, vsip_index i, vsip_index j, void **r, void **
     return va_elem_addr(v, i, j, r, im, &dev);
 }
 
-static void va_get(const VA_mview *v, vsip_index i, vsip_index j, void *re, void *im)
+void va_get(const VA_mview *v, vsip_index i, vsip_index j, void *re, void *im)
 {
     void *pr, *pi;
     int dev, rc = 0;
@@ -111,7 +111,7 @@ static void va_get(const VA_mview *v, vsip_index i, vsip_index j, void *re, void
     }
 }
 
-static void va_put(const VA_mview *v, vsip_index i, vsip_index j, const void *re, const void *im)
+void va_put(const VA_mview *v, vsip_index i, vsip_index j, const void *re, const void *im)
 {
     void *pr, *pi;
     int dev, rc = 0;

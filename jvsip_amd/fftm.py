@@ -180,3 +180,83 @@ def crfftm(lib, zblk, zview, xblk, xview, major=VSIP_ROW, scale=1.0):
     zb.destroy()
     xb.destroy()
     return res
+
+
+# ---- 1-D FFT (vsip_ccfftip / ccfftop / rcfftop / crfftop) -----------------
+
+View1 = namedtuple("View1", "offset stride length")
+
+
+def as_matrix(v):
+    """The 1 x length matrix view of vector view v (for the oracle)."""
+    return View(v.offset, v.length * v.stride, 1, v.stride, v.length)
+
+
+def ccfft(lib, blk, view, direction=VSIP_FFT_FWD, scale=1.0, out_blk=None, out_view=None, split=False):
+    """1-D complex FFT of vector view `view` of block `blk` (in place when
+    out_blk is None); returns the block(s) after release like ccfftm."""
+    t = _t(blk.real.dtype.itemsize)
+    b = _CBlock(lib, blk, split)
+    v = getattr(lib, "cvbind_" + t)(b.h, *view)
+    if out_blk is None:
+        plan = getattr(lib, "ccfftip_create_" + t)(view.length, scale, direction, 0, VSIP_ALG_TIME)
+        assert plan, "ccfftip_create failed"
+        getattr(lib, "ccfftip_" + t)(plan, v)
+        res = b.release()
+        getattr(lib, "fft_destroy_" + t)(plan)
+        getattr(lib, "cvdestroy_" + t)(v)
+        b.destroy()
+        return res
+    ob = _CBlock(lib, out_blk, split)
+    ov = getattr(lib, "cvbind_" + t)(ob.h, *out_view)
+    plan = getattr(lib, "ccfftop_create_" + t)(view.length, scale, direction, 0, VSIP_ALG_TIME)
+    assert plan, "ccfftop_create failed"
+    getattr(lib, "ccfftop_" + t)(plan, v, ov)
+    res_x = b.release()
+    res_y = ob.release()
+    getattr(lib, "fft_destroy_" + t)(plan)
+    getattr(lib, "cvdestroy_" + t)(v)
+    getattr(lib, "cvdestroy_" + t)(ov)
+    b.destroy()
+    ob.destroy()
+    return res_x, res_y
+
+
+def rcfft(lib, xblk, xview, zblk, zview, scale=1.0):
+    """vsip_rcfftop: real vector -> complex vector; returns the z block."""
+    t = _t(xblk.dtype.itemsize)
+    xb = _RBlock(lib, xblk)
+    zb = _CBlock(lib, zblk)
+    xv = getattr(lib, "vbind_" + t)(xb.h, *xview)
+    zv = getattr(lib, "cvbind_" + t)(zb.h, *zview)
+    plan = getattr(lib, "rcfftop_create_" + t)(xview.length, scale, 0, VSIP_ALG_TIME)
+    assert plan, "rcfftop_create failed"
+    getattr(lib, "rcfftop_" + t)(plan, xv, zv)
+    xb.release()
+    res = zb.release()
+    getattr(lib, "fft_destroy_" + t)(plan)
+    getattr(lib, "vdestroy_" + t)(xv)
+    getattr(lib, "cvdestroy_" + t)(zv)
+    xb.destroy()
+    zb.destroy()
+    return res
+
+
+def crfft(lib, zblk, zview, xblk, xview, scale=1.0):
+    """vsip_crfftop: complex vector -> real vector; returns the x block."""
+    t = _t(xblk.dtype.itemsize)
+    zb = _CBlock(lib, zblk)
+    xb = _RBlock(lib, xblk)
+    zv = getattr(lib, "cvbind_" + t)(zb.h, *zview)
+    xv = getattr(lib, "vbind_" + t)(xb.h, *xview)
+    plan = getattr(lib, "crfftop_create_" + t)(xview.length, scale, 0, VSIP_ALG_TIME)
+    assert plan, "crfftop_create failed"
+    getattr(lib, "crfftop_" + t)(plan, zv, xv)
+    zb.release()
+    res = xb.release()
+    getattr(lib, "fft_destroy_" + t)(plan)
+    getattr(lib, "cvdestroy_" + t)(zv)
+    getattr(lib, "vdestroy_" + t)(xv)
+    zb.destroy()
+    xb.destroy()
+    return res

@@ -60,6 +60,18 @@ class amd_result(ctypes.Structure):
                 ("stream", ctypes.c_int)]
 
 
+class fft_attr_f(ctypes.Structure):
+    _fields_ = [("input", c_vi), ("output", c_vi), ("place", c_bl), ("scale", ctypes.c_float), ("dir", c_bl)]
+
+
+class fft_attr_d(ctypes.Structure):
+    _fields_ = [("input", c_vi), ("output", c_vi), ("place", c_bl), ("scale", ctypes.c_double), ("dir", c_bl)]
+
+
+class vattr(ctypes.Structure):
+    _fields_ = [("offset", c_vi), ("stride", c_st), ("length", c_vi), ("block", P)]
+
+
 class cmattr(ctypes.Structure):
     _fields_ = [("offset", c_vi), ("row_stride", c_st), ("row_length", c_vi),
                 ("col_stride", c_st), ("col_length", c_vi), ("block", P)]
@@ -71,6 +83,7 @@ def _sig(t):
     ps = ctypes.POINTER(sc)
     cs = cscalar_f if t == "f" else cscalar_d
     attr = fftm_attr_f if t == "f" else fftm_attr_d
+    attr1 = fft_attr_f if t == "f" else fft_attr_d
     return {
         "blockbind": (P, [P, c_vi, c_bl]),
         "blockcreate": (P, [c_vi, c_bl]),
@@ -114,6 +127,35 @@ def _sig(t):
         "fftm_getattr": (None, [P, ctypes.POINTER(attr)]),
         "cmsub": (None, [P, P, P]),
         "cmmeansqval": (sc, [P]),
+        # vector views and the 1-D FFT
+        "vbind": (P, [P, c_vi, c_st, c_vi]),
+        "vcreate": (P, [c_vi, c_bl]),
+        "vdestroy": (P, [P]),
+        "valldestroy": (None, [P]),
+        "vget": (sc, [P, c_vi]),
+        "vput": (None, [P, c_vi, sc]),
+        "vgetattrib": (None, [P, ctypes.POINTER(vattr)]),
+        "vcopyfrom_user": (None, [P, P]),
+        "vcopyto_user": (None, [P, P]),
+        "cvbind": (P, [P, c_vi, c_st, c_vi]),
+        "cvcreate": (P, [c_vi, c_bl]),
+        "cvdestroy": (P, [P]),
+        "cvalldestroy": (None, [P]),
+        "cvget": (cs, [P, c_vi]),
+        "cvput": (None, [P, c_vi, cs]),
+        "cvgetattrib": (None, [P, ctypes.POINTER(vattr)]),
+        "cvcopyfrom_user": (None, [P, P, P]),
+        "cvcopyto_user": (None, [P, P, P]),
+        "ccfftip_create": (P, [c_vi, sc, c_bl, ctypes.c_uint, c_bl]),
+        "ccfftop_create": (P, [c_vi, sc, c_bl, ctypes.c_uint, c_bl]),
+        "rcfftop_create": (P, [c_vi, sc, ctypes.c_uint, c_bl]),
+        "crfftop_create": (P, [c_vi, sc, ctypes.c_uint, c_bl]),
+        "ccfftip": (None, [P, P]),
+        "ccfftop": (None, [P, P, P]),
+        "rcfftop": (None, [P, P, P]),
+        "crfftop": (None, [P, P, P]),
+        "fft_destroy": (ctypes.c_int, [P]),
+        "fft_getattr": (None, [P, ctypes.POINTER(attr1)]),
     }
 
 

@@ -205,3 +205,45 @@ def test_oracle_rc_vs_reference_library():
     xv, zv = fftm.dense_view(m, n), fftm.dense_view(m, n // 2 + 1)
     z0 = np.zeros(m * (n // 2 + 1), np.complex64)
     assert po.rel_l2(fftm.rcfftm(po.ref(), x.ravel(), xv, z0, zv), po.rcfftm(x.ravel(), xv, z0, zv)) < 1e-6
+
+
+@pytest.mark.parametrize("n", [8, 22, 24, 1000, 1001, 4096])
+def test_oracle_pins_reference_1d_fft(n):
+    """jvsip's 1-D FFT (vsip_ccfftip/op, rcfftop, crfftop) equals the oracle's
+    multiple FFT of the 1 x n view: the 1-D path is the multiple FFT of one
+    vector, including crfftop's literal pre-combine on non-Hermitian input."""
+    if not po.ref_available():
+        pytest.skip("oracle/_ref not built")
+    ref = po.ref()
+    x = datagen.complex_data(50 + n, (n,))
+    v1 = fftm.View1(0, 1, n)
+    mv = fftm.as_matrix(v1)
+    for d, s in ((-1, 1.0), (1, 0.25)):
+        a = fftm.ccfft(ref, x.copy(), v1, direction=d, scale=s)
+        assert po.rel_l2(a, po.ccfftm(x, mv, 0, d, s)) < 1e-5
+        _, y = fftm.ccfft(ref, x.copy(), v1, direction=d, scale=s, out_blk=np.zeros(n, np.complex64),
+                          out_view=v1)
+        assert po.rel_l2(y, po.ccfftm(x, mv, 0, d, s)) < 1e-5
+    if n % 2 == 0:
+        h = n // 2 + 1
+        r = datagen.real_data(60 + n, (n,))
+        z = fftm.rcfft(ref, r, fftm.View1(0, 1, n), np.zeros(h, np.complex64), fftm.View1(0, 1, h), scale=0.5)
+        want = po.rcfftm(r, fftm.dense_view(1, n), np.zeros(h, np.complex64), fftm.dense_view(1, h), scale=0.5)
+        assert po.rel_l2(z, want) < 1e-5
+        zz = datagen.complex_data(70 + n, (h,))
+        xb = fftm.crfft(ref, zz, fftm.View1(0, 1, h), np.zeros(n, np.float32), fftm.View1(0, 1, n), scale=0.5)
+        want = po.crfftm(zz, fftm.dense_view(1, h), np.zeros(n), fftm.dense_view(1, n), scale=0.5)
+        assert po.rel_l2(xb, want) < 1e-5
+
+
+@pytest.mark.parametrize("n", [8, 16, 22, 24])
+def test_ramp_closed_form_of_reference_1d_test(n):
+    """The check of c_VSIP_testing/rcfftop_f.h:40-60, restated: the ramp
+    x_j = j with scale 1/n transforms to X_0 = (n-1)/2, X_{n/2} = -1/2,
+    X_k = -1/2 + i/2 cot(pi k / n) (the conjugate of the test's table, which
+    it conjugates before comparing); the oracle must reproduce it."""
+    k = np.arange(1, n // 2)
+    want = np.concatenate([[(n - 1) / 2.0], -0.5 + 0.5j / np.tan(np.pi * k / n), [-0.5]])
+    got = po.rcfftm(np.arange(n, dtype=np.float64), fftm.dense_view(1, n), np.zeros(n // 2 + 1),
+                    fftm.dense_view(1, n // 2 + 1), scale=1.0 / n)
+    np.testing.assert_allclose(got, want, atol=1e-12)
