@@ -115,4 +115,4 @@ def test_fft_getattr(amd):
         assert (a.input, a.output, a.place, a.dir) == want and abs(a.scale - 0.5) < 1e-7, create
         assert amd.fft_destroy_f(p) == 0
     assert not amd.rcfftop_create_f(23, 1.0, 0, 1)          # odd real length refused
-    amd.clear_error()
+    amd.amd_clear_error()
