@@ -428,6 +428,48 @@ def config_c4(lib, hip, njobs=3000):
     return out
 
 
+def config_fft1d(lib, hip):
+    """1-D vsip_ccfftip_f forward on single long vectors (the two-pass split
+    over one vector) and vsip_rcfftop_f on a long real vector"""
+    from jvsip_amd import datagen
+    out = {}
+    for logn in (16, 20, 24):
+        n = 1 << logn
+        x = datagen.complex_data(6, (n,), dist="uniform")
+        v = lib.cvcreate_f(n, 0)
+        lib.cvcopyfrom_user_f(x.ctypes.data, None, v)
+        p = lib.ccfftip_create_f(n, 1.0, -1, 0, 1)
+        assert v and p, lib.last_error()
+        _, t = _time_calls(lib, hip, lambda: lib.ccfftip_f(p, v), 20)
+        y = np.empty(n, np.complex64)
+        lib.ccfftip_f(p, v)
+        lib.cvcopyfrom_user_f(x.ctypes.data, None, v)
+        lib.ccfftip_f(p, v)
+        lib.cvcopyto_user_f(v, y.ctypes.data, None)
+        want = np.fft.fft(x.astype(np.complex128))
+        out["ccfftip_2^%d" % logn] = {"us": round(t * 1e6, 2), "gb_per_s": round(2 * n * 8 / t / 1e9, 1),
+                                       "rel_l2_vs_numpy_fp64": float(np.linalg.norm(y - want) / np.linalg.norm(want)),
+                                       "schedule": [q["n"] for q in json.loads(lib.describe(p))["passes"]]}
+        lib.fft_destroy_f(p)
+        lib.cvalldestroy_f(v)
+    n = 1 << 22
+    r = datagen.real_data(6, (n,))
+    xv = lib.vcreate_f(n, 0)
+    zv = lib.cvcreate_f(n // 2 + 1, 0)
+    lib.vcopyfrom_user_f(r.ctypes.data, xv)
+    p = lib.rcfftop_create_f(n, 1.0, 0, 1)
+    _, t = _time_calls(lib, hip, lambda: lib.rcfftop_f(p, xv, zv), 20)
+    z = np.empty(n // 2 + 1, np.complex64)
+    lib.cvcopyto_user_f(zv, z.ctypes.data, None)
+    want = np.fft.rfft(r.astype(np.float64))
+    out["rcfftop_2^22"] = {"us": round(t * 1e6, 2), "gb_per_s": round((n * 4 + (n // 2 + 1) * 8) / t / 1e9, 1),
+                           "rel_l2_vs_numpy_fp64": float(np.linalg.norm(z - want) / np.linalg.norm(want))}
+    lib.fft_destroy_f(p)
+    lib.valldestroy_f(xv)
+    lib.cvalldestroy_f(zv)
+    return out
+
+
 def config_c5(lib, nreq=2000, rate=20000.0):
     """Poisson requests at `rate`/s: rows U[1,512], lengths {256, 1000, 1024,
     2187, 4096, 16384}, fwd/inv 50/50, N(0,1) inputs from a device pool,
@@ -561,7 +603,8 @@ def main():
                          ("c2_real_8192x2048", lambda: config_c2_real(lib, hip)),
                          ("c3_strided_mixed_fp64", lambda: config_c3(lib, hip)),
                          ("c4_batched_uneven", lambda: config_c4(lib, hip)),
-                         ("c5_served", lambda: config_c5(lib))):
+                         ("c5_served", lambda: config_c5(lib)),
+                         ("f1_fft1d", lambda: config_fft1d(lib, hip))):
             if args.only and not any(name.startswith(o.strip()) for o in args.only.split(",") if o.strip()):
                 continue
             t0 = time.perf_counter()

@@ -610,10 +610,22 @@ __device__ __forceinline__ void fft_body(const vhip_fftm_desc &d, long vbase, in
 
 // The 32-element register class runs at most 512 lanes per workgroup, which
 // leaves it 256 registers per lane; the smaller classes allow 1024 lanes.
+// Workgroups are dealt to the 8 XCDs of the device in turn (workgroup b runs
+// on XCD b % 8).  Renumbering them so that consecutive vector groups land on
+// one XCD lets the groups that read neighbouring columns of the same cache
+// lines (unit vector stride: fewer than 16 vectors per workgroup cover less
+// than a 128-byte line) share that XCD's L2 instead of fetching the line once
+// per XCD.
+__device__ __forceinline__ unsigned xcd_group(unsigned b, unsigned g) {
+    const unsigned x = b & 7u, i = b >> 3, per = g >> 3, rem = g & 7u;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
 template <typename T, int MAXE, int KIND>
 __global__ void __launch_bounds__(MAXE > 16 ? 512 : 1024) fftm_kernel(const vhip_fftm_desc d) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    const long vbase = (long)blockIdx.x * d.vpb;
+    const bool across = d.map_in == VHIP_MAP_ACROSS || d.map_out == VHIP_MAP_ACROSS;
+    const long vbase = (long)(across ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x) * d.vpb;
     long left = d.nvec - vbase;
     const int nv = left < d.vpb ? (int)left : d.vpb;
     fft_body<T, MAXE, KIND>(d, vbase, nv, smem);

@@ -219,9 +219,11 @@ int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max)
     if (va_pick_launch(s, prec, vpb_along, 65536, lds_max, &s->along)) return -2;
 
     /* across: 16 vectors give 128-byte (float) row segments per load when
-     * two such workgroups fit a CU; otherwise 8 (64-byte segments, measured
-     * faster than fewer vectors at two workgroups per CU) */
-    vpb_across = (va_lds_bytes(s, prec, 16) <= lds_max / 2) ? 16 : 8;
+     * two such workgroups fit a CU; otherwise 4, whose neighbouring groups
+     * share the remaining parts of each line through the XCD's L2 (the
+     * kernel places consecutive groups on one XCD; measured faster than 8
+     * vectors at one workgroup per CU) */
+    vpb_across = (va_lds_bytes(s, prec, 16) <= lds_max / 2) ? 16 : 4;
     vpb_across = va_env_cap("VSIP_AMD_VPB_ACROSS", vpb_across);
     if ((vsip_length)vpb_across > nvec && nvec > 0) vpb_across = (int)nvec;
     if (va_pick_launch(s, prec, vpb_across, lds_max, lds_max, &s->across)) return -2;

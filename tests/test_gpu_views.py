@@ -75,6 +75,7 @@ def test_view_spanning_more_than_2gib(amd):
     stride = (1 << 27) + 4096
     total = (m - 1) * stride + n
     _case("ccfftmip_f ROW %dx%d, rows %d elements apart (%.2f GiB span)" % (m, n, stride, total * 8 / 2 ** 30))
+    amd.amd_clear_error()                 # errors recorded by earlier tests
     b = amd.cblockcreate_f(total, 0)
     assert b, amd.last_error()
     x = datagen.complex_data(43, (m, n))
