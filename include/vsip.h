@@ -313,6 +313,20 @@ int vsip_fft_destroy_d(vsip_fft_d*);                                            
 void vsip_fft_getattr_f(const vsip_fft_f*, vsip_fft_attr_f*);                         /* vsip.h:945 */
 void vsip_fft_getattr_d(const vsip_fft_d*, vsip_fft_attr_d*);                         /* vsip.h:944 */
 
+/* ---- element operations on transform operands ---------------------------- */
+void vsip_rscvmul_f(vsip_scalar_f, const vsip_cvview_f*, const vsip_cvview_f*);        /* vsip.h:1216 */
+void vsip_rscvmul_d(vsip_scalar_d, const vsip_cvview_d*, const vsip_cvview_d*);        /* vsip.h:1215 */
+void vsip_rscmmul_f(vsip_scalar_f, const vsip_cmview_f*, const vsip_cmview_f*);        /* vsip.h:1208 */
+void vsip_rscmmul_d(vsip_scalar_d, const vsip_cmview_d*, const vsip_cmview_d*);        /* vsip.h:1207 */
+void vsip_cvsub_f(const vsip_cvview_f*, const vsip_cvview_f*, const vsip_cvview_f*);   /* vsip.h:941 */
+void vsip_cvsub_d(const vsip_cvview_d*, const vsip_cvview_d*, const vsip_cvview_d*);   /* vsip.h:940 */
+void vsip_cmsub_f(const vsip_cmview_f*, const vsip_cmview_f*, const vsip_cmview_f*);   /* vsip.h:805 */
+void vsip_cmsub_d(const vsip_cmview_d*, const vsip_cmview_d*, const vsip_cmview_d*);   /* vsip.h:804 */
+vsip_scalar_f vsip_cvmeansqval_f(const vsip_cvview_f*);                                /* vsip.h:619 */
+vsip_scalar_d vsip_cvmeansqval_d(const vsip_cvview_d*);                                /* vsip.h:586 */
+vsip_scalar_f vsip_cmmeansqval_f(const vsip_cmview_f*);                                /* vsip.h:618 */
+vsip_scalar_d vsip_cmmeansqval_d(const vsip_cmview_d*);                                /* vsip.h:585 */
+
 #ifdef __cplusplus
 }
 #endif

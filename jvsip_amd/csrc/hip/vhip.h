@@ -160,6 +160,17 @@ typedef struct {
     void *work;
 } vhip_xform_desc;
 
+/* Element operations on strided complex m x n matrices (vhip_mat), in the
+ * working precision `prec`:
+ *   scale   r = alpha * a (real alpha)
+ *   sub     c = a - b
+ *   sumsq   *out = sum |a_ij|^2, accumulated in double (synchronous) */
+int vhip_elem_scale(int prec, const vhip_mat *a, const vhip_mat *r, long m, long n, double alpha,
+                    void *stream);
+int vhip_elem_sub(int prec, const vhip_mat *a, const vhip_mat *b, const vhip_mat *c, long m, long n,
+                  void *stream);
+int vhip_elem_sumsq(int prec, const vhip_mat *a, long m, long n, double *out, void *stream);
+
 /* kernels */
 int vhip_xform_launch(const vhip_xform_desc *d, void *stream);
 int vhip_mat_copy(int prec, int cplx, const vhip_mat *src, const vhip_mat *dst,

@@ -162,6 +162,7 @@ void va_mview_destroy(VA_mview *v);
 int va_elem_dev(const VA_mview *v, vsip_index i, vsip_index j, void **r, void **im);
 int va_mview_copy_host(const VA_mview *v, int to_device, void *r, void *im, vsip_major major);
 void va_get(const VA_mview *v, vsip_index i, vsip_index j, void *re, void *im);
+void va_view_mat(const VA_mview *v, int dev, vhip_mat *m);
 void va_put(const VA_mview *v, vsip_index i, vsip_index j, const void *re, const void *im);
 
 /* planning and execution */

@@ -135,7 +135,7 @@ void va_put(const VA_mview *v, vsip_index i, vsip_index j, const void *re, const
 }
 
 /* matrix descriptor of a view over its block's current storage */
-static void va_view_mat(const VA_mview *v, int dev, vhip_mat *m)
+void va_view_mat(const VA_mview *v, int dev, vhip_mat *m)
 {
     VA_block *b = v->block;
     size_t p = (size_t)b->prec;

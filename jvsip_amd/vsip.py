@@ -95,6 +95,10 @@ def _sig(t):
         "cblockadmit": (ctypes.c_int, [P, c_bl]),
         "cblockrelease": (None, [P, c_bl, ctypes.POINTER(P), ctypes.POINTER(P)]),
         "cblockdestroy": (None, [P]),
+        "cblockfind": (None, [P, ctypes.POINTER(P), ctypes.POINTER(P)]),
+        "cblockrebind": (None, [P, P, P, ctypes.POINTER(P), ctypes.POINTER(P)]),
+        "blockfind": (P, [P]),
+        "blockrebind": (P, [P, P]),
         "cmbind": (P, [P, c_vi, c_st, c_vi, c_st, c_vi]),
         "cmcreate": (P, [c_vi, c_vi, c_bl, c_bl]),
         "cmsubview": (P, [P, c_vi, c_vi, c_vi, c_vi]),
@@ -127,6 +131,10 @@ def _sig(t):
         "fftm_getattr": (None, [P, ctypes.POINTER(attr)]),
         "cmsub": (None, [P, P, P]),
         "cmmeansqval": (sc, [P]),
+        "cvsub": (None, [P, P, P]),
+        "cvmeansqval": (sc, [P]),
+        "rscvmul": (None, [sc, P, P]),
+        "rscmmul": (None, [sc, P, P]),
         # vector views and the 1-D FFT
         "vbind": (P, [P, c_vi, c_st, c_vi]),
         "vcreate": (P, [c_vi, c_bl]),
