@@ -21,8 +21,10 @@
 // e^{-/+ i pi k / n} (behaviour of VI_rcfftmop_f_def.h:77-115 and
 // VI_crfftmop_f_def.h:74-124).
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 #include "vhip.h"
 #include "dft_consts.h"
 

@@ -1,0 +1,178 @@
+// fftm_jit.hip -- kernels compiled at plan creation for one schedule.
+//
+// A plan whose schedule the generic kernel would run (mixed radix, contiguous
+// rows, single pass) gets the same Stockham schedule compiled with every
+// constant fixed (fftm_static.hpp): hipRTC compiles the template instance
+// named by the schedule for the device's architecture, the code object is
+// loaded as a module, and the function is kept in a process-wide cache keyed
+// by the schedule, so later plans of the same schedule and every execution
+// find it without compiling.  The device headers travel inside the library
+// (gen_jit_sources.py).  A schedule that does not compile or load is
+// remembered as unavailable and runs on the generic kernel.
+//
+// VSIP_AMD_JIT=0 turns the path off.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "vhip.h"
+
+extern "C" {
+extern const int vhip_jit_nheaders;
+extern const char *const vhip_jit_header_names[];
+extern const char *const vhip_jit_header_texts[];
+}
+
+namespace {
+
+struct Entry {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;   // nullptr: the schedule is not available
+};
+
+std::mutex g_mu;
+std::map<std::string, Entry> g_cache;
+char g_err[512];
+
+bool jit_on() {
+    const char *e = getenv("VSIP_AMD_JIT");
+    return !(e && e[0] == '0');
+}
+
+// the template instance of a schedule; empty when the schedule has no
+// compiled form (more than 8 stages, or not a single pass)
+std::string instance(int n, int nst, const int *radix, int vpb, int threads) {
+    if (nst < 1 || nst > 8) return "";
+    int r[8];
+    for (int s = 0; s < 8; ++s) r[s] = s < nst ? radix[s] : 1;
+    char buf[256];
+    snprintf(buf, sizeof buf, "vhipk::st::fftm_static_kernel<%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d>", n, vpb,
+             threads, nst, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]);
+    return buf;
+}
+
+std::string arch_option() {
+    std::string a = "gfx950";
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) {
+        a = p.gcnArchName;
+        size_t c = a.find(':');
+        if (c != std::string::npos) a = a.substr(0, c);
+    }
+    return "--offload-arch=" + a;
+}
+
+// compile `name`; code object into `code`, lowered name into `lowered`
+int compile(const std::string &name, const std::string &arch, std::vector<char> &code, std::string &lowered) {
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, "#include \"fftm_static.hpp\"\n", "fftm_jit.hip", vhip_jit_nheaders,
+                            vhip_jit_header_texts, vhip_jit_header_names) != HIPRTC_SUCCESS)
+        return -1;
+    hiprtcAddNameExpression(prog, name.c_str());
+    const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    hiprtcResult r = hiprtcCompileProgram(prog, 7, opts);
+    int rc = 0;
+    if (r != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::vector<char> log(ls + 1, 0);
+        hiprtcGetProgramLog(prog, log.data());
+        snprintf(g_err, sizeof g_err, "%s: %.400s", hiprtcGetErrorString(r), log.data());
+        rc = -2;
+    } else {
+        const char *low = nullptr;
+        size_t cs = 0;
+        if (hiprtcGetLoweredName(prog, name.c_str(), &low) != HIPRTC_SUCCESS || !low ||
+            hiprtcGetCodeSize(prog, &cs) != HIPRTC_SUCCESS || cs == 0) {
+            rc = -3;
+        } else {
+            lowered = low;
+            code.resize(cs);
+            if (hiprtcGetCode(prog, code.data()) != HIPRTC_SUCCESS) rc = -3;
+        }
+    }
+    hiprtcDestroyProgram(&prog);
+    return rc;
+}
+
+}  // namespace
+
+// Compile (no device needed) the kernel of one schedule for `arch`
+// ("gfx950"); 0 on success.  Used to check the embedded sources.
+extern "C" int vhip_jit_compile_check(int n, int nst, const int *radix, int vpb, int threads, const char *arch,
+                                      size_t *code_bytes) {
+    std::string name = instance(n, nst, radix, vpb, threads);
+    if (name.empty()) return -1;
+    std::vector<char> code;
+    std::string lowered;
+    int rc = compile(name, std::string("--offload-arch=") + arch, code, lowered);
+    if (code_bytes) *code_bytes = code.size();
+    return rc;
+}
+
+extern "C" const char *vhip_jit_last_error(void) { return g_err; }
+
+// Make the kernel of a schedule available (compile and load once); 0 when
+// it is, nonzero when the schedule runs on the generic kernel.
+extern "C" int vhip_jit_prepare(int n, int nst, const int *radix, int vpb, int threads) {
+    if (!jit_on()) return -1;
+    std::string name = instance(n, nst, radix, vpb, threads);
+    if (name.empty()) return -1;
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_cache.find(name);
+    if (it != g_cache.end()) return it->second.fn ? 0 : -1;
+    Entry e;
+    std::vector<char> code;
+    std::string lowered;
+    if (compile(name, arch_option(), code, lowered) == 0 &&
+        hipModuleLoadData(&e.mod, code.data()) == hipSuccess) {
+        if (hipModuleGetFunction(&e.fn, e.mod, lowered.c_str()) != hipSuccess) e.fn = nullptr;
+    }
+    (void)hipGetLastError();
+    g_cache[name] = e;
+    return e.fn ? 0 : -1;
+}
+
+// Is the kernel of this schedule compiled and loaded?
+extern "C" int vhip_jit_ready(int n, int nst, const int *radix, int vpb, int threads) {
+    std::string name = instance(n, nst, radix, vpb, threads);
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_cache.find(name);
+    return (it != g_cache.end() && it->second.fn) ? 1 : 0;
+}
+
+// Launch d on its compiled kernel when one is loaded and d has the layout
+// the kernel serves; -100 otherwise (the caller uses the generic kernel).
+extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
+    if (d->prec != 4 || d->kind != VHIP_C2C || d->ptw) return -100;
+    if (!d->in.contig || !d->out.contig || d->in.split || d->out.split) return -100;
+    if (d->map_in != VHIP_MAP_ALONG || d->map_out != VHIP_MAP_ALONG) return -100;
+    if (d->in.nv0 < d->nvec || d->out.nv0 < d->nvec) return -100;
+    if (d->nvec > 1 && (d->in.vs0 < d->n || d->out.vs0 < d->n)) return -100;   // rows must not overlap
+    const long span_i = ((long)(d->vpb - 1) * d->in.vs0 + d->n) * 8, span_o = ((long)(d->vpb - 1) * d->out.vs0 + d->n) * 8;
+    if (span_i >= (1L << 31) || span_o >= (1L << 31)) return -100;
+    const size_t lds = (size_t)d->vpb * d->lds_stride * 8;
+    if (lds > 65536) return -100;
+    hipFunction_t fn = nullptr;
+    {
+        std::string name = instance(d->n, d->nst, d->radix, d->vpb, d->threads);
+        std::lock_guard<std::mutex> lock(g_mu);
+        auto it = g_cache.find(name);
+        if (it == g_cache.end() || !it->second.fn) return -100;
+        fn = it->second.fn;
+    }
+    const long grid = (d->nvec + d->vpb - 1) / d->vpb;
+    if (grid > 0x7fffffffL) return -100;
+    vhip_fftm_desc arg = *d;
+    void *args[] = {&arg};
+    hipError_t e = hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, (unsigned)d->threads, 1, 1, (unsigned)lds,
+                                         (hipStream_t)stream, args, nullptr);
+    return (int)e;
+}

@@ -44,6 +44,8 @@ extern "C" int vhip_fftm_launch(const vhip_fftm_desc *d, void *stream) {
     vhip_cu_count(&cus);
     int rc = vhipk::pow2_try_launch(d, stream, cus > 0 ? cus : 256);
     if (rc != -100) return rc;
+    rc = vhip_jit_try_launch(d, stream);
+    if (rc != -100) return rc;
     kernel_fn f = pick_kernel(d);
     if (!f) return -2;
     const long grid = (d->nvec + d->vpb - 1) / d->vpb;

@@ -12,7 +12,9 @@
 #ifndef VHIP_H
 #define VHIP_H
 
+#ifndef __HIPCC_RTC__
 #include <stddef.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -170,6 +172,18 @@ int vhip_elem_scale(int prec, const vhip_mat *a, const vhip_mat *r, long m, long
 int vhip_elem_sub(int prec, const vhip_mat *a, const vhip_mat *b, const vhip_mat *c, long m, long n,
                   void *stream);
 int vhip_elem_sumsq(int prec, const vhip_mat *a, long m, long n, double *out, void *stream);
+
+/* Kernels compiled at plan creation for one schedule (hipRTC): prepare
+ * compiles and loads the kernel of (n, radices, vpb, threads) once per
+ * process (0 when available); try_launch runs a descriptor on it when it is
+ * loaded and the layout fits (contiguous interleaved FP32 rows, single
+ * pass), else returns -100. */
+int vhip_jit_prepare(int n, int nst, const int *radix, int vpb, int threads);
+int vhip_jit_ready(int n, int nst, const int *radix, int vpb, int threads);
+int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream);
+int vhip_jit_compile_check(int n, int nst, const int *radix, int vpb, int threads, const char *arch,
+                           size_t *code_bytes);
+const char *vhip_jit_last_error(void);
 
 /* kernels */
 int vhip_xform_launch(const vhip_xform_desc *d, void *stream);

@@ -101,6 +101,24 @@ static int va_plan_split(VA_fftm *p)
     return best ? 0 : -1;
 }
 
+/* Plans of row transforms that the schedule-generic kernel would run (FP32
+ * complex, one pass, not a power of two the power-of-two kernel serves) and
+ * that move at least VSIP_AMD_JIT_MIN_ELEMS elements (default 2^18) get
+ * their schedule compiled now; execution then finds the compiled kernel. */
+static void va_jit_prepare(const VA_fftm *p)
+{
+    const VA_sched *s = &p->pass[0];
+    const char *e = getenv("VSIP_AMD_JIT_MIN_ELEMS");
+    long min_elems = e ? atol(e) : (1L << 18);
+    long n = (long)p->MN;
+    if (p->prec != 4 || (p->type != VA_CCFFTIP && p->type != VA_CCFFTOP) || p->npass != 1 || p->staged) return;
+    if (p->major != VSIP_ROW) return;
+    if ((n & (n - 1)) == 0 && n >= 256 && n <= 16384) return;
+    if ((long)p->mN * n < min_elems) return;
+    if ((long)s->along.vpb * s->lds_stride * 8 > 65536) return;
+    vhip_jit_prepare(s->n, s->nst, s->radix, s->along.vpb, s->along.threads);
+}
+
 /* allow_staged == 0 gives up where a staged transform would be needed (the
  * inner plans of staged transforms, whose lengths are always smooth); 2 makes
  * a staged plan for every length (views the kernels cannot address directly) */
@@ -157,6 +175,7 @@ static int va_fftm_init2(VA_fftm *p, int prec, int type, vsip_length M, vsip_len
         va_sched_free(&p->pass[0]);
         return -1;
     }
+    va_jit_prepare(p);
     if (real) {
         /* rcfftmop: e^{-i pi k / MN}; crfftmop: e^{+i pi k / MN}; k = 0..MN */
         int sign = (type == VA_RCFFTOP) ? -1 : 1;

@@ -1,0 +1,67 @@
+"""GPU parity of the kernels compiled at plan creation (schedule-specialised
+Stockham, fftm_static.hpp), through the C ABI.
+
+With the size threshold at zero every eligible plan (FP32 complex rows, one
+pass, not a power of two of the power-of-two kernel) compiles its schedule;
+the plan must report the compiled kernel and its results must match the
+oracle (rel L2 <= 1e-5) and the schedule-generic kernel (<= 1e-6; the two run
+the same arithmetic).
+"""
+import json
+import sys
+
+import numpy as np
+import pytest
+
+from jvsip_amd import datagen, fftm
+from jvsip_amd.vsip import VSIP_ROW, VSIP_FFT_FWD, VSIP_FFT_INV
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    print("\n[case] %s" % name, flush=True)
+    sys.stdout.flush()
+
+
+def _check(name, got, want, tol):
+    err = po.rel_l2(got, want)
+    print("  %s: rel L2 %.3e" % (name, err), flush=True)
+    assert err < tol, "%s: rel L2 %.3e >= %.1e" % (name, err, tol)
+
+
+@pytest.mark.parametrize("n,m", [(3, 7), (12, 5), (100, 33), (143, 9), (243, 10), (1000, 37), (1001, 6),
+                                 (1920, 5), (2187, 4), (3125, 3), (6561, 2), (1152, 11), (4000, 3), (60, 200)])
+def test_compiled_schedule_matches_oracle_and_generic(amd, monkeypatch, n, m):
+    x = datagen.complex_data(95, (m, n))
+    v = fftm.dense_view(m, n)
+    monkeypatch.setenv("VSIP_AMD_JIT_MIN_ELEMS", "0")
+    p = amd.ccfftmip_create_f(m, n, 1.0, VSIP_FFT_FWD, VSIP_ROW, 0, 1)
+    assert p, amd.last_error()
+    d = json.loads(amd.describe(p))
+    amd.fftm_destroy_f(p)
+    print("  schedule", d["passes"][0]["radix"], d["passes"][0]["along"], "compiled", d.get("compiled_schedule"),
+          flush=True)
+    assert d.get("compiled_schedule") is True
+    for direction, scale in ((VSIP_FFT_FWD, 1.0), (VSIP_FFT_INV, 1.0 / n)):
+        _case("ccfftmip_f ROW %dx%d dir %+d (compiled schedule)" % (m, n, direction))
+        y = fftm.ccfftm(amd, x.ravel(), v, direction=direction, scale=scale)
+        _check("vs oracle", y, po.ccfftm(x.ravel(), v, direction=direction, scale=scale), 1e-5)
+    monkeypatch.setenv("VSIP_AMD_JIT", "0")
+    g = fftm.ccfftm(amd, x.ravel(), v)
+    monkeypatch.delenv("VSIP_AMD_JIT")
+    _check("vs generic kernel", fftm.ccfftm(amd, x.ravel(), v), g, 1e-6)
+
+
+def test_compiled_schedule_strided_out_of_place(amd, monkeypatch):
+    m, n = 50, 2187
+    _case("ccfftmop_f ROW %dx%d, parent rows 2x (compiled schedule)" % (m, n))
+    monkeypatch.setenv("VSIP_AMD_JIT_MIN_ELEMS", "0")
+    x = datagen.complex_data(96, (m * 2 * n,))
+    xv = fftm.View(0, 2 * n, m, 1, n)
+    yv = fftm.View(n, 2 * n, m, 1, n)
+    y0 = np.zeros(m * 2 * n, np.complex64)
+    xr, y = fftm.ccfftm(amd, x, xv, out_blk=y0, out_view=yv, direction=VSIP_FFT_INV, scale=0.5)
+    np.testing.assert_array_equal(xr, x)
+    _check("vs oracle", y, po.ccfftm(x, xv, out_blk=y0, out_view=yv, direction=VSIP_FFT_INV, scale=0.5), 1e-5)

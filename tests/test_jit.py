@@ -1,0 +1,48 @@
+"""CPU checks of the kernels compiled at plan creation (no device needed).
+
+The library carries the device headers of the schedule-specialised Stockham
+kernel (jvsip_amd/csrc/hip/fftm_static.hpp) and compiles one instance per
+plan schedule with hipRTC.  Here the embedded sources must compile for gfx950
+for the schedules the planner produces, and schedules without a compiled
+form must be refused rather than miscompiled.
+"""
+import ctypes
+import os
+
+import pytest
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from jvsip_amd.vsip import AmdLib, LIB_PATH
+    os.environ.setdefault("VSIP_AMD_QUIET", "1")
+    return AmdLib(LIB_PATH)
+
+
+@pytest.fixture(scope="module")
+def so():
+    from jvsip_amd.vsip import LIB_PATH
+    d = ctypes.CDLL(LIB_PATH)
+    d.vhip_jit_compile_check.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]
+    d.vhip_jit_compile_check.restype = ctypes.c_int
+    d.vhip_jit_last_error.restype = ctypes.c_char_p
+    return d
+
+
+@pytest.mark.parametrize("n,nvec", [(1000, 4096), (2187, 1024), (1001, 512), (1920, 1000), (3000, 512),
+                                    (6561, 64), (100, 10000), (7 * 11 * 13 * 3, 300)])
+def test_planner_schedules_compile(lib, so, n, nvec):
+    s = lib.schedule(4, n, nvec, 163840)
+    assert s["rc"] == 0
+    vpb, thr, _ = s["along"]
+    rad = (ctypes.c_int * len(s["radix"]))(*s["radix"])
+    size = ctypes.c_size_t()
+    rc = so.vhip_jit_compile_check(n, len(s["radix"]), rad, vpb, thr, b"gfx950", ctypes.byref(size))
+    assert rc == 0, so.vhip_jit_last_error()
+    assert size.value > 1000
+
+
+def test_refuses_schedules_without_a_compiled_form(so):
+    rad = (ctypes.c_int * 9)(*([2] * 9))
+    assert so.vhip_jit_compile_check(512, 9, rad, 1, 64, b"gfx950", None) != 0
