@@ -46,14 +46,23 @@ bool jit_on() {
 
 // the template instance of a schedule; empty when the schedule has no
 // compiled form (more than 8 stages, or not a single pass)
-std::string instance(int n, int nst, const int *radix, int vpb, int threads) {
-    if (nst < 1 || nst > 8) return "";
+std::string instance(int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
+    if (nst < 1 || nst > 8 || min < 0 || min > 1 || mout < 0 || mout > 1) return "";
     int r[8];
     for (int s = 0; s < 8; ++s) r[s] = s < nst ? radix[s] : 1;
     char buf[256];
-    snprintf(buf, sizeof buf, "vhipk::st::fftm_static_kernel<%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d>", n, vpb,
-             threads, nst, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]);
+    snprintf(buf, sizeof buf, "vhipk::st::fftm_static_kernel<%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d>", n,
+             vpb, threads, min, mout, nst, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]);
     return buf;
+}
+
+// layout mode of one side of a descriptor: 0 contiguous rows (single-level
+// vectors), 1 across vectors (any strides, span known); -1 neither
+int side_mode(const vhip_view &g, int map, long nvec) {
+    if (g.split) return -1;
+    if (map == VHIP_MAP_ALONG && g.contig && g.nv0 >= nvec) return 0;
+    if (map == VHIP_MAP_ACROSS && g.span > 0) return 1;
+    return -1;
 }
 
 std::string arch_option() {
@@ -106,9 +115,9 @@ int compile(const std::string &name, const std::string &arch, std::vector<char> 
 
 // Compile (no device needed) the kernel of one schedule for `arch`
 // ("gfx950"); 0 on success.  Used to check the embedded sources.
-extern "C" int vhip_jit_compile_check(int n, int nst, const int *radix, int vpb, int threads, const char *arch,
-                                      size_t *code_bytes) {
-    std::string name = instance(n, nst, radix, vpb, threads);
+extern "C" int vhip_jit_compile_check(int n, int nst, const int *radix, int vpb, int threads, int min, int mout,
+                                      const char *arch, size_t *code_bytes) {
+    std::string name = instance(n, nst, radix, vpb, threads, min, mout);
     if (name.empty()) return -1;
     std::vector<char> code;
     std::string lowered;
@@ -121,9 +130,9 @@ extern "C" const char *vhip_jit_last_error(void) { return g_err; }
 
 // Make the kernel of a schedule available (compile and load once); 0 when
 // it is, nonzero when the schedule runs on the generic kernel.
-extern "C" int vhip_jit_prepare(int n, int nst, const int *radix, int vpb, int threads) {
+extern "C" int vhip_jit_prepare(int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
     if (!jit_on()) return -1;
-    std::string name = instance(n, nst, radix, vpb, threads);
+    std::string name = instance(n, nst, radix, vpb, threads, min, mout);
     if (name.empty()) return -1;
     std::lock_guard<std::mutex> lock(g_mu);
     auto it = g_cache.find(name);
@@ -141,8 +150,8 @@ extern "C" int vhip_jit_prepare(int n, int nst, const int *radix, int vpb, int t
 }
 
 // Is the kernel of this schedule compiled and loaded?
-extern "C" int vhip_jit_ready(int n, int nst, const int *radix, int vpb, int threads) {
-    std::string name = instance(n, nst, radix, vpb, threads);
+extern "C" int vhip_jit_ready(int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
+    std::string name = instance(n, nst, radix, vpb, threads, min, mout);
     std::lock_guard<std::mutex> lock(g_mu);
     auto it = g_cache.find(name);
     return (it != g_cache.end() && it->second.fn) ? 1 : 0;
@@ -151,18 +160,20 @@ extern "C" int vhip_jit_ready(int n, int nst, const int *radix, int vpb, int thr
 // Launch d on its compiled kernel when one is loaded and d has the layout
 // the kernel serves; -100 otherwise (the caller uses the generic kernel).
 extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
-    if (d->prec != 4 || d->kind != VHIP_C2C || d->ptw) return -100;
-    if (!d->in.contig || !d->out.contig || d->in.split || d->out.split) return -100;
-    if (d->map_in != VHIP_MAP_ALONG || d->map_out != VHIP_MAP_ALONG) return -100;
-    if (d->in.nv0 < d->nvec || d->out.nv0 < d->nvec) return -100;
-    if (d->nvec > 1 && (d->in.vs0 < d->n || d->out.vs0 < d->n)) return -100;   // rows must not overlap
-    const long span_i = ((long)(d->vpb - 1) * d->in.vs0 + d->n) * 8, span_o = ((long)(d->vpb - 1) * d->out.vs0 + d->n) * 8;
-    if (span_i >= (1L << 31) || span_o >= (1L << 31)) return -100;
-    const size_t lds = (size_t)d->vpb * d->lds_stride * 8;
-    if (lds > 65536) return -100;
+    if (d->prec != 4 || d->kind != VHIP_C2C) return -100;
+    const int min = side_mode(d->in, d->map_in, d->nvec), mout = side_mode(d->out, d->map_out, d->nvec);
+    if (min < 0 || mout < 0) return -100;
+    if (min == 0) {
+        if (d->nvec > 1 && d->in.vs0 < d->n) return -100;                 // rows must not overlap
+        if (((long)(d->vpb - 1) * d->in.vs0 + d->n) * 8 >= (1L << 31)) return -100;
+    }
+    if (mout == 0) {
+        if (d->nvec > 1 && d->out.vs0 < d->n) return -100;
+        if (((long)(d->vpb - 1) * d->out.vs0 + d->n) * 8 >= (1L << 31)) return -100;
+    }
     hipFunction_t fn = nullptr;
     {
-        std::string name = instance(d->n, d->nst, d->radix, d->vpb, d->threads);
+        std::string name = instance(d->n, d->nst, d->radix, d->vpb, d->threads, min, mout);
         std::lock_guard<std::mutex> lock(g_mu);
         auto it = g_cache.find(name);
         if (it == g_cache.end() || !it->second.fn) return -100;
@@ -172,7 +183,7 @@ extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
     if (grid > 0x7fffffffL) return -100;
     vhip_fftm_desc arg = *d;
     void *args[] = {&arg};
-    hipError_t e = hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, (unsigned)d->threads, 1, 1, (unsigned)lds,
+    hipError_t e = hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, (unsigned)d->threads, 1, 1, 0,
                                          (hipStream_t)stream, args, nullptr);
     return (int)e;
 }

@@ -1,20 +1,31 @@
 // fftm_static.hpp -- the Stockham transform of fftm_core.hpp with its whole
 // schedule fixed at compile time: length N, radices R0..R7 (NST of them),
-// vectors per workgroup VPB and lanes NT.  The library compiles one such
-// kernel per plan schedule at plan creation (hipRTC, in-process cache), for
-// contiguous rows of interleaved complex FP32 data; every butterfly index,
-// twiddle index and LDS offset then folds to constants, shifts and
-// multiply-highs instead of the runtime divisions and 64-bit address
-// arithmetic of the schedule-generic kernel.
+// vectors per workgroup VPB, lanes NT and the layout of the global load and
+// store (MIN, MOUT).  The library compiles one such kernel per plan schedule
+// at plan creation (hipRTC, in-process cache) for interleaved complex FP32
+// data; every butterfly index, twiddle index and LDS offset then folds to
+// constants, shifts and multiply-highs instead of the runtime divisions and
+// address arithmetic of the schedule-generic kernel.
 //
 // Stage s (radix R = R_s, Ns = R_0 ... R_{s-1}): butterfly b of the group
-// is vector v = b / (N/R), index j = b mod (N/R); it reads elements
-// j + r N/R (from HBM in stage 0, else from LDS), multiplies leg r by
-// W_{Ns R}^{r k}, k = j mod Ns (stage twiddles at d.tw + d.tw_off[s]), runs
-// the R-point DFT and writes leg r to (j - k) R + k + r Ns (to HBM, scaled,
-// in the last stage).  Inverse transforms conjugate on the way in and out.
+// is vector v, index j; it reads elements j + r N/R (from HBM in stage 0,
+// else from LDS), multiplies leg r by W_{Ns R}^{r k}, k = j mod Ns (stage
+// twiddles at d.tw + d.tw_off[s]), runs the R-point DFT and writes leg r to
+// (j - k) R + k + r Ns (to HBM, scaled, in the last stage).  Inverse
+// transforms conjugate on the way in and out.
+//
+// Global layouts:
+//   ST_CONTIG  rows of unit element stride, one vector stride (vs0); lanes
+//              walk along the vector (v = b / (N/R))
+//   ST_ACROSS  any element stride, two-level vector offsets
+//              (v % nv0) vs0 + (v / nv0) vs1; lanes walk across the group's
+//              vectors (v = b mod VPB), for unit vector stride (columns)
+// Middle stages always walk along the vector in LDS.
 #pragma once
 #include "fftm_core.hpp"
+
+#define ST_CONTIG 0
+#define ST_ACROSS 1
 
 namespace vhipk {
 namespace st {
@@ -34,11 +45,16 @@ __device__ __forceinline__ constexpr int lpad(int i) { return i + (i >> 5); }
 
 struct Io {
     rsrc_t in, out, tw;
-    long ivs, ovs;
+    long ivs, ovs;         // ST_CONTIG vector strides
+    int ies, oes;          // ST_ACROSS element strides
+    int nv;                // vectors in this group
+    const int *voff_in;    // ST_ACROSS: element offset of each vector of the group
+    const int *voff_out;
+    const int *vidx;       // post-twiddled launches: index of each vector within its row
     float isgn, osgn, scale;
 };
 
-template <int N, int VPB, int NT, int S, class SC>
+template <int N, int VPB, int NT, int MIN, int MOUT, int S, class SC>
 __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d, const Io &io) {
     constexpr int R = SC::rad(S);
     constexpr int NS = SC::ns(S);
@@ -46,20 +62,31 @@ __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d,
     constexpr int NBT = VPB * NBV;
     constexpr int B = (NBT + NT - 1) / NT;
     constexpr bool FIRST = (S == 0), LAST = (S == SC::NST - 1);
+    // lane mapping of this stage: the global layout's for the first and last
+    // stage (the load's when one stage does both), along the vector otherwise
+    constexpr int MODE = FIRST ? MIN : (LAST ? MOUT : ST_CONTIG);
     constexpr int LS = (lpad(N) % 2 == 0) ? lpad(N) + 1 : lpad(N);
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
     cpx<float> x[B][R];
+    int vv[B], jj[B];
+    bool ok[B];
 #pragma unroll
     for (int i = 0; i < B; ++i) {
         const int b = t + i * NT;
-        if (B * NT <= NBT || b < NBT) {
-            const int v = b / NBV, j = b - v * NBV;
+        int v, j;
+        if (MODE == ST_ACROSS) { v = b % VPB; j = b / VPB; }
+        else { v = b / NBV; j = b - v * NBV; }
+        vv[i] = v;
+        jj[i] = j;
+        ok[i] = (B * NT <= NBT || b < NBT) && v < io.nv;
+        if (ok[i]) {
             if (FIRST) {
-                const int vo = (int)(v * io.ivs + j) * 8;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    cpx<float> y = Buf<float>::ldc(io.in, vo, r * NBV * 8);
+                    cpx<float> y;
+                    if (MIN == ST_CONTIG) y = Buf<float>::ldc(io.in, (int)(v * io.ivs + j) * 8, r * NBV * 8);
+                    else y = Buf<float>::ldc(io.in, (io.voff_in[v] + (j + r * NBV) * io.ies) * 8, 0);
                     y.y *= io.isgn;
                     x[i][r] = y;
                 }
@@ -73,9 +100,8 @@ __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d,
     if (!FIRST) __syncthreads();
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-        const int b = t + i * NT;
-        if (B * NT <= NBT || b < NBT) {
-            const int v = b / NBV, j = b - v * NBV;
+        if (ok[i]) {
+            const int v = vv[i], j = jj[i];
             const int k = (NS > 1) ? j % NS : 0;
             if (NS > 1) {
                 const int toff = d.tw_off[S];
@@ -85,11 +111,14 @@ __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d,
             }
             Dft<float, R>::run(x[i]);
             if (LAST) {
-                const int vo = (int)(v * io.ovs + j) * 8;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const cpx<float> y = x[i][r];
-                    Buf<float>::stc(io.out, vo, r * NBV * 8, {y.x * io.scale, y.y * (io.scale * io.osgn)});
+                    cpx<float> y = x[i][r];
+                    if (d.ptw)      // first pass of the two-pass split: e^{-2 pi i j2 k1 / n}
+                        y = cmul(y, ((const cpx<float> *)d.ptw)[io.vidx[v] * (j + r * NBV)]);
+                    const cpx<float> z = {y.x * io.scale, y.y * (io.scale * io.osgn)};
+                    if (MOUT == ST_CONTIG) Buf<float>::stc(io.out, (int)(v * io.ovs + j) * 8, r * NBV * 8, z);
+                    else Buf<float>::stc(io.out, (io.voff_out[v] + (j + r * NBV) * io.oes) * 8, 0, z);
                 }
             } else {
                 const int base = (j - k) * R + k;
@@ -102,33 +131,62 @@ __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d,
     if (!LAST) __syncthreads();
 }
 
-template <int N, int VPB, int NT, int S, class SC>
+template <int N, int VPB, int NT, int MIN, int MOUT, int S, class SC>
 __device__ __forceinline__ void sstages(cpx<float> *lds, const vhip_fftm_desc &d, const Io &io) {
-    sstage<N, VPB, NT, S, SC>(lds, d, io);
-    if constexpr (S + 1 < SC::NST) sstages<N, VPB, NT, S + 1, SC>(lds, d, io);
+    sstage<N, VPB, NT, MIN, MOUT, S, SC>(lds, d, io);
+    if constexpr (S + 1 < SC::NST) sstages<N, VPB, NT, MIN, MOUT, S + 1, SC>(lds, d, io);
 }
 
-// One workgroup per group of VPB rows (contiguous interleaved complex FP32,
-// single-level vector stride d.in.vs0 / d.out.vs0 in complex elements);
-// rows past the end of the view read zeros and their stores fall outside the
-// group's buffer range.
-template <int N, int VPB, int NT, int NST, int R0, int R1, int R2, int R3, int R4, int R5, int R6, int R7>
+// One workgroup per group of VPB vectors.  ST_CONTIG sides address the
+// group's rows through a buffer range that ends at the last row of the view
+// (rows past it read zeros, their stores are dropped); ST_ACROSS sides
+// address the whole view (vhip_view.span) and skip vectors past its end.
+// Column-mapped launches take the XCD-aware group numbering of the generic
+// kernel (xcd_group).
+template <int N, int VPB, int NT, int MIN, int MOUT, int NST, int R0, int R1, int R2, int R3, int R4, int R5,
+          int R6, int R7>
 __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d) {
-    extern __shared__ __attribute__((aligned(16))) char smem[];
     using SC = Sched<NST, R0, R1, R2, R3, R4, R5, R6, R7>;
-    const long g = blockIdx.x;
-    long nvl = d.nvec - g * VPB;
-    const long nv = nvl < VPB ? nvl : VPB;
+    constexpr int LS = (lpad(N) % 2 == 0) ? lpad(N) + 1 : lpad(N);
+    // static LDS: the size is known here, and a static allocation may use
+    // all of the workgroup's LDS without a launch attribute
+    __shared__ __attribute__((aligned(16))) cpx<float> slds[VPB * LS];
+    __shared__ int voff[3 * VPB];
+    constexpr bool ACROSS = (MIN == ST_ACROSS || MOUT == ST_ACROSS);
+    const long g = ACROSS ? (long)xcd_group(blockIdx.x, gridDim.x) : (long)blockIdx.x;
+    const long vbase = g * VPB;
+    long nvl = d.nvec - vbase;
     Io io;
+    io.nv = nvl < VPB ? (int)nvl : VPB;
     io.ivs = d.in.vs0;
     io.ovs = d.out.vs0;
-    io.in = make_rsrc((const cpx<float> *)d.in.re + g * VPB * io.ivs, ((nv - 1) * io.ivs + N) * 8);
-    io.out = make_rsrc((cpx<float> *)d.out.re + g * VPB * io.ovs, ((nv - 1) * io.ovs + N) * 8);
+    io.ies = (int)d.in.es;
+    io.oes = (int)d.out.es;
+    io.voff_in = voff;
+    io.voff_out = voff + VPB;
+    io.vidx = voff + 2 * VPB;
+    if (ACROSS || d.ptw) {
+        for (int t = threadIdx.x; t < io.nv; t += NT) {
+            const long v = vbase + t;
+            voff[t] = (int)vec_off(d.in, v);
+            voff[VPB + t] = (int)vec_off(d.out, v);
+            voff[2 * VPB + t] = (int)(d.out.nv0 > 0 ? v % d.out.nv0 : v);
+        }
+        __syncthreads();
+    }
+    if (MIN == ST_CONTIG)
+        io.in = make_rsrc((const cpx<float> *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + N) * 8);
+    else
+        io.in = make_rsrc(d.in.re, d.in.span);
+    if (MOUT == ST_CONTIG)
+        io.out = make_rsrc((cpx<float> *)d.out.re + vbase * io.ovs, ((io.nv - 1) * io.ovs + N) * 8);
+    else
+        io.out = make_rsrc(d.out.re, d.out.span);
     io.tw = make_rsrc(d.tw, (long)N * 8);
     io.isgn = d.conj_in ? -1.f : 1.f;
     io.osgn = d.conj_out ? -1.f : 1.f;
     io.scale = (float)d.scale;
-    sstages<N, VPB, NT, 0, SC>((cpx<float> *)smem, d, io);
+    sstages<N, VPB, NT, MIN, MOUT, 0, SC>(slds, d, io);
 }
 
 }  // namespace st

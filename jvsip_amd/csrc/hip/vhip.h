@@ -174,15 +174,19 @@ int vhip_elem_sub(int prec, const vhip_mat *a, const vhip_mat *b, const vhip_mat
 int vhip_elem_sumsq(int prec, const vhip_mat *a, long m, long n, double *out, void *stream);
 
 /* Kernels compiled at plan creation for one schedule (hipRTC): prepare
- * compiles and loads the kernel of (n, radices, vpb, threads) once per
- * process (0 when available); try_launch runs a descriptor on it when it is
- * loaded and the layout fits (contiguous interleaved FP32 rows, single
- * pass), else returns -100. */
-int vhip_jit_prepare(int n, int nst, const int *radix, int vpb, int threads);
-int vhip_jit_ready(int n, int nst, const int *radix, int vpb, int threads);
+ * compiles and loads the kernel of (n, radices, vpb, threads, load layout,
+ * store layout) once per process (0 when available); try_launch runs a
+ * descriptor on it when it is loaded and the descriptor's layouts match,
+ * else returns -100.  Layouts: VHIP_JIT_CONTIG (interleaved FP32 rows of unit
+ * element stride, single-level vector stride), VHIP_JIT_ACROSS (interleaved,
+ * lanes across vectors, any strides within vhip_view.span). */
+#define VHIP_JIT_CONTIG 0
+#define VHIP_JIT_ACROSS 1
+int vhip_jit_prepare(int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);
+int vhip_jit_ready(int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);
 int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream);
-int vhip_jit_compile_check(int n, int nst, const int *radix, int vpb, int threads, const char *arch,
-                           size_t *code_bytes);
+int vhip_jit_compile_check(int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out,
+                           const char *arch, size_t *code_bytes);
 const char *vhip_jit_last_error(void);
 
 /* kernels */

@@ -101,22 +101,68 @@ static int va_plan_split(VA_fftm *p)
     return best ? 0 : -1;
 }
 
-/* Plans of row transforms that the schedule-generic kernel would run (FP32
- * complex, one pass, not a power of two the power-of-two kernel serves) and
- * that move at least VSIP_AMD_JIT_MIN_ELEMS elements (default 2^18) get
- * their schedule compiled now; execution then finds the compiled kernel. */
+/* the compiled kernels hold their vectors and offsets in static LDS */
+static int va_jit_lds_ok(const VA_sched *s, const VA_launch *L)
+{
+    int lds_max = 65536;
+    vhip_lds_max(&lds_max);
+    return (long)L->vpb * s->lds_stride * 8 + (long)L->vpb * 12 <= lds_max;
+}
+
+/* FP32 complex plans whose passes the schedule-generic kernel would run
+ * (every pass except the power-of-two kernel's contiguous rows) and that move
+ * at least VSIP_AMD_JIT_MIN_ELEMS elements (default 2^18) get their
+ * schedules compiled now, for the layouts the plan's views normally have:
+ * contiguous rows (ROW), lanes across columns (COL), and the two passes of
+ * the split.  Execution finds the compiled kernel when the layout matches. */
 static void va_jit_prepare(const VA_fftm *p)
 {
-    const VA_sched *s = &p->pass[0];
     const char *e = getenv("VSIP_AMD_JIT_MIN_ELEMS");
     long min_elems = e ? atol(e) : (1L << 18);
     long n = (long)p->MN;
-    if (p->prec != 4 || (p->type != VA_CCFFTIP && p->type != VA_CCFFTOP) || p->npass != 1 || p->staged) return;
-    if (p->major != VSIP_ROW) return;
-    if ((n & (n - 1)) == 0 && n >= 256 && n <= 16384) return;
+    if (p->prec != 4 || (p->type != VA_CCFFTIP && p->type != VA_CCFFTOP) || p->staged) return;
     if ((long)p->mN * n < min_elems) return;
-    if ((long)s->along.vpb * s->lds_stride * 8 > 65536) return;
-    vhip_jit_prepare(s->n, s->nst, s->radix, s->along.vpb, s->along.threads);
+    if (p->npass == 1) {
+        const VA_sched *s = &p->pass[0];
+        if (p->major == VSIP_ROW) {
+            if ((n & (n - 1)) == 0 && n >= 256 && n <= 16384) return;    /* the power-of-two kernel */
+            if (va_jit_lds_ok(s, &s->along))
+                vhip_jit_prepare(s->n, s->nst, s->radix, s->along.vpb, s->along.threads, VHIP_JIT_CONTIG,
+                                 VHIP_JIT_CONTIG);
+        } else if (va_jit_lds_ok(s, &s->across)) {
+            vhip_jit_prepare(s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
+                             VHIP_JIT_ACROSS);
+        }
+    } else if (p->npass == 2) {
+        /* two-pass split: both passes map lanes across vectors at the
+         * global side they share with the view; pass B reads scratch rows */
+        const VA_sched *a = &p->pass[0], *b = &p->pass[1];
+        if (va_jit_lds_ok(a, &a->across))
+            vhip_jit_prepare(a->n, a->nst, a->radix, a->across.vpb, a->across.threads, VHIP_JIT_ACROSS,
+                             VHIP_JIT_ACROSS);
+        if (va_jit_lds_ok(b, &b->across))
+            vhip_jit_prepare(b->n, b->nst, b->radix, b->across.vpb, b->across.threads, VHIP_JIT_CONTIG,
+                             VHIP_JIT_ACROSS);
+    }
+}
+
+/* does the plan have compiled kernels for its expected layouts */
+int va_jit_ready(const VA_fftm *p)
+{
+    const VA_sched *s = &p->pass[0];
+    if (p->npass == 1) {
+        if (p->major == VSIP_ROW)
+            return vhip_jit_ready(s->n, s->nst, s->radix, s->along.vpb, s->along.threads, VHIP_JIT_CONTIG,
+                                  VHIP_JIT_CONTIG);
+        return vhip_jit_ready(s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
+                              VHIP_JIT_ACROSS);
+    }
+    if (p->npass == 2)
+        return vhip_jit_ready(s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
+                              VHIP_JIT_ACROSS) &&
+               vhip_jit_ready(p->pass[1].n, p->pass[1].nst, p->pass[1].radix, p->pass[1].across.vpb,
+                              p->pass[1].across.threads, VHIP_JIT_CONTIG, VHIP_JIT_ACROSS);
+    return 0;
 }
 
 /* allow_staged == 0 gives up where a staged transform would be needed (the
@@ -390,7 +436,8 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
         d[0].ptw = ptw;
         d[0].ptw_n = MN;
         /* pass B: vectors (row, k1), elements j2 contiguous; out X[row][k1 + n1*k2] */
-        t.es = 1; t.vs0 = n2; t.nv0 = n1; t.vs1 = MN; t.contig = 1;
+        /* vector (row, k1) starts at row * MN + k1 * n2 = v * n2: one level */
+        t.es = 1; t.vs0 = n2; t.nv0 = mN * n1; t.vs1 = 0; t.contig = 1;
         t.span = va_span(&t, mN * n1, n2, va_esz(&t, p->prec, 1));
         go.es = n1 * es_o; go.vs0 = es_o; go.nv0 = n1; go.vs1 = vs_o;
         go.contig = 0;

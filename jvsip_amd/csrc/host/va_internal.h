@@ -179,6 +179,7 @@ int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, d
                  int dir, vsip_major major, unsigned int ntimes, vsip_alg_hint hint);
 int va_fftm_init_inner(VA_fftm *p, int prec, vsip_length M, vsip_length N);
 void va_fftm_release(VA_fftm *p);
+int va_jit_ready(const VA_fftm *p);
 int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void *stream);
 int va_staged_init(VA_fftm *p, int bluestein);
 void va_staged_release(VA_fftm *p);

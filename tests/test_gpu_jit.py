@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from jvsip_amd import datagen, fftm
-from jvsip_amd.vsip import VSIP_ROW, VSIP_FFT_FWD, VSIP_FFT_INV
+from jvsip_amd.vsip import VSIP_ROW, VSIP_COL, VSIP_FFT_FWD, VSIP_FFT_INV
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -65,3 +65,78 @@ def test_compiled_schedule_strided_out_of_place(amd, monkeypatch):
     xr, y = fftm.ccfftm(amd, x, xv, out_blk=y0, out_view=yv, direction=VSIP_FFT_INV, scale=0.5)
     np.testing.assert_array_equal(xr, x)
     _check("vs oracle", y, po.ccfftm(x, xv, out_blk=y0, out_view=yv, direction=VSIP_FFT_INV, scale=0.5), 1e-5)
+
+
+def _plan_info(amd, m, n, major):
+    p = amd.ccfftmip_create_f(m, n, 1.0, VSIP_FFT_FWD, major, 0, 1)
+    assert p, amd.last_error()
+    d = json.loads(amd.describe(p))
+    amd.fftm_destroy_f(p)
+    print("  passes", [(q["n"], q["radix"]) for q in d["passes"]], "compiled", d.get("compiled_schedule"),
+          flush=True)
+    return d
+
+
+@pytest.mark.parametrize("m,n", [(1000, 37), (2187, 20), (1920, 300), (1001, 3), (5, 64)])
+def test_compiled_schedule_columns(amd, monkeypatch, m, n):
+    """COL plans: lanes walk across the columns at the global load and
+    store (unit vector stride), along the vector in LDS."""
+    monkeypatch.setenv("VSIP_AMD_JIT_MIN_ELEMS", "0")
+    x = datagen.complex_data(97, (m, n))
+    v = fftm.dense_view(m, n)
+    d = _plan_info(amd, m, n, VSIP_COL)
+    assert d.get("compiled_schedule") is True
+    for direction, scale in ((VSIP_FFT_FWD, 1.0), (VSIP_FFT_INV, 1.0 / m)):
+        _case("ccfftmip_f COL %dx%d dir %+d (compiled schedule)" % (m, n, direction))
+        y = fftm.ccfftm(amd, x.ravel(), v, major=VSIP_COL, direction=direction, scale=scale)
+        _check("vs oracle", y, po.ccfftm(x.ravel(), v, major=VSIP_COL, direction=direction, scale=scale), 1e-5)
+    monkeypatch.setenv("VSIP_AMD_JIT", "0")
+    g = fftm.ccfftm(amd, x.ravel(), v, major=VSIP_COL)
+    monkeypatch.delenv("VSIP_AMD_JIT")
+    _check("vs generic kernel", fftm.ccfftm(amd, x.ravel(), v, major=VSIP_COL), g, 1e-6)
+
+
+def test_compiled_schedule_columns_strided_out_of_place(amd, monkeypatch):
+    """COL ccfftmop from a column-strided parent into a row-reversed-free
+    strided output (element stride 2 vectors, vector stride 1)."""
+    m, n = 1000, 40
+    _case("ccfftmop_f COL %dx%d, in every other column of a %dx%d parent (compiled schedule)" % (m, n, m, 2 * n))
+    monkeypatch.setenv("VSIP_AMD_JIT_MIN_ELEMS", "0")
+    x = datagen.complex_data(98, (m * 2 * n,))
+    xv = fftm.View(1, 2 * n, m, 2, n)
+    yv = fftm.View(0, n, m, 1, n)
+    y0 = np.zeros(m * n, np.complex64)
+    xr, y = fftm.ccfftm(amd, x, xv, major=VSIP_COL, out_blk=y0, out_view=yv, direction=VSIP_FFT_INV, scale=0.25)
+    np.testing.assert_array_equal(xr, x)
+    want = po.ccfftm(x, xv, major=VSIP_COL, out_blk=y0, out_view=yv, direction=VSIP_FFT_INV, scale=0.25)
+    _check("vs oracle", y, want, 1e-5)
+
+
+@pytest.mark.parametrize("m,n", [(3, 59049), (2, 100000), (4, 30000)])
+def test_compiled_schedule_two_pass(amd, monkeypatch, m, n):
+    """The two-pass split of long rows: pass A (across, post-twiddled) and
+    pass B (scratch rows in, across out) both run compiled kernels."""
+    monkeypatch.setenv("VSIP_AMD_JIT_MIN_ELEMS", "0")
+    x = datagen.complex_data(99, (m, n))
+    v = fftm.dense_view(m, n)
+    d = _plan_info(amd, m, n, VSIP_ROW)
+    assert len(d["passes"]) == 2 and d.get("compiled_schedule") is True
+    x64 = x.astype(np.complex128)
+    for direction, scale in ((VSIP_FFT_FWD, 1.0), (VSIP_FFT_INV, 1.0 / n)):
+        _case("ccfftmip_f ROW %dx%d dir %+d two-pass (compiled schedule)" % (m, n, direction))
+        y = fftm.ccfftm(amd, x.ravel(), v, direction=direction, scale=scale).reshape(m, n)
+        want = (np.fft.fft(x64, axis=1) if direction < 0 else np.fft.ifft(x64, axis=1) * n) * scale
+        _check("vs numpy fp64", y, want, 1e-5)
+    monkeypatch.setenv("VSIP_AMD_JIT", "0")
+    g = fftm.ccfftm(amd, x.ravel(), v)
+    monkeypatch.delenv("VSIP_AMD_JIT")
+    _check("vs generic kernel", fftm.ccfftm(amd, x.ravel(), v), g, 1e-6)
+
+
+def test_compiled_schedule_long_1d(amd, monkeypatch):
+    n = 1 << 20
+    _case("ccfftip_f n=2^20 (two-pass, compiled schedules)")
+    monkeypatch.setenv("VSIP_AMD_JIT_MIN_ELEMS", "0")
+    x = datagen.complex_data(100, (n,))
+    y = fftm.ccfft(amd, x.copy(), fftm.View1(0, 1, n))
+    _check("vs numpy fp64", y, np.fft.fft(x.astype(np.complex128)), 1e-5)

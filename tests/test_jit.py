@@ -24,7 +24,8 @@ def so():
     from jvsip_amd.vsip import LIB_PATH
     d = ctypes.CDLL(LIB_PATH)
     d.vhip_jit_compile_check.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
-                                         ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                         ctypes.POINTER(ctypes.c_size_t)]
     d.vhip_jit_compile_check.restype = ctypes.c_int
     d.vhip_jit_last_error.restype = ctypes.c_char_p
     return d
@@ -35,14 +36,17 @@ def so():
 def test_planner_schedules_compile(lib, so, n, nvec):
     s = lib.schedule(4, n, nvec, 163840)
     assert s["rc"] == 0
-    vpb, thr, _ = s["along"]
     rad = (ctypes.c_int * len(s["radix"]))(*s["radix"])
-    size = ctypes.c_size_t()
-    rc = so.vhip_jit_compile_check(n, len(s["radix"]), rad, vpb, thr, b"gfx950", ctypes.byref(size))
-    assert rc == 0, so.vhip_jit_last_error()
-    assert size.value > 1000
+    for shape, lin, lout in (("along", 0, 0), ("across", 1, 1), ("across", 0, 1)):
+        vpb, thr, _ = s[shape]
+        size = ctypes.c_size_t()
+        rc = so.vhip_jit_compile_check(n, len(s["radix"]), rad, vpb, thr, lin, lout, b"gfx950", ctypes.byref(size))
+        assert rc == 0, so.vhip_jit_last_error()
+        assert size.value > 1000
 
 
 def test_refuses_schedules_without_a_compiled_form(so):
     rad = (ctypes.c_int * 9)(*([2] * 9))
-    assert so.vhip_jit_compile_check(512, 9, rad, 1, 64, b"gfx950", None) != 0
+    assert so.vhip_jit_compile_check(512, 9, rad, 1, 64, 0, 0, b"gfx950", None) != 0
+    rad = (ctypes.c_int * 2)(4, 4)
+    assert so.vhip_jit_compile_check(16, 2, rad, 1, 64, 2, 0, b"gfx950", None) != 0
