@@ -470,6 +470,43 @@ def config_fft1d(lib, hip):
     return out
 
 
+def config_f3_komega(lib, hip):
+    """k-omega beamformer step (the reference's kw.c komega()): the VSIPL call
+    sequence (vmmul ROW, vmmul COL, rcfftmop ROW, ccfftmip COL, mcmagsq,
+    smmul, madd: 7 launches) and the fused step (vsip_amd_komega_f: 2
+    launches), at the reference's parameter file (128 sensors x 1024 samples)
+    and at 1024 x 8192; plane waves plus uniform noise"""
+    from jvsip_amd import komega as kw
+    out = {}
+    for m, n in ((128, 1024), (1024, 8192)):
+        nf = n // 2 + 1
+        x = kw.plane_waves(m, n, [(n // 8, 3, 1.0), (n // 5, m - 7, 0.7)], seed=7, noise=0.5)
+        ks, kf = kw.KOmega(lib, m, n, 8), kw.KOmega(lib, m, n, 8)
+        ds, df = kw.data_matrix(lib, x), kw.data_matrix(lib, x)
+        ks.zero()
+        kf.zero()
+        ks.step(ds)
+        kf.step_fused(df)
+        g_s, g_f = ks.gram(), kf.gram()
+        xt = x.astype(np.float64) * np.hanning(n + 2)[1:-1][None, :] * np.hanning(m + 2)[1:-1][:, None]
+        y = np.fft.ifft(np.fft.rfft(xt, axis=1), axis=0) * m
+        want = (y.real ** 2 + y.imag ** 2) / 8
+        _, t_s = _time_calls(lib, hip, lambda: ks.step(ds), 50)
+        _, t_f = _time_calls(lib, hip, lambda: kf.step_fused(df), 50)
+        fused_bytes = 4 * m * n + 24 * m * nf
+        out["%dx%d" % (m, n)] = {
+            "sequence_us": round(t_s * 1e6, 2), "fused_us": round(t_f * 1e6, 2),
+            "speedup": round(t_s / t_f, 2),
+            "fused_gb_per_s": round(fused_bytes / t_f / 1e9, 1), "fused_algorithmic_bytes": fused_bytes,
+            "rel_l2_fused_vs_sequence": float(np.linalg.norm(g_f - g_s) / np.linalg.norm(g_s)),
+            "rel_l2_fused_vs_numpy_fp64": float(np.linalg.norm(g_f - want) / np.linalg.norm(want))}
+        lib.malldestroy_f(ds)
+        lib.malldestroy_f(df)
+        ks.destroy()
+        kf.destroy()
+    return out
+
+
 def config_c5(lib, nreq=2000, rate=20000.0):
     """Poisson requests at `rate`/s: rows U[1,512], lengths {256, 1000, 1024,
     2187, 4096, 16384}, fwd/inv 50/50, N(0,1) inputs from a device pool,
@@ -604,7 +641,8 @@ def main():
                          ("c3_strided_mixed_fp64", lambda: config_c3(lib, hip)),
                          ("c4_batched_uneven", lambda: config_c4(lib, hip)),
                          ("c5_served", lambda: config_c5(lib)),
-                         ("f1_fft1d", lambda: config_fft1d(lib, hip))):
+                         ("f1_fft1d", lambda: config_fft1d(lib, hip)),
+                         ("f3_komega", lambda: config_f3_komega(lib, hip))):
             if args.only and not any(name.startswith(o.strip()) for o in args.only.split(",") if o.strip()):
                 continue
             t0 = time.perf_counter()

@@ -200,6 +200,12 @@ void vsip_mcopyfrom_user_f(vsip_scalar_f* const, vsip_major, const vsip_mview_f*
 void vsip_mcopyfrom_user_d(vsip_scalar_d* const, vsip_major, const vsip_mview_d*);     /* vsip.h:1039 */
 void vsip_mcopyto_user_f(const vsip_mview_f*, vsip_major, vsip_scalar_f* const);       /* vsip.h:1042 */
 void vsip_mcopyto_user_d(const vsip_mview_d*, vsip_major, vsip_scalar_d* const);       /* vsip.h:1041 */
+/* real and imaginary parts of a complex matrix view (views of the complex
+ * block's derived real blocks) */
+vsip_mview_f* vsip_mrealview_f(const vsip_cmview_f*);                                  /* vsip.h:1914 */
+vsip_mview_d* vsip_mrealview_d(const vsip_cmview_d*);                                  /* vsip.h:1913 */
+vsip_mview_f* vsip_mimagview_f(const vsip_cmview_f*);                                  /* vsip.h:1924 */
+vsip_mview_d* vsip_mimagview_d(const vsip_cmview_d*);                                  /* vsip.h:1923 */
 
 /* ---- multiple FFT ------------------------------------------------------ */
 vsip_fftm_f* vsip_ccfftmip_create_f(vsip_length M, vsip_length N, vsip_scalar_f scale,
@@ -326,6 +332,20 @@ vsip_scalar_f vsip_cvmeansqval_f(const vsip_cvview_f*);                         
 vsip_scalar_d vsip_cvmeansqval_d(const vsip_cvview_d*);                                /* vsip.h:586 */
 vsip_scalar_f vsip_cmmeansqval_f(const vsip_cmview_f*);                                /* vsip.h:618 */
 vsip_scalar_d vsip_cmmeansqval_d(const vsip_cmview_d*);                                /* vsip.h:585 */
+
+/* ---- element operations of the k-omega beamformer (kw.c:7-25) ------------ */
+void vsip_vmmul_f(const vsip_vview_f*, const vsip_mview_f*, vsip_major, const vsip_mview_f*);  /* vsip.h:1520 */
+void vsip_vmmul_d(const vsip_vview_d*, const vsip_mview_d*, vsip_major, const vsip_mview_d*);  /* vsip.h:1519 */
+void vsip_mcmagsq_f(const vsip_cmview_f*, const vsip_mview_f*);                        /* vsip.h:1013 */
+void vsip_mcmagsq_d(const vsip_cmview_d*, const vsip_mview_d*);                        /* vsip.h:1012 */
+void vsip_smmul_f(vsip_scalar_f, const vsip_mview_f*, const vsip_mview_f*);            /* vsip.h:1226 */
+void vsip_smmul_d(vsip_scalar_d, const vsip_mview_d*, const vsip_mview_d*);            /* vsip.h:1225 */
+void vsip_madd_f(const vsip_mview_f*, const vsip_mview_f*, const vsip_mview_f*);       /* vsip.h:961 */
+void vsip_madd_d(const vsip_mview_d*, const vsip_mview_d*, const vsip_mview_d*);       /* vsip.h:960 */
+void vsip_mfill_f(vsip_scalar_f, const vsip_mview_f*);                                 /* vsip.h:1063 */
+void vsip_mfill_d(vsip_scalar_d, const vsip_mview_d*);                                 /* vsip.h:1062 */
+vsip_vview_f* vsip_vcreate_hanning_f(vsip_length, vsip_memory_hint);                   /* vsip.h:477 */
+vsip_vview_d* vsip_vcreate_hanning_d(vsip_length, vsip_memory_hint);                   /* vsip.h:473 */
 
 #ifdef __cplusplus
 }

@@ -113,6 +113,19 @@ int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_c
                            const vsip_cblock_f *out_pool, size_t hbm_budget, vsip_amd_result *res,
                            char *stats, size_t stats_len);
 
+/* One k-omega beamformer step, the komega() of the reference's example
+ * (examples/komegaExamples/kw_c/kw.c:7-25), in two fused launches:
+ *     gram += gain * |ccfftm( rcfftm( data .* ts_taper (rows) .* array_taper (columns) ) )|^2
+ * rcfftm: an rcfftmop ROW plan of data's shape (M x N, N even); ccfftm: a
+ * ccfftmip or ccfftmop COL plan of freq's shape (M x N/2+1), both single
+ * pass; freq: interleaved complex; gram: real, with freq's strides.  data is
+ * not modified (the reference tapers it in place); freq holds the row
+ * transforms of the tapered data afterwards.  Returns 0, or -1 with the
+ * reason in vsip_amd_last_error. */
+int vsip_amd_komega_f(const vsip_fftm_f *rcfftm, const vsip_fftm_f *ccfftm, const vsip_vview_f *ts_taper,
+                      const vsip_vview_f *array_taper, const vsip_mview_f *data, const vsip_cmview_f *freq,
+                      const vsip_mview_f *gram, vsip_scalar_f gain);
+
 #ifdef __cplusplus
 }
 #endif

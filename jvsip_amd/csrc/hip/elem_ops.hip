@@ -86,6 +86,35 @@ __global__ void __launch_bounds__(256) sumsq_kernel(const vhip_mat a, long m, lo
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// real element operations (VHIP_EOP_*); products and sums round separately
+// (no fused multiply-add), as the reference's scalar C loops do, so results
+// are bit-identical to them
+template <typename T>
+__global__ void __launch_bounds__(256) real_kernel(int op, const vhip_mat a, const vhip_mat b, const vhip_mat r,
+                                                   const T *vec, long vs, T alpha, long m, long n,
+                                                   bool rows_inner) {
+#pragma clang fp contract(off)
+    const long total = m * n;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        long i, j;
+        ij(t, m, n, rows_inner, i, j);
+        const long ro = i * r.cs + j * r.rs;
+        T y;
+        switch (op) {
+        case VHIP_EOP_FILL: y = alpha; break;
+        case VHIP_EOP_SMUL: y = alpha * ((const T *)a.re)[i * a.cs + j * a.rs]; break;
+        case VHIP_EOP_ADD: y = ((const T *)a.re)[i * a.cs + j * a.rs] + ((const T *)b.re)[i * b.cs + j * b.rs]; break;
+        case VHIP_EOP_VMUL_ROW: y = vec[j * vs] * ((const T *)a.re)[i * a.cs + j * a.rs]; break;
+        case VHIP_EOP_VMUL_COL: y = vec[i * vs] * ((const T *)a.re)[i * a.cs + j * a.rs]; break;
+        default: {   // VHIP_EOP_MAGSQ: a complex
+            const Cx<T> x = ld<T>(a, i * a.cs + j * a.rs);
+            y = x.r * x.r + x.i * x.i;
+        }
+        }
+        ((T *)r.re)[ro] = y;
+    }
+}
+
 long grid_for(long total) {
     int cus = 256;
     vhip_cu_count(&cus);
@@ -150,4 +179,24 @@ extern "C" int vhip_elem_sumsq(int prec, const vhip_mat *a, long m, long n, doub
     free(host);
     vhip_free(partial);
     return rc;
+}
+
+extern "C" int vhip_elem_real(int prec, int op, const vhip_mat *a, const vhip_mat *b, const vhip_mat *r,
+                              const void *vec, long vs, double alpha, long m, long n, void *stream) {
+    if (m <= 0 || n <= 0) return 0;
+    if (op < VHIP_EOP_FILL || op > VHIP_EOP_MAGSQ) return -1;
+    const long g = grid_for(m * n);
+    const bool ri = rows_inner_of(r);
+    const vhip_mat z = {};
+    const vhip_mat &aa = a ? *a : z, &bb = b ? *b : z;
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == 4)
+        hipLaunchKernelGGL(real_kernel<float>, dim3((unsigned)g), dim3(256), 0, s, op, aa, bb, *r,
+                           (const float *)vec, vs, (float)alpha, m, n, ri);
+    else if (prec == 8)
+        hipLaunchKernelGGL(real_kernel<double>, dim3((unsigned)g), dim3(256), 0, s, op, aa, bb, *r,
+                           (const double *)vec, vs, alpha, m, n, ri);
+    else
+        return -1;
+    return (int)hipGetLastError();
 }

@@ -329,6 +329,11 @@ __device__ __forceinline__ cpx<T> load_in(const vhip_fftm_desc &d, int vb, int j
             v.x = Buf<T>::lds(r, o, so);
             v.y = Buf<T>::lds(r, o + es * ES, so);
         }
+        if (d.tap) {
+            const long s0 = 2L * (j + leg) * d.tap_s;
+            v.x *= ((const T *)d.tap)[s0];
+            v.y *= ((const T *)d.tap)[s0 + d.tap_s];
+        }
         return v;
     }
     if (d.in.split) {
@@ -337,6 +342,11 @@ __device__ __forceinline__ cpx<T> load_in(const vhip_fftm_desc &d, int vb, int j
         v.y = Buf<T>::lds(make_rsrc(d.in.im, d.in.span), o, so);
     } else {
         v = Buf<T>::ldc(make_rsrc(d.in.re, d.in.span), (vb + (j + lv) * es) * 2 * ES, ls * es * 2 * ES);
+    }
+    if (d.tap) {
+        const T t = ((const T *)d.tap)[(long)(j + leg) * d.tap_s];
+        v.x *= t;
+        v.y *= t;
     }
     v.y *= isgn;
     return v;
@@ -379,6 +389,11 @@ __device__ __forceinline__ void store_out(const vhip_fftm_desc &d, int vb, int v
             Buf<T>::sts(r, o, so, v.x);
             Buf<T>::sts(r, o + es * ES, so, v.y);
         }
+        return;
+    }
+    if (d.acc) {                    // power accumulation in place of the store
+        T *a = (T *)d.acc + (vb + (j + leg) * es);
+        *a += (T)d.acc_gain * (v.x * v.x + v.y * v.y);
         return;
     }
     if (d.out.split) {

@@ -90,7 +90,7 @@ int p2_set_lds(int bytes) {
 // Launch the power-of-two row kernel when it applies; returns -100 when it
 // does not (the caller then uses the generic kernel).
 int pow2_try_launch(const vhip_fftm_desc *d, void *stream, int cus) {
-    if (d->prec != 4 || d->ptw) return -100;
+    if (d->prec != 4 || d->ptw || d->tap || d->acc) return -100;
     if (d->kind != VHIP_C2C) return pow2_real_try_launch(d, stream);
     if (!d->in.contig || !d->out.contig || d->in.split || d->out.split) return -100;
     if (d->map_in != VHIP_MAP_ALONG || d->map_out != VHIP_MAP_ALONG) return -100;

@@ -78,6 +78,18 @@ typedef struct {
     const void *ptw;               /* post-twiddle table e^{-2 pi i m / ptw_n}, or NULL */
     long ptw_n;
     vhip_view in, out;
+    /* fused element work of a pipeline (NULL when unused; only the generic
+     * kernel implements it):
+     *   tap   real weights applied on load, by element position within the
+     *         vector: R2C real sample s times tap[s * tap_s]; C2C element j
+     *         times tap[j * tap_s]
+     *   acc   C2C: instead of storing y, acc[o] += acc_gain * |y|^2 where o
+     *         is y's element offset in `out` (acc is real, laid out with the
+     *         element and vector strides of `out`) */
+    const void *tap;
+    long tap_s;
+    void *acc;
+    double acc_gain;
 } vhip_fftm_desc;
 
 /* runtime */
@@ -172,6 +184,22 @@ int vhip_elem_scale(int prec, const vhip_mat *a, const vhip_mat *r, long m, long
 int vhip_elem_sub(int prec, const vhip_mat *a, const vhip_mat *b, const vhip_mat *c, long m, long n,
                   void *stream);
 int vhip_elem_sumsq(int prec, const vhip_mat *a, long m, long n, double *out, void *stream);
+/* Real m x n element operations r = op(...), r real:
+ *   FILL      r = alpha
+ *   SMUL      r = alpha * a
+ *   ADD       r = a + b
+ *   VMUL_ROW  r_ij = vec[j * vs] * a_ij      (vector along each row)
+ *   VMUL_COL  r_ij = vec[i * vs] * a_ij      (vector down each column)
+ *   MAGSQ     r = |a|^2, a complex
+ * Products and sums are rounded separately (no fused multiply-add). */
+#define VHIP_EOP_FILL 0
+#define VHIP_EOP_SMUL 1
+#define VHIP_EOP_ADD 2
+#define VHIP_EOP_VMUL_ROW 3
+#define VHIP_EOP_VMUL_COL 4
+#define VHIP_EOP_MAGSQ 5
+int vhip_elem_real(int prec, int op, const vhip_mat *a, const vhip_mat *b, const vhip_mat *r, const void *vec,
+                   long vs, double alpha, long m, long n, void *stream);
 
 /* Kernels compiled at plan creation for one schedule (hipRTC): prepare
  * compiles and loads the kernel of (n, radices, vpb, threads, load layout,

@@ -43,6 +43,49 @@ static int va_block_alloc_dev(VA_block *b)
     return 0;
 }
 
+/* keep the derived real/imaginary blocks of complex block b pointing at b's
+ * current data and admission state */
+static void va_block_sync_parts(VA_block *b)
+{
+    int k;
+    for (k = 0; k < 2; k++) {
+        VA_block *s = b->part[k];
+        size_t off = (size_t)k * (size_t)b->prec;
+        if (!s) continue;
+        s->admitted = b->admitted;
+        s->size = b->size;
+        if (b->split) {
+            s->rstride = 1;
+            s->user_r = k ? b->user_i : b->user_r;
+            s->dev_r = k ? b->dev_i : b->dev_r;
+        } else {
+            s->rstride = 2;
+            s->user_r = b->user_r ? (void *)((char *)b->user_r + off) : NULL;
+            s->dev_r = b->dev_r ? (void *)((char *)b->dev_r + off) : NULL;
+        }
+    }
+}
+
+/* the real (imag = 0) or imaginary part of complex block b as a derived real
+ * block, made on first use and owned by b */
+VA_block *va_block_part(VA_block *b, int imag)
+{
+    VA_block *s;
+    if (!b || !b->cplx) return NULL;
+    if (b->part[imag]) return b->part[imag];
+    s = (VA_block *)calloc(1, sizeof(VA_block));
+    if (!s) return NULL;
+    s->prec = b->prec;
+    s->cplx = 0;
+    s->kind = VA_DERIVED_BLOCK;
+    s->markings = VA_VALID;
+    s->hint = b->hint;
+    s->parent = b;
+    b->part[imag] = s;
+    va_block_sync_parts(b);
+    return s;
+}
+
 VA_block *va_block_bind(int prec, int cplx, void *r, void *i, vsip_length n, vsip_memory_hint h)
 {
     VA_block *b;
@@ -97,6 +140,7 @@ int va_block_admit(VA_block *b, int update)
         if (rc) { va_error("blockadmit", "host-to-device copy failed", rc); return 1; }
     }
     b->admitted = 1;
+    va_block_sync_parts(b);
     return 0;
 }
 
@@ -116,6 +160,7 @@ int va_block_release(VA_block *b, int update, void **r, void **i)
         vhip_sync();
     }
     b->admitted = 0;
+    va_block_sync_parts(b);
     if (r) *r = b->user_r;
     if (i) *i = b->split ? b->user_i : NULL;
     return 0;
@@ -123,7 +168,14 @@ int va_block_release(VA_block *b, int update, void **r, void **i)
 
 void va_block_destroy(VA_block *b)
 {
-    if (!b) return;
+    int k;
+    if (!b || b->kind == VA_DERIVED_BLOCK) return;   /* parts go with their complex block */
+    for (k = 0; k < 2; k++) {
+        if (b->part[k]) {
+            b->part[k]->markings = VA_FREED;
+            free(b->part[k]);
+        }
+    }
     if (b->dev_r || b->dev_i) vhip_sync();
     if (b->dev_r) vhip_free(b->dev_r);
     if (b->dev_i) vhip_free(b->dev_i);
@@ -206,6 +258,7 @@ static void *va_rebind(VA_block *b, void *r, void *i, void **old_i)
     }
     b->user_r = r;
     b->user_i = b->cplx ? i : NULL;
+    va_block_sync_parts(b);
     return old;
 }
 

@@ -248,10 +248,13 @@ int va_fftm_init_inner(VA_fftm *p, int prec, vsip_length M, vsip_length N)
 static void va_axis(const VA_mview *v, vsip_major major, long *es, long *vs,
                     vsip_length *n, vsip_length *nvec)
 {
+    /* strides in scalars of the block's arrays: x VA_RS for the real or
+     * imaginary part of an interleaved complex block */
+    const long rs = VA_RS(v->block);
     if (major == VSIP_ROW) {
-        *es = v->row_stride; *vs = v->col_stride; *n = v->row_length; *nvec = v->col_length;
+        *es = v->row_stride * rs; *vs = v->col_stride * rs; *n = v->row_length; *nvec = v->col_length;
     } else {
-        *es = v->col_stride; *vs = v->row_stride; *n = v->col_length; *nvec = v->row_length;
+        *es = v->col_stride * rs; *vs = v->row_stride * rs; *n = v->col_length; *nvec = v->row_length;
     }
 }
 
@@ -269,7 +272,7 @@ static int va_view_dev(const VA_mview *v, vhip_view *g)
         g->im = (char *)b->dev_i + (size_t)v->offset * p;
         g->split = 1;
     } else {
-        g->re = (char *)b->dev_r + (size_t)v->offset * p;
+        g->re = (char *)b->dev_r + (size_t)v->offset * (size_t)VA_RS(b) * p;
     }
     return 0;
 }
@@ -389,11 +392,13 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
         gi.es = es_i; gi.vs0 = vs_i; gi.nv0 = (long)p->mN; gi.vs1 = 0;
         go.es = es_o; go.vs0 = vs_o; go.nv0 = (long)p->mN; go.vs1 = 0;
         if (kind == VHIP_R2C)
-            gi.contig = (es_i == 1 && (in->offset % 2) == 0 && (p->mN <= 1 || vs_i % 2 == 0));
+            gi.contig = (es_i == 1 && ((long)in->offset * VA_RS(in->block)) % 2 == 0 &&
+                         (p->mN <= 1 || vs_i % 2 == 0));
         else
             gi.contig = (!gi.split && es_i == 1);
         if (kind == VHIP_C2R)
-            go.contig = (es_o == 1 && (out->offset % 2) == 0 && (p->mN <= 1 || vs_o % 2 == 0));
+            go.contig = (es_o == 1 && ((long)out->offset * VA_RS(out->block)) % 2 == 0 &&
+                         (p->mN <= 1 || vs_o % 2 == 0));
         else
             go.contig = (!go.split && es_o == 1);
         gi.span = va_span(&gi, (long)p->mN, (long)(kind == VHIP_R2C ? 2 * p->MN : kind == VHIP_C2R ? p->MN + 1 : p->MN),

@@ -28,6 +28,7 @@
 
 #define VA_USER_BLOCK 1
 #define VA_LIB_BLOCK 0
+#define VA_DERIVED_BLOCK 2   /* real or imaginary part of a complex block */
 #define VA_VALID 0x5555
 #define VA_FREED 0xAAAA
 
@@ -44,7 +45,17 @@ typedef struct VA_block {
     void *dev_r;           /* device data, same layout as the user data */
     void *dev_i;
     vsip_memory_hint hint;
+    /* derived blocks (the real and imaginary parts of a complex block, as
+     * vsip_mrealview/mimagview see them; vsip_cblockattributes_f.h:42-67 R
+     * and I): element e of the derived block is scalar e * rstride of
+     * user_r/dev_r, which the parent keeps pointing into its own data */
+    int rstride;                   /* 0 or 1: unit; 2: part of an interleaved block */
+    struct VA_block *parent;       /* derived blocks: the complex block */
+    struct VA_block *part[2];      /* complex blocks: real and imaginary parts, made on demand */
 } VA_block;
+
+/* scalar stride of a block's elements within its arrays */
+#define VA_RS(b) ((b)->rstride > 1 ? (long)(b)->rstride : 1L)
 
 typedef struct VA_mview {
     VA_block *block;
@@ -154,6 +165,7 @@ VA_block *va_block_create(int prec, int cplx, vsip_length n, vsip_memory_hint h)
 int va_block_admit(VA_block *b, int update);
 int va_block_release(VA_block *b, int update, void **r, void **i);
 void va_block_destroy(VA_block *b);
+VA_block *va_block_part(VA_block *b, int imag);
 
 /* views */
 VA_mview *va_mview_bind(VA_block *b, vsip_offset o, vsip_stride cs, vsip_length cl,

@@ -76,7 +76,7 @@ static int va_elem_addr(const VA_mview *v, vsip_index i, vsip_index j, void **r,
         *r = base_r + (size_t)k * 2 * p;
         *im = base_r + (size_t)k * 2 * p + p;
     } else {
-        *r = base_r + (size_t)k * p;
+        *r = base_r + (size_t)k * (size_t)VA_RS(b) * p;
         *im = b->cplx ? (void *)(base_i + (size_t)k * p) : NULL;
     }
     return 0;
@@ -141,15 +141,15 @@ void va_view_mat(const VA_mview *v, int dev, vhip_mat *m)
     size_t p = (size_t)b->prec;
     char *br = (char *)(dev ? b->dev_r : b->user_r);
     char *bi = (char *)(dev ? b->dev_i : b->user_i);
-    m->rs = v->row_stride;
-    m->cs = v->col_stride;
+    m->rs = v->row_stride * VA_RS(b);
+    m->cs = v->col_stride * VA_RS(b);
     m->split = b->cplx && b->split;
     m->pad_ = 0;
     if (b->cplx && !b->split) {
         m->re = br + (size_t)v->offset * 2 * p;
         m->im = NULL;
     } else {
-        m->re = br + (size_t)v->offset * p;
+        m->re = br + (size_t)v->offset * (size_t)VA_RS(b) * p;
         m->im = (b->cplx && bi) ? (void *)(bi + (size_t)v->offset * p) : NULL;
     }
 }
@@ -272,6 +272,21 @@ vsip_cmview_d *vsip_cmtransview_d(const vsip_cmview_d *v)
     return (vsip_cmview_d *)va_mview_bind(v->v.block, v->v.offset, v->v.row_stride,
                                           v->v.row_length, v->v.col_stride, v->v.col_length);
 }
+
+/* real / imaginary part of a complex matrix view: the same offset, strides
+ * and lengths over the complex block's derived real block
+ * (VI_mrealview_f.h:20-33, vsip_mimagview_f.c) */
+static VA_mview *va_part_view(const VA_mview *v, int imag)
+{
+    VA_block *pb = v ? va_block_part(v->block, imag) : NULL;
+    if (!pb) return NULL;
+    return va_mview_bind(pb, v->offset, v->col_stride, v->col_length, v->row_stride, v->row_length);
+}
+
+vsip_mview_f *vsip_mrealview_f(const vsip_cmview_f *v) { return (vsip_mview_f *)va_part_view(v ? &v->v : NULL, 0); }
+vsip_mview_d *vsip_mrealview_d(const vsip_cmview_d *v) { return (vsip_mview_d *)va_part_view(v ? &v->v : NULL, 0); }
+vsip_mview_f *vsip_mimagview_f(const vsip_cmview_f *v) { return (vsip_mview_f *)va_part_view(v ? &v->v : NULL, 1); }
+vsip_mview_d *vsip_mimagview_d(const vsip_cmview_d *v) { return (vsip_mview_d *)va_part_view(v ? &v->v : NULL, 1); }
 
 vsip_cblock_f *vsip_cmdestroy_f(vsip_cmview_f *v)
 {

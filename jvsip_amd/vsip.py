@@ -135,6 +135,15 @@ def _sig(t):
         "cvmeansqval": (sc, [P]),
         "rscvmul": (None, [sc, P, P]),
         "rscmmul": (None, [sc, P, P]),
+        # real element operations and part views (k-omega beamformer)
+        "vmmul": (None, [P, P, c_bl, P]),
+        "mcmagsq": (None, [P, P]),
+        "smmul": (None, [sc, P, P]),
+        "madd": (None, [P, P, P]),
+        "mfill": (None, [sc, P]),
+        "vcreate_hanning": (P, [c_vi, c_bl]),
+        "mrealview": (P, [P]),
+        "mimagview": (P, [P]),
         # vector views and the 1-D FFT
         "vbind": (P, [P, c_vi, c_st, c_vi]),
         "vcreate": (P, [c_vi, c_bl]),
@@ -230,6 +239,7 @@ class AmdLib(VsipLib):
             "vsip_amd_serve_plain_f": (ctypes.c_int, [ctypes.POINTER(amd_request), ctypes.c_long, P, P,
                                                       ctypes.c_size_t, ctypes.POINTER(amd_result),
                                                       ctypes.c_char_p, ctypes.c_size_t]),
+            "vsip_amd_komega_f": (ctypes.c_int, [P, P, P, P, P, P, P, ctypes.c_float]),
         }
         for name, (res, args) in ext.items():
             fn = getattr(d, name)

@@ -92,6 +92,41 @@ def crfftm(zblk, zview, xblk, xview, major=0, scale=1.0):
     return x
 
 
+def hanning(n, prec="f"):
+    """vsip_vcreate_hanning_f/_d (c_VSIP_src/vsip_vcreate_hanning_f.c:21-37,
+    vsip_vcreate_hanning_d.c:21-37): w[k-1] = 0.5 (1 - cos(k t)), k = 1..n,
+    t = 6.2831853071796 / (n + 1); the _f form rounds t and k t to float and
+    takes the cosine in double."""
+    k = np.arange(1, n + 1)
+    if prec == "f":
+        t = np.float32(6.2831853071796) / np.float32(n + 1)
+        x = (t * k.astype(np.float32)).astype(np.float32)
+        c = np.cos(x.astype(np.float64)).astype(np.float32)
+        return (np.float32(0.5) * (np.float32(1.0) - c)).astype(np.float32)
+    t = 6.2831853071796 / (n + 1)
+    return 0.5 * (1.0 - np.cos(t * k.astype(np.float64)))
+
+
+def komega(data, ts_taper, array_taper, gram, gain, rc_scale=1.0, cc_scale=-1.0, cc_dir=1):
+    """One komega() step of the reference's k-omega beamformer
+    (examples/komegaExamples/kw_c/kw.c:7-25) in double precision:
+    taper every row by ts_taper and every column by array_taper
+    (vsip_vmmul_f ROW, COL), rcfftmop ROW into an M x (N/2+1) COL-major
+    matrix, ccfftmip COL with kw_init's arguments (kw.c:52 passes scale
+    VSIP_FFT_FWD = -1 and dir 1, i.e. the inverse sign), then
+    gram + gain * |.|^2 (mcmagsq, smmul, madd).  data: M x N real; returns the
+    new gram (M x N/2+1, float64)."""
+    data = np.asarray(data, dtype=np.float64)
+    m, n = data.shape
+    nf = n // 2 + 1
+    x = data * np.asarray(ts_taper, np.float64)[None, :] * np.asarray(array_taper, np.float64)[:, None]
+    z = np.zeros(m * nf, np.complex128)
+    z = rcfftm(x.ravel(), (0, n, m, 1, n), z, (0, 1, m, m, nf), 0, rc_scale)
+    y = ccfftm(z, (0, 1, m, m, nf), 1, cc_dir, cc_scale)
+    y = y.reshape(nf, m).T                       # COL-major block -> (m, nf)
+    return np.asarray(gram, np.float64) + gain * (y.real ** 2 + y.imag ** 2)
+
+
 _REF = None
 
 
