@@ -44,6 +44,7 @@ typedef signed long int vsip_stride;
 typedef struct { vsip_scalar_vi r, c; } vsip_scalar_mi;
 typedef struct { vsip_scalar_d r, i; } vsip_cscalar_d;
 typedef struct { vsip_scalar_f r, i; } vsip_cscalar_f;
+typedef unsigned int vsip_scalar_ue32;                                   /* vsip.h:123 */
 
 #define VSIP_TRUE 1
 #define VSIP_FALSE 0
@@ -57,6 +58,7 @@ typedef enum { VSIP_ROW = 0, VSIP_COL = 1 } vsip_major;
 typedef enum { VSIP_FFT_FWD = -1, VSIP_FFT_INV = +1 } vsip_fft_dir;
 typedef enum { VSIP_FFT_IP = 0, VSIP_FFT_OP = 1 } vsip_fft_place;
 typedef enum { VSIP_ALG_SPACE = 0, VSIP_ALG_TIME = 1, VSIP_ALG_NOISE = 2 } vsip_alg_hint;
+typedef enum { VSIP_PRNG = 0, VSIP_NPRNG = 1 } vsip_rng;                  /* vsip.h:159 */
 
 /* ---- opaque handles (vsip.h:165-208) -------------------------------- */
 typedef struct vsip_blockattributes_d vsip_block_d;
@@ -75,6 +77,7 @@ typedef struct vsip_cvviewattributes_d vsip_cvview_d;
 typedef struct vsip_cvviewattributes_f vsip_cvview_f;
 typedef struct vsip_fftattributes_f vsip_fft_f;
 typedef struct vsip_fftattributes_d vsip_fft_d;
+typedef struct vsip_randobject vsip_randstate;                           /* vsip.h:191 */
 
 /* ---- attribute structs (vsip.h:242-245, 277-278) -------------------- */
 typedef struct { vsip_offset offset; vsip_stride row_stride; vsip_length row_length;
@@ -346,6 +349,34 @@ void vsip_mfill_f(vsip_scalar_f, const vsip_mview_f*);                          
 void vsip_mfill_d(vsip_scalar_d, const vsip_mview_d*);                                 /* vsip.h:1062 */
 vsip_vview_f* vsip_vcreate_hanning_f(vsip_length, vsip_memory_hint);                   /* vsip.h:477 */
 vsip_vview_d* vsip_vcreate_hanning_d(vsip_length, vsip_memory_hint);                   /* vsip.h:473 */
+
+/* ---- random numbers (state on the host, vector/matrix fills on the device) */
+vsip_randstate* vsip_randcreate(vsip_index, vsip_index, vsip_index, vsip_rng);          /* vsip.h:1826 */
+int vsip_randdestroy(vsip_randstate*);                                                 /* vsip.h:451 */
+vsip_scalar_f vsip_randu_f(vsip_randstate*);                                           /* vsip.h:633 */
+vsip_scalar_d vsip_randu_d(vsip_randstate*);                                           /* vsip.h:600 */
+vsip_scalar_f vsip_randn_f(vsip_randstate*);                                           /* vsip.h:632 */
+vsip_scalar_d vsip_randn_d(vsip_randstate*);                                           /* vsip.h:599 */
+vsip_cscalar_f vsip_crandu_f(vsip_randstate*);                                         /* vsip.h:560 */
+vsip_cscalar_d vsip_crandu_d(vsip_randstate*);                                         /* vsip.h:531 */
+vsip_cscalar_f vsip_crandn_f(vsip_randstate*);                                         /* vsip.h:559 */
+vsip_cscalar_d vsip_crandn_d(vsip_randstate*);                                         /* vsip.h:530 */
+void vsip_vrandu_f(vsip_randstate*, const vsip_vview_f*);                              /* vsip.h:1573 */
+void vsip_vrandu_d(vsip_randstate*, const vsip_vview_d*);                              /* vsip.h:1572 */
+void vsip_vrandn_f(vsip_randstate*, const vsip_vview_f*);                              /* vsip.h:1571 */
+void vsip_vrandn_d(vsip_randstate*, const vsip_vview_d*);                              /* vsip.h:1570 */
+void vsip_cvrandu_f(vsip_randstate*, const vsip_cvview_f*);                            /* vsip.h:919 */
+void vsip_cvrandu_d(vsip_randstate*, const vsip_cvview_d*);                            /* vsip.h:918 */
+void vsip_cvrandn_f(vsip_randstate*, const vsip_cvview_f*);                            /* vsip.h:917 */
+void vsip_cvrandn_d(vsip_randstate*, const vsip_cvview_d*);                            /* vsip.h:916 */
+void vsip_mrandu_f(vsip_randstate*, const vsip_mview_f*);                              /* vsip.h:1577 */
+void vsip_mrandu_d(vsip_randstate*, const vsip_mview_d*);                              /* vsip.h:1576 */
+void vsip_mrandn_f(vsip_randstate*, const vsip_mview_f*);                              /* vsip.h:1575 */
+void vsip_mrandn_d(vsip_randstate*, const vsip_mview_d*);                              /* vsip.h:1574 */
+void vsip_cmrandu_f(vsip_randstate*, const vsip_cmview_f*);                            /* vsip.h:923 */
+void vsip_cmrandu_d(vsip_randstate*, const vsip_cmview_d*);                            /* vsip.h:922 */
+void vsip_cmrandn_f(vsip_randstate*, const vsip_cmview_f*);                            /* vsip.h:921 */
+void vsip_cmrandn_d(vsip_randstate*, const vsip_cmview_d*);                            /* vsip.h:920 */
 
 #ifdef __cplusplus
 }

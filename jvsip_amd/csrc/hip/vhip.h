@@ -201,6 +201,27 @@ int vhip_elem_sumsq(int prec, const vhip_mat *a, long m, long n, double *out, vo
 int vhip_elem_real(int prec, int op, const vhip_mat *a, const vhip_mat *b, const vhip_mat *r, const void *vec,
                    long vs, double alpha, long m, long n, void *stream);
 
+/* One fill of a strided m x n view by a VSIPL generator (rng.hip).  Element
+ * e (column-major order when by_col, else row-major) takes draws
+ * D e + 1 .. D e + D of this launch.  x0 / x10 are the generator states
+ * before the first draw; (ja, jc) and (ja1, jc1) the maps of 2^i steps;
+ * stride_* the maps of `lanes` elements' draws (one grid stride); kstar the
+ * 1-based draw at which X1 meets its marker x2 (then X1 continues from
+ * x2 + 1), 0 when that does not happen in this launch. */
+#define VHIP_RNG_UNIFORM 0
+#define VHIP_RNG_NORMAL 1
+typedef struct {
+    int prec, dist, cplx, portable, by_col;
+    int p24;          /* portable draw t as (t >> 8 | 1) / 2^24, else t / 2^32 */
+    long m, n, lanes, kstar;
+    unsigned a, c, a1, c1, x0, x10, x2;
+    unsigned stride_a, stride_c, stride_a1, stride_c1;
+    unsigned ja[32], jc[32], ja1[32], jc1[32];
+    vhip_mat r;
+} vhip_rng_desc;
+long vhip_rng_lanes(long total);
+int vhip_rng_fill(const vhip_rng_desc *d, void *stream);
+
 /* Kernels compiled at plan creation for one schedule (hipRTC): prepare
  * compiles and loads the kernel of (n, radices, vpb, threads, load layout,
  * store layout) once per process (0 when available); try_launch runs a

@@ -144,6 +144,19 @@ def _sig(t):
         "vcreate_hanning": (P, [c_vi, c_bl]),
         "mrealview": (P, [P]),
         "mimagview": (P, [P]),
+        # random numbers
+        "randu": (sc, [P]),
+        "randn": (sc, [P]),
+        "crandu": (cs, [P]),
+        "crandn": (cs, [P]),
+        "vrandu": (None, [P, P]),
+        "vrandn": (None, [P, P]),
+        "cvrandu": (None, [P, P]),
+        "cvrandn": (None, [P, P]),
+        "mrandu": (None, [P, P]),
+        "mrandn": (None, [P, P]),
+        "cmrandu": (None, [P, P]),
+        "cmrandn": (None, [P, P]),
         # vector views and the 1-D FFT
         "vbind": (P, [P, c_vi, c_st, c_vi]),
         "vcreate": (P, [c_vi, c_bl]),
@@ -201,6 +214,15 @@ class VsipLib:
             fn = getattr(self.dll, name)
             fn.restype = ctypes.c_int
             fn.argtypes = [P]
+            setattr(self, name[5:], fn)
+        for name, res, args in (("vsip_randcreate", P, [c_vi, c_vi, c_vi, c_bl]),
+                                ("vsip_randdestroy", ctypes.c_int, [P])):
+            fn = getattr(self.dll, name, None)
+            if fn is None:
+                self.missing.append(name)
+                continue
+            fn.restype = res
+            fn.argtypes = args
             setattr(self, name[5:], fn)
         self.init(None)
 
