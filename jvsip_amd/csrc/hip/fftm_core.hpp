@@ -265,33 +265,49 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *p, long bytes) {
 }
 typedef decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc_t(), 0, 0, 0)) raw64_t;
 typedef decltype(__builtin_amdgcn_raw_buffer_load_b128(rsrc_t(), 0, 0, 0)) raw128_t;
+// Cache policy (the buffer instructions' aux field) of the data's global
+// loads and stores.  Non-temporal (2, stores 18) streams contiguous rows
+// faster (the power-of-two c2c kernel uses it), but these kernels also
+// serve column layouts, whose workgroups write 32-byte pieces of lines that
+// neighbouring workgroups complete in L2, and there it measured up to 1.75x
+// slower (COL 1920 x 1000); they keep the default policy.
+constexpr int kLdData = 0;
+constexpr int kStData = 0;
 template <typename T> struct Buf;
 template <> struct Buf<float> {
+    template <int AUX = 0>
     static __device__ __forceinline__ cpx<float> ldc(rsrc_t r, int v, int s) {
-        return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+        return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, AUX));
     }
+    template <int AUX = 0>
     static __device__ __forceinline__ float lds(rsrc_t r, int v, int s) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, AUX));
     }
+    template <int AUX = 0>
     static __device__ __forceinline__ void stc(rsrc_t r, int v, int s, cpx<float> x) {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(raw64_t, x), r, v, s, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(raw64_t, x), r, v, s, AUX);
     }
+    template <int AUX = 0>
     static __device__ __forceinline__ void sts(rsrc_t r, int v, int s, float x) {
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s, AUX);
     }
 };
 template <> struct Buf<double> {
+    template <int AUX = 0>
     static __device__ __forceinline__ cpx<double> ldc(rsrc_t r, int v, int s) {
-        return __builtin_bit_cast(cpx<double>, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+        return __builtin_bit_cast(cpx<double>, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, AUX));
     }
+    template <int AUX = 0>
     static __device__ __forceinline__ double lds(rsrc_t r, int v, int s) {
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, AUX));
     }
+    template <int AUX = 0>
     static __device__ __forceinline__ void stc(rsrc_t r, int v, int s, cpx<double> x) {
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(raw128_t, x), r, v, s, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(raw128_t, x), r, v, s, AUX);
     }
+    template <int AUX = 0>
     static __device__ __forceinline__ void sts(rsrc_t r, int v, int s, double x) {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(raw64_t, x), r, v, s, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(raw64_t, x), r, v, s, AUX);
     }
 };
 
@@ -323,11 +339,11 @@ __device__ __forceinline__ cpx<T> load_in(const vhip_fftm_desc &d, int vb, int j
     if (KIND == VHIP_R2C) {
         const rsrc_t r = make_rsrc(d.in.re, d.in.span);
         if (d.in.contig) {
-            v = Buf<T>::ldc(r, vb * ES + (j + lv) * 2 * ES, ls * 2 * ES);
+            v = Buf<T>::template ldc<kLdData>(r, vb * ES + (j + lv) * 2 * ES, ls * 2 * ES);
         } else {
             const int o = (vb + 2 * (j + lv) * es) * ES, so = 2 * ls * es * ES;
-            v.x = Buf<T>::lds(r, o, so);
-            v.y = Buf<T>::lds(r, o + es * ES, so);
+            v.x = Buf<T>::template lds<kLdData>(r, o, so);
+            v.y = Buf<T>::template lds<kLdData>(r, o + es * ES, so);
         }
         if (d.tap) {
             const long s0 = 2L * (j + leg) * d.tap_s;
@@ -338,10 +354,10 @@ __device__ __forceinline__ cpx<T> load_in(const vhip_fftm_desc &d, int vb, int j
     }
     if (d.in.split) {
         const int o = (vb + (j + lv) * es) * ES, so = ls * es * ES;
-        v.x = Buf<T>::lds(make_rsrc(d.in.re, d.in.span), o, so);
-        v.y = Buf<T>::lds(make_rsrc(d.in.im, d.in.span), o, so);
+        v.x = Buf<T>::template lds<kLdData>(make_rsrc(d.in.re, d.in.span), o, so);
+        v.y = Buf<T>::template lds<kLdData>(make_rsrc(d.in.im, d.in.span), o, so);
     } else {
-        v = Buf<T>::ldc(make_rsrc(d.in.re, d.in.span), (vb + (j + lv) * es) * 2 * ES, ls * es * 2 * ES);
+        v = Buf<T>::template ldc<kLdData>(make_rsrc(d.in.re, d.in.span), (vb + (j + lv) * es) * 2 * ES, ls * es * 2 * ES);
     }
     if (d.tap) {
         const T t = ((const T *)d.tap)[(long)(j + leg) * d.tap_s];
@@ -383,11 +399,11 @@ __device__ __forceinline__ void store_out(const vhip_fftm_desc &d, int vb, int v
     if (KIND == VHIP_C2R) {
         const rsrc_t r = make_rsrc(d.out.re, d.out.span);
         if (d.out.contig) {
-            Buf<T>::stc(r, vb * ES + (j + lv) * 2 * ES, ls * 2 * ES, v);
+            Buf<T>::template stc<kStData>(r, vb * ES + (j + lv) * 2 * ES, ls * 2 * ES, v);
         } else {
             const int o = (vb + 2 * (j + lv) * es) * ES, so = 2 * ls * es * ES;
-            Buf<T>::sts(r, o, so, v.x);
-            Buf<T>::sts(r, o + es * ES, so, v.y);
+            Buf<T>::template sts<kStData>(r, o, so, v.x);
+            Buf<T>::template sts<kStData>(r, o + es * ES, so, v.y);
         }
         return;
     }
@@ -398,10 +414,10 @@ __device__ __forceinline__ void store_out(const vhip_fftm_desc &d, int vb, int v
     }
     if (d.out.split) {
         const int o = (vb + (j + lv) * es) * ES, so = ls * es * ES;
-        Buf<T>::sts(make_rsrc(d.out.re, d.out.span), o, so, v.x);
-        Buf<T>::sts(make_rsrc(d.out.im, d.out.span), o, so, v.y);
+        Buf<T>::template sts<kStData>(make_rsrc(d.out.re, d.out.span), o, so, v.x);
+        Buf<T>::template sts<kStData>(make_rsrc(d.out.im, d.out.span), o, so, v.y);
     } else {
-        Buf<T>::stc(make_rsrc(d.out.re, d.out.span), (vb + (j + lv) * es) * 2 * ES, ls * es * 2 * ES, v);
+        Buf<T>::template stc<kStData>(make_rsrc(d.out.re, d.out.span), (vb + (j + lv) * es) * 2 * ES, ls * es * 2 * ES, v);
     }
 }
 

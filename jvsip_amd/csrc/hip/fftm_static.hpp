@@ -85,8 +85,8 @@ __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d,
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     cpx<float> y;
-                    if (MIN == ST_CONTIG) y = Buf<float>::ldc(io.in, (int)(v * io.ivs + j) * 8, r * NBV * 8);
-                    else y = Buf<float>::ldc(io.in, (io.voff_in[v] + (j + r * NBV) * io.ies) * 8, 0);
+                    if (MIN == ST_CONTIG) y = Buf<float>::ldc<kLdData>(io.in, (int)(v * io.ivs + j) * 8, r * NBV * 8);
+                    else y = Buf<float>::ldc<kLdData>(io.in, (io.voff_in[v] + (j + r * NBV) * io.ies) * 8, 0);
                     y.y *= io.isgn;
                     x[i][r] = y;
                 }
@@ -117,8 +117,8 @@ __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d,
                     if (d.ptw)      // first pass of the two-pass split: e^{-2 pi i j2 k1 / n}
                         y = cmul(y, ((const cpx<float> *)d.ptw)[io.vidx[v] * (j + r * NBV)]);
                     const cpx<float> z = {y.x * io.scale, y.y * (io.scale * io.osgn)};
-                    if (MOUT == ST_CONTIG) Buf<float>::stc(io.out, (int)(v * io.ovs + j) * 8, r * NBV * 8, z);
-                    else Buf<float>::stc(io.out, (io.voff_out[v] + (j + r * NBV) * io.oes) * 8, 0, z);
+                    if (MOUT == ST_CONTIG) Buf<float>::stc<kStData>(io.out, (int)(v * io.ovs + j) * 8, r * NBV * 8, z);
+                    else Buf<float>::stc<kStData>(io.out, (io.voff_out[v] + (j + r * NBV) * io.oes) * 8, 0, z);
                 }
             } else {
                 const int base = (j - k) * R + k;
