@@ -39,11 +39,13 @@ void *vsip_amd_cblock_device_d(const vsip_cblock_d *b, void **im);
 void *vsip_amd_block_device_f(const vsip_block_f *b);
 void *vsip_amd_block_device_d(const vsip_block_d *b);
 
-/* HBM reference rates on this device: four buffers of `bytes` each visited
- * in turn (so caches do not serve the data), `reps` visits timed with
- * stream events; *rmw_gbs gets the in-place read-modify-write rate (bytes
- * read + written per second / 1e9, the traffic pattern of an in-place
- * transform), *read_gbs the read-only rate.  Returns 0 or an error code. */
+/* HBM reference rates on this device: four buffers of `bytes` each (rounded
+ * down to whole 32 KiB rows) visited in turn (so caches do not serve the
+ * data), `reps` visits timed with stream events; *rmw_gbs gets the
+ * in-place read-modify-write rate (bytes read + written per second / 1e9)
+ * of the in-place transforms' pattern -- each workgroup reads a 32 KiB row,
+ * synchronises and writes it back, with non-temporal loads and stores --
+ * and *read_gbs the read-only rate.  Returns 0 or an error code. */
 int vsip_amd_hbm_probe(size_t bytes, int reps, double *rmw_gbs, double *read_gbs);
 
 /* Single-pass schedule the planner picks for `nvec` transforms of length n

@@ -276,11 +276,26 @@ extern "C" int vhip_event_time(void *stream, int (*fn)(void *arg), void *arg, in
 // HBM reference kernels: in-place read-modify-write and read-only streams
 // ---------------------------------------------------------------------------
 namespace {
-__global__ void probe_rmw(float4 *p, long n) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        float4 v = p[i];
-        v.x *= 1.0000001f;
-        p[i] = v;
+// In-place read+write in the FFT kernels' pattern, at the best cache policy
+// measured for it: one 32 KiB row per workgroup of 256 lanes, each lane
+// reads 16 8-byte values (non-temporal), the workgroup synchronises, each
+// lane writes them back (non-temporal, sc1).  On MI355X this streams
+// 6.3 TB/s, against 5.1-5.2 TB/s for a grid-stride float4 loop at the
+// default policy.
+typedef decltype(__builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_rsrc_t(), 0, 0, 0)) probe_raw64_t;
+__global__ void __launch_bounds__(256) probe_rmw(float *p) {
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(p + (long)blockIdx.x * 8192, (short)0, 32768, 0x00020000);
+    float2 x[16];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        x[k] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (t + k * 256) * 8, 0, 2));
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        x[k].x *= 1.0000001f;
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(probe_raw64_t, x[k]), r, (t + k * 256) * 8, 0, 18);
     }
 }
 __global__ void probe_read(const float4 *p, long n, float *sink) {
@@ -296,7 +311,11 @@ __global__ void probe_read(const float4 *p, long n, float *sink) {
 extern "C" int vhip_probe_launch(int kind, void *buf, size_t bytes, void *sink, void *stream) {
     const long n = (long)(bytes / 16);
     hipStream_t s = stream ? (hipStream_t)stream : g_stream;
-    if (kind == 0) hipLaunchKernelGGL(probe_rmw, dim3(32768), dim3(256), 0, s, (float4 *)buf, n);
+    if (kind == 0) {
+        const long rows = (long)(bytes / 32768);     // whole 32 KiB rows only
+        if (rows < 1 || rows > 0x7fffffffL) return -1;
+        hipLaunchKernelGGL(probe_rmw, dim3((unsigned)rows), dim3(256), 0, s, (float *)buf);
+    }
     else hipLaunchKernelGGL(probe_read, dim3(32768), dim3(256), 0, s, (const float4 *)buf, n, (float *)sink);
     return (int)hipGetLastError();
 }

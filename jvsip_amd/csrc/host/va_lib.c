@@ -142,6 +142,8 @@ int vsip_amd_hbm_probe(size_t bytes, int reps, double *rmw_gbs, double *read_gbs
     float ms = 0.f;
     int k, rc = 0;
     *rmw_gbs = *read_gbs = 0.0;
+    bytes -= bytes % 32768;               /* the in-place probe moves whole 32 KiB rows */
+    if (bytes == 0) return -1;
     if (va_ensure_init()) return va_init_rc;
     memset(&p, 0, sizeof(p));
     p.bytes = bytes;
