@@ -391,8 +391,9 @@ __device__ __forceinline__ void store_out(const vhip_fftm_desc &d, int vb, int v
     const int lv = SOFF ? 0 : leg;
     const int ls = SOFF ? leg : 0;
     if (d.ptw) {
-        const cpx<T> w = ((const cpx<T> *)d.ptw)[vi0 * (j + leg)];
-        v = cmul(v, w);
+        const int t = vi0 * (j + leg);
+        const cpx<T> *w = (const cpx<T> *)d.ptw;
+        v = cmul(v, cmul(w[t & ((1 << d.ptw_shift) - 1)], w[(1 << d.ptw_shift) + (t >> d.ptw_shift)]));
     }
     v.x *= scale;
     v.y *= scale * osgn;
@@ -616,7 +617,7 @@ __device__ __forceinline__ void fft_body(const vhip_fftm_desc &d, long vbase, in
         const long v = vbase + t;
         sh.voff_in[t] = vec_off(d.in, v);
         sh.voff_out[t] = vec_off(d.out, v);
-        sh.vidx0[t] = (int)(d.out.nv0 > 0 ? v % d.out.nv0 : v);
+        sh.vidx0[t] = (int)(d.ptw_n > 0 ? v % d.ptw_n : v);
     }
     __syncthreads();
 

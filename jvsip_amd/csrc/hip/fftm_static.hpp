@@ -114,8 +114,11 @@ __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d,
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     cpx<float> y = x[i][r];
-                    if (d.ptw)      // first pass of the two-pass split: e^{-2 pi i j2 k1 / n}
-                        y = cmul(y, ((const cpx<float> *)d.ptw)[io.vidx[v] * (j + r * NBV)]);
+                    if (d.ptw) {    // split passes: e^{-2 pi i t / N} from the two-level table
+                        const int t = io.vidx[v] * (j + r * NBV);
+                        const cpx<float> *w = (const cpx<float> *)d.ptw;
+                        y = cmul(y, cmul(w[t & ((1 << d.ptw_shift) - 1)], w[(1 << d.ptw_shift) + (t >> d.ptw_shift)]));
+                    }
                     const cpx<float> z = {y.x * io.scale, y.y * (io.scale * io.osgn)};
                     if (MOUT == ST_CONTIG) Buf<float>::stc<kStData>(io.out, (int)(v * io.ovs + j) * 8, r * NBV * 8, z);
                     else Buf<float>::stc<kStData>(io.out, (io.voff_out[v] + (j + r * NBV) * io.oes) * 8, 0, z);
@@ -170,7 +173,7 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
             const long v = vbase + t;
             voff[t] = (int)vec_off(d.in, v);
             voff[VPB + t] = (int)vec_off(d.out, v);
-            voff[2 * VPB + t] = (int)(d.out.nv0 > 0 ? v % d.out.nv0 : v);
+            voff[2 * VPB + t] = (int)(d.ptw_n > 0 ? v % d.ptw_n : v);
         }
         __syncthreads();
     }

@@ -75,8 +75,13 @@ typedef struct {
     double scale;                  /* applied in the final store */
     const void *tw;                /* stage twiddles, forward sign */
     const void *euler;             /* R2C: e^{-i pi k/n}; C2R: e^{+i pi k/n}; k = 0..n */
-    const void *ptw;               /* post-twiddle table e^{-2 pi i m / ptw_n}, or NULL */
+    const void *ptw;               /* post-twiddle table, or NULL: output element k of
+                                      vector v is multiplied by w(t) = e^{-2 pi i t / N},
+                                      t = (v mod ptw_n) k, stored in two levels:
+                                      w(t) = ptw[t mod L] ptw[L + t / L], L = 2^ptw_shift */
     long ptw_n;
+    int ptw_shift;
+    int ptw_pad_;
     vhip_view in, out;
     /* fused element work of a pipeline (NULL when unused; only the generic
      * kernel implements it):

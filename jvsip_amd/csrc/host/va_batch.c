@@ -57,25 +57,10 @@ struct vsip_amd_batch_f {
 static const void *va_batch_ptw(struct vsip_amd_batch_f *b, long n, int prec)
 {
     int i;
-    long k;
-    char *host;
     void *dev = NULL;
-    size_t esz = (size_t)(2 * prec);
     for (i = 0; i < b->nptw; i++)
         if (b->ptw[i].n == n) return b->ptw[i].dev;
-    host = (char *)malloc(esz * (size_t)n);
-    if (!host) return NULL;
-    for (k = 0; k < n; k++) {
-        double a = -2.0 * VA_PI * (double)k / (double)n;
-        ((float *)host)[2 * k] = (float)cos(a);
-        ((float *)host)[2 * k + 1] = (float)sin(a);
-    }
-    if (vhip_malloc(&dev, esz * (size_t)n) || vhip_memcpy_h2d(dev, host, esz * (size_t)n)) {
-        free(host);
-        if (dev) vhip_free(dev);
-        return NULL;
-    }
-    free(host);
+    if (va_ptw_upload(&dev, prec, n)) return NULL;
     b->ptw[b->nptw].n = n;
     b->ptw[b->nptw].dev = dev;
     b->nptw++;

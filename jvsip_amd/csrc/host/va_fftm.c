@@ -62,17 +62,59 @@ static int va_upload_table(void **dev, int prec, long count, int sign, double de
     return rc;
 }
 
+/* low-table size of the split's post-twiddle for modulus n: 2^ceil(log2(n)/2) */
+int va_ptw_shift(long n)
+{
+    int b = 0;
+    while ((1L << b) < n) b++;
+    return (b + 1) / 2;
+}
+
+/* the post-twiddle e^{-2 pi i t / n}, t < n, as two tables whose product
+ * gives it (vhip_fftm_desc.ptw): lo[r] = w(r), r < L; hi[q] = w(q L),
+ * q < ceil(n / L); angles from exact integer residues, in double */
+int va_ptw_upload(void **dev, int prec, long n)
+{
+    const int sh = va_ptw_shift(n);
+    const long L = 1L << sh, H = (n + L - 1) / L;
+    size_t esz = (size_t)(2 * prec), cnt = (size_t)(L + H);
+    char *host = (char *)malloc(esz * cnt);
+    long k;
+    int rc;
+    if (!host) return -1;
+    for (k = 0; k < L + H; k++) {
+        long t = k < L ? k : ((k - L) * L) % n;
+        double a = -2.0 * VA_PI * (double)t / (double)n;
+        if (prec == 4) {
+            ((float *)host)[2 * k] = (float)cos(a);
+            ((float *)host)[2 * k + 1] = (float)sin(a);
+        } else {
+            ((double *)host)[2 * k] = cos(a);
+            ((double *)host)[2 * k + 1] = sin(a);
+        }
+    }
+    *dev = NULL;
+    rc = vhip_malloc(dev, esz * cnt);
+    if (!rc) rc = vhip_memcpy_h2d(*dev, host, esz * cnt);
+    free(host);
+    if (rc && *dev) { vhip_free(*dev); *dev = NULL; }
+    return rc;
+}
+
 void va_fftm_release(VA_fftm *p)
 {
     if (!p) return;
     vhip_sync();
     va_sched_free(&p->pass[0]);
     va_sched_free(&p->pass[1]);
+    va_sched_free(&p->pass[2]);
     va_staged_release(p);
     if (p->dev_euler) vhip_free(p->dev_euler);
     if (p->dev_ptw) vhip_free(p->dev_ptw);
+    if (p->dev_ptw2) vhip_free(p->dev_ptw2);
     if (p->dev_work) vhip_free(p->dev_work);
-    p->dev_euler = p->dev_ptw = p->dev_work = NULL;
+    if (p->dev_work2) vhip_free(p->dev_work2);
+    p->dev_euler = p->dev_ptw = p->dev_ptw2 = p->dev_work = p->dev_work2 = NULL;
     p->markings = VA_FREED;
 }
 
@@ -99,6 +141,58 @@ static int va_plan_split(VA_fftm *p)
         break;
     }
     return best ? 0 : -1;
+}
+
+/* three passes for lengths from VSIP_AMD_SPLIT3_MIN (default 2^21): every
+ * factor small enough that the passes' column-mapped sides take 16 vectors
+ * per workgroup (128-byte pieces of each line); the factors as even as the
+ * length allows, the largest last (the pass that reads contiguous rows) */
+static int va_plan_split3(VA_fftm *p)
+{
+    const char *e = getenv("VSIP_AMD_SPLIT3_MIN");
+    long MN = (long)p->MN, lim = e ? atol(e) : (1L << 21), a, b, best = -1, ba = 0, bb = 0;
+    int lds_max = 65536;
+    if (MN < lim || lim <= 0) return -1;
+    vhip_lds_max(&lds_max);
+    for (a = 2; a * a * a <= MN; a++) {
+        long rest;
+        if (MN % a) continue;
+        rest = MN / a;
+        for (b = a; b * b <= rest; b++) {
+            long c;
+            VA_sched t;
+            if (rest % b) continue;
+            c = rest / b;                   /* a <= b <= c */
+            if (c > 1024 || (best >= 0 && c >= best)) continue;
+            memset(&t, 0, sizeof(t));
+            if (va_sched_shape(&t, p->prec, (int)a, p->mN, lds_max) ||
+                va_sched_shape(&t, p->prec, (int)b, p->mN, lds_max) ||
+                va_sched_shape(&t, p->prec, (int)c, p->mN, lds_max))
+                continue;                   /* a factor without a schedule */
+            best = c; ba = a; bb = b;
+        }
+    }
+    if (best < 0) return -1;
+    {
+        VA_sched s[3];
+        long f[3];
+        int k;
+        f[0] = ba; f[1] = bb; f[2] = MN / (ba * bb);
+        memset(s, 0, sizeof(s));
+        for (k = 0; k < 3; k++) {
+            vsip_length others = p->mN * (vsip_length)(MN / f[k]);
+            if (va_sched_plan(&s[k], p->prec, (int)f[k], others)) {
+                int q;
+                for (q = 0; q <= k; q++) va_sched_free(&s[q]);
+                return -1;
+            }
+        }
+        for (k = 0; k < 3; k++) p->pass[k] = s[k];
+        p->n1 = (vsip_length)f[0];
+        p->n2 = (vsip_length)f[1];
+        p->n3 = (vsip_length)f[2];
+    }
+    return 0;
 }
 
 /* the compiled kernels hold their vectors and offsets in static LDS */
@@ -133,16 +227,16 @@ static void va_jit_prepare(const VA_fftm *p)
             vhip_jit_prepare(s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
                              VHIP_JIT_ACROSS);
         }
-    } else if (p->npass == 2) {
-        /* two-pass split: both passes map lanes across vectors at the
-         * global side they share with the view; pass B reads scratch rows */
-        const VA_sched *a = &p->pass[0], *b = &p->pass[1];
-        if (va_jit_lds_ok(a, &a->across))
-            vhip_jit_prepare(a->n, a->nst, a->radix, a->across.vpb, a->across.threads, VHIP_JIT_ACROSS,
-                             VHIP_JIT_ACROSS);
-        if (va_jit_lds_ok(b, &b->across))
-            vhip_jit_prepare(b->n, b->nst, b->radix, b->across.vpb, b->across.threads, VHIP_JIT_CONTIG,
-                             VHIP_JIT_ACROSS);
+    } else if (p->npass >= 2) {
+        /* split: every pass maps lanes across vectors at its global sides
+         * except the last one's load, which reads scratch rows */
+        int k;
+        for (k = 0; k < p->npass; k++) {
+            const VA_sched *a = &p->pass[k];
+            if (va_jit_lds_ok(a, &a->across))
+                vhip_jit_prepare(a->n, a->nst, a->radix, a->across.vpb, a->across.threads,
+                                 k == p->npass - 1 ? VHIP_JIT_CONTIG : VHIP_JIT_ACROSS, VHIP_JIT_ACROSS);
+        }
     }
 }
 
@@ -157,11 +251,16 @@ int va_jit_ready(const VA_fftm *p)
         return vhip_jit_ready(s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
                               VHIP_JIT_ACROSS);
     }
-    if (p->npass == 2)
-        return vhip_jit_ready(s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
-                              VHIP_JIT_ACROSS) &&
-               vhip_jit_ready(p->pass[1].n, p->pass[1].nst, p->pass[1].radix, p->pass[1].across.vpb,
-                              p->pass[1].across.threads, VHIP_JIT_CONTIG, VHIP_JIT_ACROSS);
+    if (p->npass >= 2) {
+        int k;
+        for (k = 0; k < p->npass; k++) {
+            const VA_sched *a = &p->pass[k];
+            if (!vhip_jit_ready(a->n, a->nst, a->radix, a->across.vpb, a->across.threads,
+                                k == p->npass - 1 ? VHIP_JIT_CONTIG : VHIP_JIT_ACROSS, VHIP_JIT_ACROSS))
+                return 0;
+        }
+        return 1;
+    }
     return 0;
 }
 
@@ -202,9 +301,16 @@ static int va_fftm_init2(VA_fftm *p, int prec, int type, vsip_length M, vsip_len
         va_sched_free(&p->pass[0]);
         rc = -4;                      /* staged around the plain schedule */
     }
-    if (rc == -2 && !real && !va_plan_split(p)) {
+    if (rc == -2 && !real && !va_plan_split3(p)) {
+        long n23 = (long)(p->n2 * p->n3);
+        p->npass = 3;
+        if (va_ptw_upload(&p->dev_ptw, prec, (long)p->MN)) goto fail;
+        if (va_ptw_upload(&p->dev_ptw2, prec, n23)) goto fail;
+        if (vhip_malloc(&p->dev_work, (size_t)p->mN * p->MN * 2 * (size_t)prec)) goto fail;
+        if (vhip_malloc(&p->dev_work2, (size_t)p->mN * p->MN * 2 * (size_t)prec)) goto fail;
+    } else if (rc == -2 && !real && !va_plan_split(p)) {
         p->npass = 2;
-        if (va_upload_table(&p->dev_ptw, prec, (long)p->MN, -1, (double)p->MN, 2.0)) goto fail;
+        if (va_ptw_upload(&p->dev_ptw, prec, (long)p->MN)) goto fail;
         if (vhip_malloc(&p->dev_work, (size_t)p->mN * p->MN * 2 * (size_t)prec)) goto fail;
     } else if (allow_staged && (rc == -1 || rc == -2 || rc == -4)) {
         /* a prime factor without a butterfly (-1), or a length beyond one
@@ -368,6 +474,73 @@ int va_exec_views(const VA_fftm *p, const VA_mview *in, const VA_mview *out, vhi
     return kind;
 }
 
+/*
+ * The three-pass split, MN = n1 n2 n3, input element j = j1 n2 n3 + j2 n3 + j3
+ * and output element k = k1 + n1 k2 + n1 n2 k3, on scratch T1, T2 (mN x MN):
+ *   pass 1  length n1 over j1 for each (row, m = j2 n3 + j3), times
+ *           e^{-2 pi i m k1 / MN}; T1[row][k1][m]
+ *   pass 2  length n2 over j2 for each (row, k1, j3), times
+ *           e^{-2 pi i j3 k2 / (n2 n3)}; T2[row][k2][k1][j3]
+ *   pass 3  length n3 over j3 for each (row, k2, k1) -- T2's rows of n3 in
+ *           order -- into X[row][k1 + n1 k2 + n1 n2 k3]
+ * Vectors are numbered so that consecutive ones are adjacent in memory on
+ * every column-mapped side (unit vector stride): j3 fastest in passes 1 and
+ * 2, k1 fastest in pass 3.
+ */
+static int va_descs3(const VA_fftm *p, const vhip_view *gin, const vhip_view *gout, long es_i, long vs_i,
+                     long es_o, long vs_o, int inv, vhip_fftm_desc *d)
+{
+    long n1 = (long)p->n1, n2 = (long)p->n2, n3 = (long)p->n3, MN = (long)p->MN, mN = (long)p->mN;
+    long n23 = n2 * n3;
+    vhip_view gi = *gin, go = *gout, t1, t2, u1, u2;
+    memset(&t1, 0, sizeof(t1));
+    memset(&t2, 0, sizeof(t2));
+    t1.re = p->dev_work;
+    t2.re = p->dev_work2;
+    u1 = t1;
+    u2 = t2;
+    /* pass 1 */
+    gi.es = n23 * es_i; gi.vs0 = es_i; gi.nv0 = n23; gi.vs1 = vs_i; gi.contig = 0;
+    gi.span = va_span(&gi, mN * n23, n1, va_esz(&gi, p->prec, 1));
+    t1.es = n23; t1.vs0 = 1; t1.nv0 = n23; t1.vs1 = MN;
+    t1.span = va_span(&t1, mN * n23, n1, va_esz(&t1, p->prec, 1));
+    va_desc_common(&d[0], p, &p->pass[0], VHIP_C2C, mN * n23, VHIP_MAP_ACROSS, VHIP_MAP_ACROSS);
+    d[0].in = gi;
+    d[0].out = t1;
+    d[0].conj_in = inv;
+    d[0].scale = 1.0;
+    d[0].ptw = p->dev_ptw;
+    d[0].ptw_n = n23;                       /* m */
+    d[0].ptw_shift = va_ptw_shift(MN);
+    /* pass 2: vector v = j3 + n3 (k1 + n1 row) */
+    u1.es = n3; u1.vs0 = 1; u1.nv0 = n3; u1.vs1 = n23;
+    u1.span = va_span(&u1, mN * n1 * n3, n2, va_esz(&u1, p->prec, 1));
+    t2.es = n1 * n3; t2.vs0 = 1; t2.nv0 = n1 * n3; t2.vs1 = MN;
+    t2.span = va_span(&t2, mN * n1 * n3, n2, va_esz(&t2, p->prec, 1));
+    va_desc_common(&d[1], p, &p->pass[1], VHIP_C2C, mN * n1 * n3, VHIP_MAP_ACROSS, VHIP_MAP_ACROSS);
+    d[1].in = u1;
+    d[1].out = t2;
+    d[1].scale = 1.0;
+    d[1].ptw = p->dev_ptw2;
+    d[1].ptw_n = n3;                        /* j3 */
+    d[1].ptw_shift = va_ptw_shift(n23);
+    /* pass 3: vector v = k1 + n1 (k2 + n2 row), T2 row v */
+    u2.es = 1; u2.vs0 = n3; u2.nv0 = mN * n1 * n2; u2.vs1 = 0; u2.contig = 1;
+    u2.span = va_span(&u2, mN * n1 * n2, n3, va_esz(&u2, p->prec, 1));
+    go.es = n1 * n2 * es_o; go.vs0 = es_o; go.nv0 = n1 * n2; go.vs1 = vs_o; go.contig = 0;
+    go.span = va_span(&go, mN * n1 * n2, n3, va_esz(&go, p->prec, 1));
+    va_desc_common(&d[2], p, &p->pass[2], VHIP_C2C, mN * n1 * n2, VHIP_MAP_ALONG, VHIP_MAP_ACROSS);
+    d[2].in = u2;
+    d[2].out = go;
+    d[2].conj_out = inv;
+    d[2].scale = p->scale;
+    if (!gi.span || !go.span || !t1.span || !t2.span || !u1.span || !u2.span) {
+        va_error("fftm", "a view of one launch spans 2 GiB or more or has a negative stride", -1);
+        return -1;
+    }
+    return 3;
+}
+
 int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, const VA_sched *sA,
                   const VA_sched *sB, void *work, const void *ptw, vhip_fftm_desc *d)
 {
@@ -378,6 +551,10 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
     if (kind < 0) return -1;
     if (p->staged) { va_error("fftm", "a staged plan has no single launch descriptor", -1); return -1; }
     inv = (kind == VHIP_C2C && p->dir > 0);
+    if (!sA && p->npass == 3) {
+        if (kind != VHIP_C2C) { va_error("fftm", "three-pass real transform", -1); return -1; }
+        return va_descs3(p, &gi, &go, es_i, vs_i, es_o, vs_o, inv, d);
+    }
     if (!sA) {
         sA = &p->pass[0];
         sB = (p->npass == 2) ? &p->pass[1] : NULL;
@@ -439,7 +616,8 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
         d[0].conj_out = 0;
         d[0].scale = 1.0;
         d[0].ptw = ptw;
-        d[0].ptw_n = MN;
+        d[0].ptw_n = n2;                 /* row index j2 of the vector */
+        d[0].ptw_shift = va_ptw_shift(MN);
         /* pass B: vectors (row, k1), elements j2 contiguous; out X[row][k1 + n1*k2] */
         /* vector (row, k1) starts at row * MN + k1 * n2 = v * n2: one level */
         t.es = 1; t.vs0 = n2; t.nv0 = mN * n1; t.vs1 = 0; t.contig = 1;
@@ -480,7 +658,7 @@ static long va_exec_chunk(const VA_fftm *p, const VA_mview *in, const VA_mview *
         c = vs > 0 ? (lim / esz - 1 - (len - 1) * es) / vs + 1 : chunk;
         if (c < chunk) chunk = c;
     }
-    if (p->npass == 2) {                        /* the split's scratch rows */
+    if (p->npass >= 2) {                        /* the split's scratch rows */
         c = lim / ((long)p->MN * 2L * p->prec);
         if (c < chunk) chunk = c;
     }
@@ -511,7 +689,7 @@ static int va_exec_via_copy(const VA_fftm *p, const VA_mview *in, const VA_mview
 
 int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void *stream)
 {
-    vhip_fftm_desc d[2];
+    vhip_fftm_desc d[VA_MAX_DESC];
     int n, k, rc = 0;
     long chunk, v0, mN;
     if (p && p->mN == 0) return 0;

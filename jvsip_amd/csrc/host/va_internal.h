@@ -119,13 +119,15 @@ typedef struct VA_fftm {
     double scale;
     unsigned int ntimes;
     vsip_alg_hint hint;
-    int npass;               /* 1 single pass, 2 four-step */
-    vsip_length n1, n2;      /* four-step factors, MN = n1 * n2 */
-    VA_sched pass[2];        /* npass == 1: pass[0] of length MN;
-                                npass == 2: pass[0] length n1, pass[1] length n2 */
+    int npass;               /* 1 single pass, 2 or 3 split passes */
+    vsip_length n1, n2, n3;  /* split factors, MN = n1 * n2 (* n3) */
+    VA_sched pass[3];        /* npass == 1: pass[0] of length MN;
+                                npass == 2: lengths n1, n2; npass == 3: n1, n2, n3 */
     void *dev_euler;         /* real transforms: e^{-/+ i pi k/MN}, k = 0..MN */
-    void *dev_ptw;           /* four-step: e^{-2 pi i m / MN}, m = 0..MN-1 */
-    void *dev_work;          /* four-step: mN * MN complex scratch */
+    void *dev_ptw;           /* split: e^{-2 pi i m / MN}, m = 0..MN-1 */
+    void *dev_ptw2;          /* three passes: e^{-2 pi i m / (n2 n3)} */
+    void *dev_work;          /* split: mN * MN complex scratch */
+    void *dev_work2;         /* three passes: a second one */
     /* staged transforms (va_stage.c): PRE pass view -> scratch rows of length
      * sm, a forward c2c plan on them (twice, with the chirp filter product
      * between, for Bluestein), POST pass scratch -> view; `chunk` vectors per
@@ -154,6 +156,9 @@ struct vsip_fftattributes_d { VA_fftm p; };
 #define VA_BATCH_THREADS 512
 #define VA_BATCH_MAXE 16
 #define VA_BATCH_LDS 68480
+
+/* launch descriptors of one plan execution: up to three passes */
+#define VA_MAX_DESC 3
 
 /* ---- shared helpers ------------------------------------------------------ */
 int va_ensure_init(void);
@@ -192,6 +197,8 @@ int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, d
 int va_fftm_init_inner(VA_fftm *p, int prec, vsip_length M, vsip_length N);
 void va_fftm_release(VA_fftm *p);
 int va_jit_ready(const VA_fftm *p);
+int va_ptw_shift(long n);
+int va_ptw_upload(void **dev, int prec, long n);
 int va_fftm_exec(const VA_fftm *p, const VA_mview *in, const VA_mview *out, void *stream);
 int va_staged_init(VA_fftm *p, int bluestein);
 void va_staged_release(VA_fftm *p);

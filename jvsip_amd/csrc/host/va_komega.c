@@ -42,7 +42,7 @@ int vsip_amd_komega_f(const vsip_fftm_f *rcfftm, const vsip_fftm_f *ccfftm, cons
     const char *fn = "vsip_amd_komega_f";
     const VA_fftm *pr, *pc;
     const VA_mview *g;
-    vhip_fftm_desc d[2];
+    vhip_fftm_desc d[VA_MAX_DESC];
     int n, rc;
     if (va_ensure_init()) return -1;
     if (!rcfftm || !ccfftm || !ts_taper || !array_taper || !data || !freq || !gram) {

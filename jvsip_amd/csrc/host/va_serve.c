@@ -164,26 +164,11 @@ typedef struct {
 static const void *va_ptw_get(VA_ptw_cache *c, long n)
 {
     int i;
-    long k;
-    float *host;
     void *dev = NULL;
     for (i = 0; i < c->n; i++)
         if (c->tab[i].n == n) return c->tab[i].dev;
     if (c->n >= 64) return NULL;
-    host = (float *)malloc(sizeof(float) * 2 * (size_t)n);
-    if (!host) return NULL;
-    for (k = 0; k < n; k++) {
-        double a = -6.283185307179586476925286766559 * (double)k / (double)n;
-        host[2 * k] = (float)cos(a);
-        host[2 * k + 1] = (float)sin(a);
-    }
-    if (vhip_malloc(&dev, sizeof(float) * 2 * (size_t)n) ||
-        vhip_memcpy_h2d(dev, host, sizeof(float) * 2 * (size_t)n)) {
-        free(host);
-        if (dev) vhip_free(dev);
-        return NULL;
-    }
-    free(host);
+    if (va_ptw_upload(&dev, 4, n)) return NULL;
     c->tab[c->n].n = n;
     c->tab[c->n].dev = dev;
     c->n++;
@@ -375,7 +360,7 @@ static int va_direct_launch(const VA_plan_cache *pc, const vsip_amd_request *r, 
     VA_fftm p;
     VA_block bi, bo;
     VA_mview vi, vo;
-    vhip_fftm_desc d[2];
+    vhip_fftm_desc d[VA_MAX_DESC];
     int nd, k, rc = 0;
     if (!base) return -1;
     p = *base;

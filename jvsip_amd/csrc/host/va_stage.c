@@ -226,7 +226,7 @@ static int va_inner_fft(const VA_fftm *p, long cnt, void *stream)
 {
     VA_block blk;
     VA_mview sv;
-    vhip_fftm_desc d[2];
+    vhip_fftm_desc d[VA_MAX_DESC];
     int nd, k, rc = 0;
     memset(&blk, 0, sizeof(blk));
     blk.prec = p->prec;
