@@ -35,13 +35,13 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kBufWord3);
 }
 // Cache policy of the data's HBM loads and stores (the buffer
-// instructions' aux field).  NT = true marks both non-temporal (loads 2 =
-// nt, stores 18 = nt | sc1): the c2c kernel reads every element once and
-// writes it once, and streaming them raised the 4096 x 4096 in-place
-// transform from 4.80 to 5.24 TB/s on MI355X.  The real kernels and the
-// batched items keep the default policy: their rows share cache lines at
-// row ends (n + 1 outputs) or read elements twice (C2R), and they measured
-// slower streamed.
+// instructions' aux field).  NT = true marks them non-temporal (loads 2 =
+// nt, stores 18 = nt | sc1).  Measured on MI355X: the c2c kernel streams
+// both (4096 x 4096 in place: 4.80 -> 5.24 TB/s); C2R streams its stores
+// (8192 x 2048: 5.04 -> 5.53 TB/s) but not its loads, which read each
+// element twice; R2C keeps the default for both (streamed loads 4.57 ->
+// 3.95 TB/s; its n + 1 outputs per row share lines between rows); the
+// batched items keep the default.
 __device__ __forceinline__ cpx<float> bload(rsrc_t r, int voff, int soff) {
     return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
@@ -283,10 +283,10 @@ __device__ __forceinline__ void pow2_real_body(const vhip_fftm_desc &d, cpx<floa
     const rsrc_t o = make_rsrc(out, group_bytes<LOG2N, VPB>(d.nvec, g, ovs));
     constexpr bool TOL = (KIND == VHIP_R2C);
     const float osgn = KIND == VHIP_C2R ? -1.f : 1.f;
-    p2_stage<LOG2N, VPB, 0, false, TOL>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
-    if constexpr (P::NST > 1) p2_stage<LOG2N, VPB, (P::NST > 1 ? 1 : 0), false, TOL>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
-    if constexpr (P::NST > 2) p2_stage<LOG2N, VPB, (P::NST > 2 ? 2 : 0), false, TOL>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
-    if constexpr (P::NST > 3) p2_stage<LOG2N, VPB, (P::NST > 3 ? 3 : 0), false, TOL>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    p2_stage<LOG2N, VPB, 0, false, TOL, KIND == VHIP_C2R>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 1) p2_stage<LOG2N, VPB, (P::NST > 1 ? 1 : 0), false, TOL, KIND == VHIP_C2R>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 2) p2_stage<LOG2N, VPB, (P::NST > 2 ? 2 : 0), false, TOL, KIND == VHIP_C2R>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 3) p2_stage<LOG2N, VPB, (P::NST > 3 ? 3 : 0), false, TOL, KIND == VHIP_C2R>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
     if (KIND == VHIP_R2C) {
         // the n + 1 outputs of each row from Y in LDS (natural order)
         const float half = 0.5f * scale;
