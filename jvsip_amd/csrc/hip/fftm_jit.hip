@@ -1,7 +1,7 @@
 // fftm_jit.hip -- kernels compiled at plan creation for one schedule.
 //
-// A plan whose schedule the generic kernel would run (mixed radix, contiguous
-// rows, single pass) gets the same Stockham schedule compiled with every
+// A plan whose passes the generic kernel would run (complex FP32 or FP64;
+// rows, columns or the passes of a split) gets the same Stockham schedule compiled with every
 // constant fixed (fftm_static.hpp): hipRTC compiles the template instance
 // named by the schedule for the device's architecture, the code object is
 // loaded as a module, and the function is kept in a process-wide cache keyed
@@ -46,13 +46,14 @@ bool jit_on() {
 
 // the template instance of a schedule; empty when the schedule has no
 // compiled form (more than 8 stages, or not a single pass)
-std::string instance(int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
-    if (nst < 1 || nst > 8 || min < 0 || min > 1 || mout < 0 || mout > 1) return "";
+std::string instance(int prec, int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
+    if (nst < 1 || nst > 8 || min < 0 || min > 1 || mout < 0 || mout > 1 || (prec != 4 && prec != 8)) return "";
     int r[8];
     for (int s = 0; s < 8; ++s) r[s] = s < nst ? radix[s] : 1;
     char buf[256];
-    snprintf(buf, sizeof buf, "vhipk::st::fftm_static_kernel<%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d>", n,
-             vpb, threads, min, mout, nst, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]);
+    snprintf(buf, sizeof buf, "vhipk::st::fftm_static_kernel<%s,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d>",
+             prec == 4 ? "float" : "double", n, vpb, threads, min, mout, nst, r[0], r[1], r[2], r[3], r[4], r[5],
+             r[6], r[7]);
     return buf;
 }
 
@@ -115,9 +116,9 @@ int compile(const std::string &name, const std::string &arch, std::vector<char> 
 
 // Compile (no device needed) the kernel of one schedule for `arch`
 // ("gfx950"); 0 on success.  Used to check the embedded sources.
-extern "C" int vhip_jit_compile_check(int n, int nst, const int *radix, int vpb, int threads, int min, int mout,
-                                      const char *arch, size_t *code_bytes) {
-    std::string name = instance(n, nst, radix, vpb, threads, min, mout);
+extern "C" int vhip_jit_compile_check(int prec, int n, int nst, const int *radix, int vpb, int threads, int min,
+                                      int mout, const char *arch, size_t *code_bytes) {
+    std::string name = instance(prec, n, nst, radix, vpb, threads, min, mout);
     if (name.empty()) return -1;
     std::vector<char> code;
     std::string lowered;
@@ -130,9 +131,10 @@ extern "C" const char *vhip_jit_last_error(void) { return g_err; }
 
 // Make the kernel of a schedule available (compile and load once); 0 when
 // it is, nonzero when the schedule runs on the generic kernel.
-extern "C" int vhip_jit_prepare(int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
+extern "C" int vhip_jit_prepare(int prec, int n, int nst, const int *radix, int vpb, int threads, int min,
+                                int mout) {
     if (!jit_on()) return -1;
-    std::string name = instance(n, nst, radix, vpb, threads, min, mout);
+    std::string name = instance(prec, n, nst, radix, vpb, threads, min, mout);
     if (name.empty()) return -1;
     std::lock_guard<std::mutex> lock(g_mu);
     auto it = g_cache.find(name);
@@ -150,8 +152,8 @@ extern "C" int vhip_jit_prepare(int n, int nst, const int *radix, int vpb, int t
 }
 
 // Is the kernel of this schedule compiled and loaded?
-extern "C" int vhip_jit_ready(int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
-    std::string name = instance(n, nst, radix, vpb, threads, min, mout);
+extern "C" int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
+    std::string name = instance(prec, n, nst, radix, vpb, threads, min, mout);
     std::lock_guard<std::mutex> lock(g_mu);
     auto it = g_cache.find(name);
     return (it != g_cache.end() && it->second.fn) ? 1 : 0;
@@ -160,20 +162,21 @@ extern "C" int vhip_jit_ready(int n, int nst, const int *radix, int vpb, int thr
 // Launch d on its compiled kernel when one is loaded and d has the layout
 // the kernel serves; -100 otherwise (the caller uses the generic kernel).
 extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
-    if (d->prec != 4 || d->kind != VHIP_C2C || d->tap || d->acc) return -100;
+    if ((d->prec != 4 && d->prec != 8) || d->kind != VHIP_C2C || d->tap || d->acc) return -100;
     const int min = side_mode(d->in, d->map_in, d->nvec), mout = side_mode(d->out, d->map_out, d->nvec);
     if (min < 0 || mout < 0) return -100;
+    const long es = 2L * d->prec;
     if (min == 0) {
         if (d->nvec > 1 && d->in.vs0 < d->n) return -100;                 // rows must not overlap
-        if (((long)(d->vpb - 1) * d->in.vs0 + d->n) * 8 >= (1L << 31)) return -100;
+        if (((long)(d->vpb - 1) * d->in.vs0 + d->n) * es >= (1L << 31)) return -100;
     }
     if (mout == 0) {
         if (d->nvec > 1 && d->out.vs0 < d->n) return -100;
-        if (((long)(d->vpb - 1) * d->out.vs0 + d->n) * 8 >= (1L << 31)) return -100;
+        if (((long)(d->vpb - 1) * d->out.vs0 + d->n) * es >= (1L << 31)) return -100;
     }
     hipFunction_t fn = nullptr;
     {
-        std::string name = instance(d->n, d->nst, d->radix, d->vpb, d->threads, min, mout);
+        std::string name = instance(d->prec, d->n, d->nst, d->radix, d->vpb, d->threads, min, mout);
         std::lock_guard<std::mutex> lock(g_mu);
         auto it = g_cache.find(name);
         if (it == g_cache.end() || !it->second.fn) return -100;

@@ -1,9 +1,10 @@
 // fftm_static.hpp -- the Stockham transform of fftm_core.hpp with its whole
-// schedule fixed at compile time: length N, radices R0..R7 (NST of them),
-// vectors per workgroup VPB, lanes NT and the layout of the global load and
-// store (MIN, MOUT).  The library compiles one such kernel per plan schedule
-// at plan creation (hipRTC, in-process cache) for interleaved complex FP32
-// data; every butterfly index, twiddle index and LDS offset then folds to
+// schedule fixed at compile time: element type T (float or double), length
+// N, radices R0..R7 (NST of them), vectors per workgroup VPB, lanes NT and
+// the layout of the global load and store (MIN, MOUT).  The library compiles
+// one such kernel per plan schedule at plan creation (hipRTC, in-process
+// cache) for interleaved complex data; every butterfly index, twiddle index
+// and LDS offset then folds to
 // constants, shifts and multiply-highs instead of the runtime divisions and
 // address arithmetic of the schedule-generic kernel.
 //
@@ -39,10 +40,14 @@ struct Sched {
     static constexpr int ns(int s) { return s == 0 ? 1 : ns(s - 1) * rad(s - 1); }
 };
 
-// padded LDS index (one spare slot per 32 complex FP32 values), as the host
-// planner sizes it
-__device__ __forceinline__ constexpr int lpad(int i) { return i + (i >> 5); }
+// padded LDS index (one spare slot per 32 complex FP32 or 16 FP64 values),
+// as the host planner sizes it
+template <typename T>
+__device__ __forceinline__ constexpr int lpad(int i) { return i + (i >> (sizeof(T) == 4 ? 5 : 4)); }
+template <typename T>
+constexpr int lstride(int n) { return (lpad<T>(n) % 2 == 0) ? lpad<T>(n) + 1 : lpad<T>(n); }
 
+template <typename T>
 struct Io {
     rsrc_t in, out, tw;
     long ivs, ovs;         // ST_CONTIG vector strides
@@ -51,11 +56,12 @@ struct Io {
     const int *voff_in;    // ST_ACROSS: element offset of each vector of the group
     const int *voff_out;
     const int *vidx;       // post-twiddled launches: index of each vector within its row
-    float isgn, osgn, scale;
+    T isgn, osgn, scale;
 };
 
-template <int N, int VPB, int NT, int MIN, int MOUT, int S, class SC>
-__device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d, const Io &io) {
+template <typename T, int N, int VPB, int NT, int MIN, int MOUT, int S, class SC>
+__device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, const Io<T> &io) {
+    constexpr int ES = (int)sizeof(cpx<T>);
     constexpr int R = SC::rad(S);
     constexpr int NS = SC::ns(S);
     constexpr int NBV = N / R;
@@ -65,10 +71,10 @@ __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d,
     // lane mapping of this stage: the global layout's for the first and last
     // stage (the load's when one stage does both), along the vector otherwise
     constexpr int MODE = FIRST ? MIN : (LAST ? MOUT : ST_CONTIG);
-    constexpr int LS = (lpad(N) % 2 == 0) ? lpad(N) + 1 : lpad(N);
+    constexpr int LS = lstride<T>(N);
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
-    cpx<float> x[B][R];
+    cpx<T> x[B][R];
     int vv[B], jj[B];
     bool ok[B];
 #pragma unroll
@@ -84,16 +90,16 @@ __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d,
             if (FIRST) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    cpx<float> y;
-                    if (MIN == ST_CONTIG) y = Buf<float>::ldc<kLdData>(io.in, (int)(v * io.ivs + j) * 8, r * NBV * 8);
-                    else y = Buf<float>::ldc<kLdData>(io.in, (io.voff_in[v] + (j + r * NBV) * io.ies) * 8, 0);
+                    cpx<T> y;
+                    if (MIN == ST_CONTIG) y = Buf<T>::template ldc<kLdData>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
+                    else y = Buf<T>::template ldc<kLdData>(io.in, (io.voff_in[v] + (j + r * NBV) * io.ies) * ES, 0);
                     y.y *= io.isgn;
                     x[i][r] = y;
                 }
             } else {
-                const cpx<float> *src = lds + v * LS;
+                const cpx<T> *src = lds + v * LS;
 #pragma unroll
-                for (int r = 0; r < R; ++r) x[i][r] = src[lpad(j + r * NBV)];
+                for (int r = 0; r < R; ++r) x[i][r] = src[lpad<T>(j + r * NBV)];
             }
         }
     }
@@ -107,37 +113,37 @@ __device__ __forceinline__ void sstage(cpx<float> *lds, const vhip_fftm_desc &d,
                 const int toff = d.tw_off[S];
 #pragma unroll
                 for (int r = 1; r < R; ++r)
-                    x[i][r] = cmul(x[i][r], Buf<float>::ldc(io.tw, (toff + k + (r - 1) * NS) * 8, 0));
+                    x[i][r] = cmul(x[i][r], Buf<T>::template ldc<0>(io.tw, (toff + k + (r - 1) * NS) * ES, 0));
             }
-            Dft<float, R>::run(x[i]);
+            Dft<T, R>::run(x[i]);
             if (LAST) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    cpx<float> y = x[i][r];
+                    cpx<T> y = x[i][r];
                     if (d.ptw) {    // split passes: e^{-2 pi i t / N} from the two-level table
                         const int t = io.vidx[v] * (j + r * NBV);
-                        const cpx<float> *w = (const cpx<float> *)d.ptw;
+                        const cpx<T> *w = (const cpx<T> *)d.ptw;
                         y = cmul(y, cmul(w[t & ((1 << d.ptw_shift) - 1)], w[(1 << d.ptw_shift) + (t >> d.ptw_shift)]));
                     }
-                    const cpx<float> z = {y.x * io.scale, y.y * (io.scale * io.osgn)};
-                    if (MOUT == ST_CONTIG) Buf<float>::stc<kStData>(io.out, (int)(v * io.ovs + j) * 8, r * NBV * 8, z);
-                    else Buf<float>::stc<kStData>(io.out, (io.voff_out[v] + (j + r * NBV) * io.oes) * 8, 0, z);
+                    const cpx<T> z = {y.x * io.scale, y.y * (io.scale * io.osgn)};
+                    if (MOUT == ST_CONTIG) Buf<T>::template stc<kStData>(io.out, (int)(v * io.ovs + j) * ES, r * NBV * ES, z);
+                    else Buf<T>::template stc<kStData>(io.out, (io.voff_out[v] + (j + r * NBV) * io.oes) * ES, 0, z);
                 }
             } else {
                 const int base = (j - k) * R + k;
-                cpx<float> *dst = lds + v * LS;
+                cpx<T> *dst = lds + v * LS;
 #pragma unroll
-                for (int r = 0; r < R; ++r) dst[lpad(base + r * NS)] = x[i][r];
+                for (int r = 0; r < R; ++r) dst[lpad<T>(base + r * NS)] = x[i][r];
             }
         }
     }
     if (!LAST) __syncthreads();
 }
 
-template <int N, int VPB, int NT, int MIN, int MOUT, int S, class SC>
-__device__ __forceinline__ void sstages(cpx<float> *lds, const vhip_fftm_desc &d, const Io &io) {
-    sstage<N, VPB, NT, MIN, MOUT, S, SC>(lds, d, io);
-    if constexpr (S + 1 < SC::NST) sstages<N, VPB, NT, MIN, MOUT, S + 1, SC>(lds, d, io);
+template <typename T, int N, int VPB, int NT, int MIN, int MOUT, int S, class SC>
+__device__ __forceinline__ void sstages(cpx<T> *lds, const vhip_fftm_desc &d, const Io<T> &io) {
+    sstage<T, N, VPB, NT, MIN, MOUT, S, SC>(lds, d, io);
+    if constexpr (S + 1 < SC::NST) sstages<T, N, VPB, NT, MIN, MOUT, S + 1, SC>(lds, d, io);
 }
 
 // One workgroup per group of VPB vectors.  ST_CONTIG sides address the
@@ -146,20 +152,21 @@ __device__ __forceinline__ void sstages(cpx<float> *lds, const vhip_fftm_desc &d
 // address the whole view (vhip_view.span) and skip vectors past its end.
 // Column-mapped launches take the XCD-aware group numbering of the generic
 // kernel (xcd_group).
-template <int N, int VPB, int NT, int MIN, int MOUT, int NST, int R0, int R1, int R2, int R3, int R4, int R5,
-          int R6, int R7>
+template <typename T, int N, int VPB, int NT, int MIN, int MOUT, int NST, int R0, int R1, int R2, int R3, int R4,
+          int R5, int R6, int R7>
 __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d) {
     using SC = Sched<NST, R0, R1, R2, R3, R4, R5, R6, R7>;
-    constexpr int LS = (lpad(N) % 2 == 0) ? lpad(N) + 1 : lpad(N);
+    constexpr int LS = lstride<T>(N);
+    constexpr int ES = (int)sizeof(cpx<T>);
     // static LDS: the size is known here, and a static allocation may use
     // all of the workgroup's LDS without a launch attribute
-    __shared__ __attribute__((aligned(16))) cpx<float> slds[VPB * LS];
+    __shared__ __attribute__((aligned(16))) cpx<T> slds[VPB * LS];
     __shared__ int voff[3 * VPB];
     constexpr bool ACROSS = (MIN == ST_ACROSS || MOUT == ST_ACROSS);
     const long g = ACROSS ? (long)xcd_group(blockIdx.x, gridDim.x) : (long)blockIdx.x;
     const long vbase = g * VPB;
     long nvl = d.nvec - vbase;
-    Io io;
+    Io<T> io;
     io.nv = nvl < VPB ? (int)nvl : VPB;
     io.ivs = d.in.vs0;
     io.ovs = d.out.vs0;
@@ -178,18 +185,18 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
         __syncthreads();
     }
     if (MIN == ST_CONTIG)
-        io.in = make_rsrc((const cpx<float> *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + N) * 8);
+        io.in = make_rsrc((const cpx<T> *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + N) * ES);
     else
         io.in = make_rsrc(d.in.re, d.in.span);
     if (MOUT == ST_CONTIG)
-        io.out = make_rsrc((cpx<float> *)d.out.re + vbase * io.ovs, ((io.nv - 1) * io.ovs + N) * 8);
+        io.out = make_rsrc((cpx<T> *)d.out.re + vbase * io.ovs, ((io.nv - 1) * io.ovs + N) * ES);
     else
         io.out = make_rsrc(d.out.re, d.out.span);
-    io.tw = make_rsrc(d.tw, (long)N * 8);
-    io.isgn = d.conj_in ? -1.f : 1.f;
-    io.osgn = d.conj_out ? -1.f : 1.f;
-    io.scale = (float)d.scale;
-    sstages<N, VPB, NT, MIN, MOUT, 0, SC>(slds, d, io);
+    io.tw = make_rsrc(d.tw, (long)N * ES);
+    io.isgn = d.conj_in ? (T)-1 : (T)1;
+    io.osgn = d.conj_out ? (T)-1 : (T)1;
+    io.scale = (T)d.scale;
+    sstages<T, N, VPB, NT, MIN, MOUT, 0, SC>(slds, d, io);
 }
 
 }  // namespace st

@@ -231,16 +231,17 @@ int vhip_rng_fill(const vhip_rng_desc *d, void *stream);
  * compiles and loads the kernel of (n, radices, vpb, threads, load layout,
  * store layout) once per process (0 when available); try_launch runs a
  * descriptor on it when it is loaded and the descriptor's layouts match,
- * else returns -100.  Layouts: VHIP_JIT_CONTIG (interleaved FP32 rows of unit
+ * else returns -100.  Layouts: VHIP_JIT_CONTIG (interleaved rows of unit
  * element stride, single-level vector stride), VHIP_JIT_ACROSS (interleaved,
  * lanes across vectors, any strides within vhip_view.span). */
 #define VHIP_JIT_CONTIG 0
 #define VHIP_JIT_ACROSS 1
-int vhip_jit_prepare(int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);
-int vhip_jit_ready(int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);
+int vhip_jit_prepare(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in,
+                     int layout_out);
+int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);
 int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream);
-int vhip_jit_compile_check(int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out,
-                           const char *arch, size_t *code_bytes);
+int vhip_jit_compile_check(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in,
+                           int layout_out, const char *arch, size_t *code_bytes);
 const char *vhip_jit_last_error(void);
 
 /* kernels */

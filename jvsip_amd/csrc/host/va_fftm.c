@@ -196,47 +196,48 @@ static int va_plan_split3(VA_fftm *p)
 }
 
 /* the compiled kernels hold their vectors and offsets in static LDS */
-static int va_jit_lds_ok(const VA_sched *s, const VA_launch *L)
+static int va_jit_lds_ok(const VA_sched *s, const VA_launch *L, int prec)
 {
     int lds_max = 65536;
     vhip_lds_max(&lds_max);
-    return (long)L->vpb * s->lds_stride * 8 + (long)L->vpb * 12 <= lds_max;
+    return (long)L->vpb * s->lds_stride * 2 * prec + (long)L->vpb * 12 <= lds_max;
 }
 
-/* FP32 complex plans whose passes the schedule-generic kernel would run
- * (every pass except the power-of-two kernel's contiguous rows) and that move
- * at least VSIP_AMD_JIT_MIN_ELEMS elements (default 2^18) get their
- * schedules compiled now, for the layouts the plan's views normally have:
- * contiguous rows (ROW), lanes across columns (COL), and the two passes of
- * the split.  Execution finds the compiled kernel when the layout matches. */
+/* Complex plans (FP32, FP64) whose passes the schedule-generic kernel would
+ * run (every pass except the FP32 power-of-two kernel's contiguous rows)
+ * and that move at least VSIP_AMD_JIT_MIN_ELEMS elements (default 2^18) get
+ * their schedules compiled now, for the layouts the plan's views normally
+ * have: contiguous rows (ROW), lanes across columns (COL), and the passes
+ * of the splits.  Execution finds the compiled kernel when the layout
+ * matches. */
 static void va_jit_prepare(const VA_fftm *p)
 {
     const char *e = getenv("VSIP_AMD_JIT_MIN_ELEMS");
     long min_elems = e ? atol(e) : (1L << 18);
     long n = (long)p->MN;
-    if (p->prec != 4 || (p->type != VA_CCFFTIP && p->type != VA_CCFFTOP) || p->staged) return;
+    int k;
+    if ((p->type != VA_CCFFTIP && p->type != VA_CCFFTOP) || p->staged) return;
     if ((long)p->mN * n < min_elems) return;
     if (p->npass == 1) {
         const VA_sched *s = &p->pass[0];
         if (p->major == VSIP_ROW) {
-            if ((n & (n - 1)) == 0 && n >= 256 && n <= 16384) return;    /* the power-of-two kernel */
-            if (va_jit_lds_ok(s, &s->along))
-                vhip_jit_prepare(s->n, s->nst, s->radix, s->along.vpb, s->along.threads, VHIP_JIT_CONTIG,
+            if (p->prec == 4 && (n & (n - 1)) == 0 && n >= 256 && n <= 16384) return;   /* the power-of-two kernel */
+            if (va_jit_lds_ok(s, &s->along, p->prec))
+                vhip_jit_prepare(p->prec, s->n, s->nst, s->radix, s->along.vpb, s->along.threads, VHIP_JIT_CONTIG,
                                  VHIP_JIT_CONTIG);
-        } else if (va_jit_lds_ok(s, &s->across)) {
-            vhip_jit_prepare(s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
+        } else if (va_jit_lds_ok(s, &s->across, p->prec)) {
+            vhip_jit_prepare(p->prec, s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
                              VHIP_JIT_ACROSS);
         }
-    } else if (p->npass >= 2) {
-        /* split: every pass maps lanes across vectors at its global sides
-         * except the last one's load, which reads scratch rows */
-        int k;
-        for (k = 0; k < p->npass; k++) {
-            const VA_sched *a = &p->pass[k];
-            if (va_jit_lds_ok(a, &a->across))
-                vhip_jit_prepare(a->n, a->nst, a->radix, a->across.vpb, a->across.threads,
-                                 k == p->npass - 1 ? VHIP_JIT_CONTIG : VHIP_JIT_ACROSS, VHIP_JIT_ACROSS);
-        }
+        return;
+    }
+    /* split: every pass maps lanes across vectors at its global sides except
+     * the last one's load, which reads scratch rows */
+    for (k = 0; k < p->npass; k++) {
+        const VA_sched *a = &p->pass[k];
+        if (va_jit_lds_ok(a, &a->across, p->prec))
+            vhip_jit_prepare(p->prec, a->n, a->nst, a->radix, a->across.vpb, a->across.threads,
+                             k == p->npass - 1 ? VHIP_JIT_CONTIG : VHIP_JIT_ACROSS, VHIP_JIT_ACROSS);
     }
 }
 
@@ -244,24 +245,22 @@ static void va_jit_prepare(const VA_fftm *p)
 int va_jit_ready(const VA_fftm *p)
 {
     const VA_sched *s = &p->pass[0];
+    int k;
     if (p->npass == 1) {
         if (p->major == VSIP_ROW)
-            return vhip_jit_ready(s->n, s->nst, s->radix, s->along.vpb, s->along.threads, VHIP_JIT_CONTIG,
+            return vhip_jit_ready(p->prec, s->n, s->nst, s->radix, s->along.vpb, s->along.threads, VHIP_JIT_CONTIG,
                                   VHIP_JIT_CONTIG);
-        return vhip_jit_ready(s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
+        return vhip_jit_ready(p->prec, s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
                               VHIP_JIT_ACROSS);
     }
-    if (p->npass >= 2) {
-        int k;
-        for (k = 0; k < p->npass; k++) {
-            const VA_sched *a = &p->pass[k];
-            if (!vhip_jit_ready(a->n, a->nst, a->radix, a->across.vpb, a->across.threads,
-                                k == p->npass - 1 ? VHIP_JIT_CONTIG : VHIP_JIT_ACROSS, VHIP_JIT_ACROSS))
-                return 0;
-        }
-        return 1;
+    if (p->npass < 2) return 0;
+    for (k = 0; k < p->npass; k++) {
+        const VA_sched *a = &p->pass[k];
+        if (!vhip_jit_ready(p->prec, a->n, a->nst, a->radix, a->across.vpb, a->across.threads,
+                            k == p->npass - 1 ? VHIP_JIT_CONTIG : VHIP_JIT_ACROSS, VHIP_JIT_ACROSS))
+            return 0;
     }
-    return 0;
+    return 1;
 }
 
 /* allow_staged == 0 gives up where a staged transform would be needed (the

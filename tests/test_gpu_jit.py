@@ -140,3 +140,29 @@ def test_compiled_schedule_long_1d(amd, monkeypatch):
     x = datagen.complex_data(100, (n,))
     y = fftm.ccfft(amd, x.copy(), fftm.View1(0, 1, n))
     _check("vs numpy fp64", y, np.fft.fft(x.astype(np.complex128)), 1e-5)
+
+
+@pytest.mark.parametrize("m,n,major", [(2048, 2048, VSIP_ROW), (300, 1000, VSIP_ROW), (1000, 300, VSIP_COL),
+                                       (4, 3 * 5 * 7 * 11, VSIP_ROW)])
+def test_compiled_schedule_fp64(amd, monkeypatch, m, n, major):
+    """FP64 plans compile the same schedule with double elements; 1e-12 to
+    the FP64 oracle, 1e-13 to the generic kernel."""
+    monkeypatch.setenv("VSIP_AMD_JIT_MIN_ELEMS", "0")
+    x = datagen.complex_data(101, (m, n), dtype=np.complex128)
+    v = fftm.dense_view(m, n)
+    p = amd.ccfftmip_create_d(m, n, 1.0, VSIP_FFT_FWD, major, 0, 1)
+    assert p, amd.last_error()
+    d = json.loads(amd.describe(p, "d"))
+    amd.fftm_destroy_d(p)
+    print("  passes", [(q["n"], q["radix"]) for q in d["passes"]], "compiled", d.get("compiled_schedule"),
+          flush=True)
+    assert d.get("compiled_schedule") is True
+    for direction, scale in ((VSIP_FFT_FWD, 1.0), (VSIP_FFT_INV, 1.0 / (n if major == VSIP_ROW else m))):
+        _case("ccfftmip_d %s %dx%d dir %+d (compiled schedule)" % ("ROW" if major == VSIP_ROW else "COL", m, n,
+                                                                  direction))
+        y = fftm.ccfftm(amd, x.ravel(), v, major=major, direction=direction, scale=scale)
+        _check("vs oracle", y, po.ccfftm(x.ravel(), v, major=major, direction=direction, scale=scale), 1e-12)
+    monkeypatch.setenv("VSIP_AMD_JIT", "0")
+    g = fftm.ccfftm(amd, x.ravel(), v, major=major)
+    monkeypatch.delenv("VSIP_AMD_JIT")
+    _check("vs generic kernel", fftm.ccfftm(amd, x.ravel(), v, major=major), g, 1e-13)

@@ -23,30 +23,33 @@ def lib():
 def so():
     from jvsip_amd.vsip import LIB_PATH
     d = ctypes.CDLL(LIB_PATH)
-    d.vhip_jit_compile_check.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
-                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+    d.vhip_jit_compile_check.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                          ctypes.POINTER(ctypes.c_size_t)]
     d.vhip_jit_compile_check.restype = ctypes.c_int
     d.vhip_jit_last_error.restype = ctypes.c_char_p
     return d
 
 
-@pytest.mark.parametrize("n,nvec", [(1000, 4096), (2187, 1024), (1001, 512), (1920, 1000), (3000, 512),
-                                    (6561, 64), (100, 10000), (7 * 11 * 13 * 3, 300)])
-def test_planner_schedules_compile(lib, so, n, nvec):
-    s = lib.schedule(4, n, nvec, 163840)
+@pytest.mark.parametrize("prec,n,nvec", [(4, 1000, 4096), (4, 2187, 1024), (4, 1001, 512), (4, 1920, 1000),
+                                         (4, 3000, 512), (4, 6561, 64), (4, 100, 10000), (4, 7 * 11 * 13 * 3, 300),
+                                         (8, 2048, 2048), (8, 1000, 512), (8, 4096, 64)])
+def test_planner_schedules_compile(lib, so, prec, n, nvec):
+    s = lib.schedule(prec, n, nvec, 163840)
     assert s["rc"] == 0
     rad = (ctypes.c_int * len(s["radix"]))(*s["radix"])
     for shape, lin, lout in (("along", 0, 0), ("across", 1, 1), ("across", 0, 1)):
         vpb, thr, _ = s[shape]
         size = ctypes.c_size_t()
-        rc = so.vhip_jit_compile_check(n, len(s["radix"]), rad, vpb, thr, lin, lout, b"gfx950", ctypes.byref(size))
+        rc = so.vhip_jit_compile_check(prec, n, len(s["radix"]), rad, vpb, thr, lin, lout, b"gfx950",
+                                       ctypes.byref(size))
         assert rc == 0, so.vhip_jit_last_error()
         assert size.value > 1000
 
 
 def test_refuses_schedules_without_a_compiled_form(so):
     rad = (ctypes.c_int * 9)(*([2] * 9))
-    assert so.vhip_jit_compile_check(512, 9, rad, 1, 64, 0, 0, b"gfx950", None) != 0
+    assert so.vhip_jit_compile_check(4, 512, 9, rad, 1, 64, 0, 0, b"gfx950", None) != 0
     rad = (ctypes.c_int * 2)(4, 4)
-    assert so.vhip_jit_compile_check(16, 2, rad, 1, 64, 2, 0, b"gfx950", None) != 0
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 2, 0, b"gfx950", None) != 0
+    assert so.vhip_jit_compile_check(2, 16, 2, rad, 1, 64, 0, 0, b"gfx950", None) != 0
