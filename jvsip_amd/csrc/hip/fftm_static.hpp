@@ -91,7 +91,10 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     cpx<T> y;
-                    if (MIN == ST_CONTIG) y = Buf<T>::template ldc<kLdData>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
+                    if (MIN == ST_CONTIG && (d.stream & 1))
+                        y = Buf<T>::template ldc<2>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
+                    else if (MIN == ST_CONTIG)
+                        y = Buf<T>::template ldc<kLdData>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
                     else y = Buf<T>::template ldc<kLdData>(io.in, (io.voff_in[v] + (j + r * NBV) * io.ies) * ES, 0);
                     y.y *= io.isgn;
                     x[i][r] = y;
@@ -126,7 +129,10 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
                         y = cmul(y, cmul(w[t & ((1 << d.ptw_shift) - 1)], w[(1 << d.ptw_shift) + (t >> d.ptw_shift)]));
                     }
                     const cpx<T> z = {y.x * io.scale, y.y * (io.scale * io.osgn)};
-                    if (MOUT == ST_CONTIG) Buf<T>::template stc<kStData>(io.out, (int)(v * io.ovs + j) * ES, r * NBV * ES, z);
+                    if (MOUT == ST_CONTIG && (d.stream & 2))
+                        Buf<T>::template stc<18>(io.out, (int)(v * io.ovs + j) * ES, r * NBV * ES, z);
+                    else if (MOUT == ST_CONTIG)
+                        Buf<T>::template stc<kStData>(io.out, (int)(v * io.ovs + j) * ES, r * NBV * ES, z);
                     else Buf<T>::template stc<kStData>(io.out, (io.voff_out[v] + (j + r * NBV) * io.oes) * ES, 0, z);
                 }
             } else {

@@ -81,7 +81,9 @@ typedef struct {
                                       w(t) = ptw[t mod L] ptw[L + t / L], L = 2^ptw_shift */
     long ptw_n;
     int ptw_shift;
-    int ptw_pad_;
+    int stream;                    /* bit 0: load, bit 1: store the data non-temporally
+                                      (set by the host for dense, line-aligned rows;
+                                      honoured by the compiled-schedule kernels) */
     vhip_view in, out;
     /* fused element work of a pipeline (NULL when unused; only the generic
      * kernel implements it):
