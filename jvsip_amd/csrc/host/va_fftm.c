@@ -221,7 +221,8 @@ static void va_jit_prepare(const VA_fftm *p)
     if (p->npass == 1) {
         const VA_sched *s = &p->pass[0];
         if (p->major == VSIP_ROW) {
-            if (p->prec == 4 && (n & (n - 1)) == 0 && n >= 256 && n <= 16384) return;   /* the power-of-two kernel */
+            if (p->prec == 4 && (n & (n - 1)) == 0 && n >= 256 && n <= 16384 && !getenv("VSIP_AMD_NO_POW2"))
+                return;                                                   /* the power-of-two kernel */
             if (va_jit_lds_ok(s, &s->along, p->prec))
                 vhip_jit_prepare(p->prec, s->n, s->nst, s->radix, s->along.vpb, s->along.threads, VHIP_JIT_CONTIG,
                                  VHIP_JIT_CONTIG);
