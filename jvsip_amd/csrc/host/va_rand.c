@@ -108,12 +108,13 @@ vsip_randstate *vsip_randcreate(vsip_index seed, vsip_index numseqs, vsip_index 
     if (!s) return NULL;
     s->type = (int)type;
     if (type != VSIP_PRNG) {
-        vsip_scalar_ue32 a = 1u, sum = 0u, i;
-        x0 = va_lcg_jump(VA_A0, VA_C0, (unsigned long)id, x0);
-        for (i = 0; i < k; i++) { sum += a; a *= VA_A0; }    /* a = A0^k, sum = 1 + ... + A0^(k-1) */
-        s->X = x0;
+        /* every numseqs-th value of the base sequence: the map of numseqs
+         * steps, a = A0^k, c = C0 (1 + A0 + ... + A0^(k-1)) */
+        unsigned a, c;
+        va_lcg_compose(VA_A0, VA_C0, (unsigned long)k, &a, &c);
+        s->X = va_lcg_jump(VA_A0, VA_C0, (unsigned long)id, x0);
         s->a = a;
-        s->c = sum * VA_C0;
+        s->c = c;
     } else {
         vsip_scalar_ue32 p = 1u, found = 0u;
         while (found < (vsip_scalar_ue32)id) {            /* the id-th odd prime */

@@ -44,7 +44,7 @@ def _val(v):
     return (v.r, v.i) if hasattr(v, "r") else v
 
 
-CREATE = [(0, 1, 1), (17, 1, 1), (12345, 4, 3), (99, 7, 7), (2 ** 31 + 5, 100, 100), (7, 3, 1)]
+CREATE = [(0, 1, 1), (17, 1, 1), (12345, 4, 3), (99, 7, 7), (2 ** 31 + 5, 100, 100), (7, 3, 1), (5, 1000003, 77)]
 
 
 @pytest.mark.parametrize("seed,numseqs,ident", CREATE)
