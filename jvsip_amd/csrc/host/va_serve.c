@@ -333,10 +333,15 @@ static int va_plan_cache_fill(VA_plan_cache *c, const vsip_amd_request *reqs, lo
     c->n = 0;
     if (!c->p) return -1;
     for (i = 0; i < nreq; i++) {
+        long j, rows = 0;
+        if (reqs[i].rows < 1 || reqs[i].n < 1) return -1;
         if (va_plan_get(c, reqs[i].n)) continue;
         if (c->n == 64) return -1;
-        if (va_fftm_init(&c->p[c->n], 4, VA_CCFFTOP, 512, (vsip_length)reqs[i].n, 1.0, -1, VSIP_ROW, 0,
-                         VSIP_ALG_TIME))
+        /* the plan's split scratch must hold the largest request of this length */
+        for (j = i; j < nreq; j++)
+            if (reqs[j].n == reqs[i].n && reqs[j].rows > rows) rows = reqs[j].rows;
+        if (va_fftm_init(&c->p[c->n], 4, VA_CCFFTOP, (vsip_length)rows, (vsip_length)reqs[i].n, 1.0, -1, VSIP_ROW,
+                         0, VSIP_ALG_TIME))
             return -1;
         c->n++;
     }
@@ -363,6 +368,7 @@ static int va_direct_launch(const VA_plan_cache *pc, const vsip_amd_request *r, 
     vhip_fftm_desc d[VA_MAX_DESC];
     int nd, k, rc = 0;
     if (!base) return -1;
+    if ((vsip_length)r->rows > base->mN) return -1;     /* beyond the plan's scratch */
     p = *base;
     p.mN = (vsip_length)r->rows;
     p.M = (vsip_length)r->rows;
