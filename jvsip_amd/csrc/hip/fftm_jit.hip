@@ -162,7 +162,7 @@ extern "C" int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vp
 // Launch d on its compiled kernel when one is loaded and d has the layout
 // the kernel serves; -100 otherwise (the caller uses the generic kernel).
 extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
-    if ((d->prec != 4 && d->prec != 8) || d->kind != VHIP_C2C || d->tap || d->acc) return -100;
+    if (!d->compiled || (d->prec != 4 && d->prec != 8) || d->kind != VHIP_C2C || d->tap || d->acc) return -100;
     const int min = side_mode(d->in, d->map_in, d->nvec), mout = side_mode(d->out, d->map_out, d->nvec);
     if (min < 0 || mout < 0) return -100;
     const long es = 2L * d->prec;

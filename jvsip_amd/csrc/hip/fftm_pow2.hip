@@ -40,8 +40,8 @@ int p2_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     if (span_in >= (1L << 31) || span_out >= (1L << 31)) return -100;
     const long ngroups = (d->nvec + VPB - 1) / VPB;
     const size_t lds = (size_t)VPB * P2<LOG2N>::LDSV * 8;
-    const char *venv = getenv("VSIP_AMD_P2_VARIANT");
-    int variant = venv ? atoi(venv) : 1;
+    static const int venv = getenv("VSIP_AMD_P2_VARIANT") ? atoi(getenv("VSIP_AMD_P2_VARIANT")) : 1;
+    int variant = venv;
     if (T >= 1024) variant = 0;
     // one group per workgroup (measured best up to 8192); beyond that a
     // persistent grid of as many workgroups as the LDS lets every CU hold
@@ -54,8 +54,8 @@ int p2_launch(const vhip_fftm_desc *d, void *stream, int cus) {
         per_wg = (int)((ngroups + wgs - 1) / wgs);
         if (per_wg < 1) per_wg = 1;
     }
-    const char *env = getenv("VSIP_AMD_P2_PER_WG");
-    if (env && atoi(env) > 0) per_wg = atoi(env);
+    static const int per_wg_env = getenv("VSIP_AMD_P2_PER_WG") ? atoi(getenv("VSIP_AMD_P2_PER_WG")) : 0;
+    if (per_wg_env > 0) per_wg = per_wg_env;
     const long grid = (ngroups + per_wg - 1) / per_wg;
     const bool plain = !d->conj_in && !d->conj_out && d->scale == 1.0;
     const dim3 gd((unsigned)grid), bd(T);
@@ -95,7 +95,8 @@ int pow2_try_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     if (!d->in.contig || !d->out.contig || d->in.split || d->out.split) return -100;
     if (d->map_in != VHIP_MAP_ALONG || d->map_out != VHIP_MAP_ALONG) return -100;
     if (d->in.nv0 < d->nvec || d->out.nv0 < d->nvec) return -100;   // single-level vectors only
-    if (getenv("VSIP_AMD_NO_POW2")) return -100;
+    static const bool no_pow2 = getenv("VSIP_AMD_NO_POW2") != nullptr;   // read once per process
+    if (no_pow2) return -100;
     switch (d->n) {
     case 256: return p2_launch<8>(d, stream, cus);
     case 512: return p2_launch<9>(d, stream, cus);

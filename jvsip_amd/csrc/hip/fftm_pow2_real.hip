@@ -61,7 +61,8 @@ int pow2_real_try_launch(const vhip_fftm_desc *d, void *stream) {
     if (!d->in.contig || !d->out.contig || d->in.split || d->out.split) return -100;
     if (d->map_in != VHIP_MAP_ALONG || d->map_out != VHIP_MAP_ALONG) return -100;
     if (d->in.nv0 < d->nvec || d->out.nv0 < d->nvec) return -100;   // single-level vectors only
-    if (getenv("VSIP_AMD_NO_POW2")) return -100;
+    static const bool no_pow2 = getenv("VSIP_AMD_NO_POW2") != nullptr;   // read once per process
+    if (no_pow2) return -100;
     switch (d->n) {
     case 256: return p2_real_dispatch<8>(d, stream);
     case 512: return p2_real_dispatch<9>(d, stream);

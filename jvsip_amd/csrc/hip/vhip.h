@@ -84,6 +84,8 @@ typedef struct {
     int stream;                    /* bit 0: load, bit 1: store the data non-temporally
                                       (set by the host for dense, line-aligned rows;
                                       honoured by the compiled-schedule kernels) */
+    int compiled;                  /* the plan may run on its compiled-schedule kernels */
+    int pad2_;
     vhip_view in, out;
     /* fused element work of a pipeline (NULL when unused; only the generic
      * kernel implements it):

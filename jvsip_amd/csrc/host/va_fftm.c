@@ -210,7 +210,7 @@ static int va_jit_lds_ok(const VA_sched *s, const VA_launch *L, int prec)
  * have: contiguous rows (ROW), lanes across columns (COL), and the passes
  * of the splits.  Execution finds the compiled kernel when the layout
  * matches. */
-static void va_jit_prepare(const VA_fftm *p)
+static void va_jit_prepare(VA_fftm *p)
 {
     const char *e = getenv("VSIP_AMD_JIT_MIN_ELEMS");
     long min_elems = e ? atol(e) : (1L << 18);
@@ -218,6 +218,8 @@ static void va_jit_prepare(const VA_fftm *p)
     int k;
     if ((p->type != VA_CCFFTIP && p->type != VA_CCFFTOP) || p->staged) return;
     if ((long)p->mN * n < min_elems) return;
+    if (getenv("VSIP_AMD_JIT") && getenv("VSIP_AMD_JIT")[0] == '0') return;
+    p->jit = 1;
     if (p->npass == 1) {
         const VA_sched *s = &p->pass[0];
         if (p->major == VSIP_ROW) {
@@ -432,6 +434,7 @@ static void va_desc_common(vhip_fftm_desc *d, const VA_fftm *p, const VA_sched *
     d->map_out = map_out;
     d->tw = s->dev_tw;
     d->scale = 1.0;
+    d->compiled = p->jit;
 }
 
 /*

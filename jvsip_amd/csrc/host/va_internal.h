@@ -132,6 +132,7 @@ typedef struct VA_fftm {
      * sm, a forward c2c plan on them (twice, with the chirp filter product
      * between, for Bluestein), POST pass scratch -> view; `chunk` vectors per
      * slab */
+    int jit;                 /* 1: schedules compiled at creation may run this plan */
     int staged;              /* 0 none, 1 real transform around a long c2c, 2 Bluestein */
     vsip_length sm;
     vsip_length chunk;
