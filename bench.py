@@ -20,7 +20,8 @@ Also reported:
   cpu_baseline  rrjudd/jvsip's own C (oracle/_ref/libjvsip_ref.so) on the host
                 cores of this machine, on a bounded sample of the same workload
   accuracy      relative L2 of 8 output rows against jvsip's output fixture
-                and against the double-precision oracle
+                (tests/golden) and against numpy's double-precision FFT;
+                the oracle under oracle/ is used only by the cpu_baseline leg
 """
 import argparse
 import ctypes
@@ -118,6 +119,7 @@ def cpu_baseline(threads, steps=5, rows_single=256):
         cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
     except Exception:
         cpu = "unknown"
+    per_config = cpu_configs(ref)
     return {
         "value": round(gbs_all, 4), "unit": "GB/s", "cores": threads, "kind": "reference",
         "sample": "jvsip vsip_ccfftmip_f fwd+inv on the full 4096x4096 matrix as %d row slabs "
@@ -125,20 +127,97 @@ def cpu_baseline(threads, steps=5, rows_single=256):
         "single_core": {"value": round(gbs_one, 4), "unit": "GB/s", "cores": 1,
                         "sample": "fwd+inv on %d rows x 4096, %.3f s" % (rows_single, dt1)},
         "cpu_model": cpu,
+        "configs": per_config,
     }
 
 
+def cpu_configs(ref):
+    """jvsip's C, one core, one call each (after one untimed call) on the
+    other configurations' shapes: seconds per call, to set beside the GPU
+    figures of the same shapes."""
+    from jvsip_amd import datagen
+
+    def timed(fn):
+        fn()
+        t0 = time.perf_counter()
+        fn()
+        return round(time.perf_counter() - t0, 5)
+
+    out = {}
+    for name, m, n, major, parent in (("c3_col_1920x1000", 1920, 1000, 1, 2000),
+                                      ("c3_row_1024x2187", 1024, 2187, 0, 4374),
+                                      ("c3_row_512x1001", 512, 1001, 0, 2002)):
+        x = datagen.complex_data(3, (m * parent,))
+        blk = ref.cblockbind_f(x.ctypes.data, None, m * parent, 0)
+        ref.cblockadmit_f(blk, 1)
+        xv = ref.cmbind_f(blk, 0, parent, m, 1, n)
+        yb = np.zeros(m * n, np.complex64)
+        yblk = ref.cblockbind_f(yb.ctypes.data, None, m * n, 0)
+        ref.cblockadmit_f(yblk, 1)
+        yv = ref.cmbind_f(yblk, 0, n, m, 1, n)
+        p = ref.ccfftmop_create_f(m, n, 1.0, -1, major, 0, 1)
+        out[name] = {"s_per_call": timed(lambda: ref.ccfftmop_f(p, xv, yv))}
+        ref.fftm_destroy_f(p)
+        ref.cmdestroy_f(xv)
+        ref.cmdestroy_f(yv)
+        ref.cblockdestroy_f(blk)
+        ref.cblockdestroy_f(yblk)
+    xd = datagen.complex_data(3, (2048 * 2048,), dtype=np.complex128)
+    blk = ref.cblockbind_d(xd.ctypes.data, None, xd.size, 0)
+    ref.cblockadmit_d(blk, 1)
+    v = ref.cmbind_d(blk, 0, 2048, 2048, 1, 2048)
+    p = ref.ccfftmip_create_d(2048, 2048, 1.0, -1, 0, 0, 1)
+    out["c3_fp64_2048x2048"] = {"s_per_call": timed(lambda: ref.ccfftmip_d(p, v))}
+    ref.fftm_destroy_d(p)
+    ref.cmdestroy_d(v)
+    ref.cblockdestroy_d(blk)
+    xr = datagen.real_data(2, (8192 * 2048,))
+    rb = ref.blockbind_f(xr.ctypes.data, xr.size, 0)
+    ref.blockadmit_f(rb, 1)
+    xv = ref.mbind_f(rb, 0, 2048, 8192, 1, 2048)
+    z = np.zeros(8192 * 1025, np.complex64)
+    zb = ref.cblockbind_f(z.ctypes.data, None, z.size, 0)
+    ref.cblockadmit_f(zb, 1)
+    zv = ref.cmbind_f(zb, 0, 1025, 8192, 1, 1025)
+    p = ref.rcfftmop_create_f(8192, 2048, 1.0, 0, 0, 1)
+    out["c2_rcfftmop_8192x2048"] = {"s_per_call": timed(lambda: ref.rcfftmop_f(p, xv, zv))}
+    ref.fftm_destroy_f(p)
+    ref.mdestroy_f(xv)
+    ref.cmdestroy_f(zv)
+    ref.blockdestroy_f(rb)
+    ref.cblockdestroy_f(zb)
+    from jvsip_amd import komega as kw
+    k = kw.KOmega(ref, 1024, 8192, 8)
+    k.zero()
+    d = kw.data_matrix(ref, kw.plane_waves(1024, 8192, [(1024, 3, 1.0)], seed=7, noise=0.5))
+    out["f3_komega_1024x8192"] = {"s_per_call": timed(lambda: k.step(d))}
+    ref.malldestroy_f(d)
+    k.destroy()
+    return out
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+def fft64(x, axis=-1):
+    """numpy's FFT in double precision (the accuracy reference of the bench)"""
+    return np.fft.fft(np.asarray(x, dtype=np.complex128), axis=axis)
+
+
 def accuracy(lib, view_fn):
-    """Forward transform of the seed-2 input: 8 rows against jvsip and the oracle."""
+    """Forward transform of the seed-2 input: 8 rows against jvsip's output
+    (tests/golden fixture) and numpy's FP64 FFT."""
     from jvsip_amd import datagen, fftm
-    from oracle import pyoracle as po
     f = np.load(os.path.join(ROOT, "tests", "golden", "jvsip_c2_4096x4096_fwd.npz"))
     x = datagen.complex_data(2, (M, N), dist="uniform")
     y = fftm.ccfftm(lib, x.ravel(), fftm.dense_view(M, N)).reshape(M, N)
     rows = f["rows"]
-    want = po.ccfftm(x[rows].ravel(), fftm.dense_view(len(rows), N)).reshape(len(rows), N)
-    return {"rel_l2_vs_jvsip": po.rel_l2(y[rows], f["out_rows"]),
-            "rel_l2_vs_fp64_oracle": po.rel_l2(y[rows], want),
+    return {"rel_l2_vs_jvsip": rel_l2(y[rows], f["out_rows"]),
+            "rel_l2_vs_fp64": rel_l2(y[rows], fft64(x[rows], axis=1)),
             "rows": [int(r) for r in rows]}
 
 
@@ -183,7 +262,6 @@ def _time_calls(lib, hip, fn, reps, warm_s=0.03):
 def config_c1(lib, hip):
     """plain vsip_ccfftmip_f forward, ROW, 64 x 1024, scale 1, N(0,1) seed 1"""
     from jvsip_amd import datagen, fftm
-    from oracle import pyoracle as po
     x = datagen.complex_data(1, (64, 1024))
     v = _dev_view(lib, x)
     p = lib.ccfftmip_create_f(64, 1024, 1.0, -1, 0, 0, 1)
@@ -200,8 +278,8 @@ def config_c1(lib, hip):
     out = {"shape": "64x1024", "us_per_call_back_to_back": round(wall * 1e6, 2),
            "us_per_call_kernel_stream": round(gpu * 1e6, 2), "us_call_plus_sync": round(lat * 1e6, 2),
            "gb_per_s": round(2 * 64 * 1024 * 8 / gpu / 1e9, 1),
-           "rel_l2_vs_jvsip": po.rel_l2(y, f["out"]),
-           "rel_l2_vs_fp64_oracle": po.rel_l2(y, po.ccfftm(x.ravel(), fftm.dense_view(64, 1024)))}
+           "rel_l2_vs_jvsip": rel_l2(y, f["out"]),
+           "rel_l2_vs_fp64": rel_l2(y, fft64(x, axis=1))}
     lib.fftm_destroy_f(p)
     lib.cmalldestroy_f(v)
     return out
@@ -249,7 +327,6 @@ def config_host_buffers(lib, reps=3):
 def config_c2_real(lib, hip):
     """vsip_rcfftmop_f 8192 x 2048 real N(0,1) then vsip_crfftmop_f (scale 1/2048) back"""
     from jvsip_amd import datagen
-    from oracle import pyoracle as po
     M, N = 8192, 2048
     x = datagen.real_data(2, (M, N))
     xv = lib.mcreate_f(M, N, 0, 0)
@@ -271,7 +348,7 @@ def config_c2_real(lib, hip):
     bytes_ = M * N * 4 + M * (N // 2 + 1) * 8
     out = {"shape": "8192x2048 real", "rc_us": round(t_rc * 1e6, 2), "cr_us": round(t_cr * 1e6, 2),
            "rc_gb_per_s": round(bytes_ / t_rc / 1e9, 1), "cr_gb_per_s": round(bytes_ / t_cr / 1e9, 1),
-           "rc_rel_l2_vs_fp64": po.rel_l2(z[rows], zz), "round_trip_rel_l2": po.rel_l2(back, x)}
+           "rc_rel_l2_vs_fp64": rel_l2(z[rows], zz), "round_trip_rel_l2": rel_l2(back, x)}
     for h in (rc, cr):
         lib.fftm_destroy_f(h)
     for v in (xv, bv):
@@ -283,7 +360,6 @@ def config_c2_real(lib, hip):
 def config_c3(lib, hip):
     """strided / mixed-radix ccfftmop_f (parent rows 2x the row length) and the FP64 2048^2 case"""
     from jvsip_amd import datagen, fftm
-    from oracle import pyoracle as po
     out = {}
     cases = [("col_1920x1000", "jvsip_c3_col_1920x1000.npz", 1920, 1000, 1, 2000),
              ("row_1024x2187", "jvsip_c3_row_1024x2187.npz", 1024, 2187, 0, 4374),
@@ -304,10 +380,10 @@ def config_c3(lib, hip):
         v = fftm.View(0, parent, m, 1, n)
         xs = fftm.gather(x, v)
         sel = f["rows"]
-        want = po.ccfftm((xs[sel] if major == 0 else xs[:, sel].T).ravel(), fftm.dense_view(len(sel), xs.shape[1] if major == 0 else m))
+        want = fft64(xs[sel] if major == 0 else xs[:, sel].T, axis=1)
         out[name] = {"us": round(t * 1e6, 2), "gb_per_s": round(2 * m * n * 8 / t / 1e9, 1),
-                     "rel_l2_vs_jvsip": po.rel_l2(vecs[sel], f["out_rows"]),
-                     "rel_l2_vs_fp64_oracle": po.rel_l2(vecs[sel], want),
+                     "rel_l2_vs_jvsip": rel_l2(vecs[sel], f["out_rows"]),
+                     "rel_l2_vs_fp64": rel_l2(vecs[sel], want),
                      "schedule": json.loads(lib.describe(p))["passes"][0]["radix"]}
         lib.fftm_destroy_f(p)
         lib.cmalldestroy_f(yv)
@@ -328,10 +404,9 @@ def config_c3(lib, hip):
     yd = np.empty((2048, 2048), np.complex128)
     lib.cmcopyto_user_d(vd, 0, yd.ctypes.data, None)
     rows = np.arange(0, 2048, 256)
-    want = po.ccfftm(xd[rows].ravel(), fftm.dense_view(len(rows), 2048)).reshape(len(rows), 2048)
     out["fp64_2048x2048"] = {"us_per_transform": round(t / 2 * 1e6, 2),
                              "gb_per_s": round(2 * 2048 * 2048 * 16 / (t / 2) / 1e9, 1),
-                             "rel_l2_vs_fp64_oracle": po.rel_l2(yd[rows], want)}
+                             "rel_l2_vs_numpy_fp64": rel_l2(yd[rows], fft64(xd[rows], axis=1))}
     lib.fftm_destroy_d(pd)
     lib.fftm_destroy_d(pinv)
     lib.cmalldestroy_d(vd)
@@ -352,7 +427,6 @@ def config_c4(lib, hip, njobs=3000):
     """njobs FFTs of log-uniform 2^a 3^b 5^c lengths in [64, 65536], rows so the
     total is ~2^27 complex elements: one batched graph launch vs plain per-call"""
     from jvsip_amd import datagen, fftm
-    from oracle import pyoracle as po
     lengths = _c4_lengths(njobs, 4)
     per_job = (1 << 27) / njobs
     rows = [max(1, int(round(per_job / n))) for n in lengths]
@@ -389,7 +463,7 @@ def config_c4(lib, hip, njobs=3000):
         lib.ccfftmip_f(p, v)
         lib.amd_sync()
     wall_ps = time.perf_counter() - t0
-    # accuracy: batched vs plain on the same fresh input, and a sample vs the oracle
+    # accuracy: batched vs plain on the same fresh input, and a sample vs numpy's FP64 FFT
     lib.cmcopyfrom_user_f(pool.ctypes.data, None, 0, whole)
     lib.cmcopyfrom_user_f(pool.ctypes.data, None, 0, ref_whole)
     lib.amd_batch_run_f(b)
@@ -404,9 +478,9 @@ def config_c4(lib, hip, njobs=3000):
     worst_plain = 0.0
     for j, (r, n) in enumerate(zip(rows, lengths)):
         sl = slice(off, off + r * n)
-        worst_plain = max(worst_plain, po.rel_l2(got[sl], want[sl]))
+        worst_plain = max(worst_plain, rel_l2(got[sl], want[sl]))
         if j in sample:
-            errs.append(po.rel_l2(got[sl], po.ccfftm(pool[sl], fftm.dense_view(r, n))))
+            errs.append(rel_l2(got[sl], fft64(pool[sl].reshape(r, n), axis=1)))
         off += r * n
     bytes_ = 2 * total * 8
     out = {"jobs": njobs, "elements": total, "distinct_lengths": len(set(lengths)), "batch": info,
@@ -417,7 +491,7 @@ def config_c4(lib, hip, njobs=3000):
            "speedup_vs_plain_back_to_back": round(wall_p / wall_b, 2),
            "speedup_vs_plain_sync_each": round(wall_ps / wall_b, 2),
            "max_rel_l2_batched_vs_plain": worst_plain,
-           "max_rel_l2_vs_fp64_oracle_sample": max(errs)}
+           "max_rel_l2_vs_fp64_sample": max(errs)}
     lib.amd_batch_destroy_f(b)
     for p in plans:
         lib.fftm_destroy_f(p)
