@@ -224,6 +224,19 @@ int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max)
      * kernel places consecutive groups on one XCD; measured faster than 8
      * vectors at one workgroup per CU) */
     vpb_across = (va_lds_bytes(s, prec, 16) <= lds_max / 2) ? 16 : 4;
+    /* few vectors: fewer per workgroup, until at least half the CUs get one
+     * (k-omega's 128-row real transform: 26.6 -> 19.2 us per step; 1000
+     * columns keep 4, which beats 2 there) */
+    if (nvec > 0) {
+        int cus = 256;
+        long half;
+        vhip_cu_count(&cus);
+        half = (cus > 1 ? cus : 256) / 2;
+        if ((long)nvec < (long)vpb_across * half) {
+            long v = (long)nvec / half;
+            vpb_across = v < 1 ? 1 : (int)v;
+        }
+    }
     vpb_across = va_env_cap("VSIP_AMD_VPB_ACROSS", vpb_across);
     if ((vsip_length)vpb_across > nvec && nvec > 0) vpb_across = (int)nvec;
     if (va_pick_launch(s, prec, vpb_across, lds_max, lds_max, &s->across)) return -2;
