@@ -106,7 +106,7 @@ def test_headline_schedule(lib):
 
 
 def test_no_device_constructors_fail_cleanly(lib):
-    import torch  # noqa: F401  (only to skip on a GPU box)
+    import torch  # noqa: F401  (only to skip where a device is present)
     if torch.cuda.is_available():
         pytest.skip("a device is present")
     assert not lib.ccfftmip_create_f(8, 4, 1.0, -1, 0, 0, 0)
