@@ -345,7 +345,7 @@ __device__ __forceinline__ cpx<T> load_in(const vhip_fftm_desc &d, int vb, int j
             v.x = Buf<T>::template lds<kLdData>(r, o, so);
             v.y = Buf<T>::template lds<kLdData>(r, o + es * ES, so);
         }
-        if (d.tap) {
+        if (!SOFF && d.tap) {   // fused taper: standalone launches only
             const long s0 = 2L * (j + leg) * d.tap_s;
             v.x *= ((const T *)d.tap)[s0];
             v.y *= ((const T *)d.tap)[s0 + d.tap_s];
@@ -359,7 +359,7 @@ __device__ __forceinline__ cpx<T> load_in(const vhip_fftm_desc &d, int vb, int j
     } else {
         v = Buf<T>::template ldc<kLdData>(make_rsrc(d.in.re, d.in.span), (vb + (j + lv) * es) * 2 * ES, ls * es * 2 * ES);
     }
-    if (d.tap) {
+    if (!SOFF && d.tap) {   // fused taper: standalone launches only
         const T t = ((const T *)d.tap)[(long)(j + leg) * d.tap_s];
         v.x *= t;
         v.y *= t;
@@ -408,7 +408,7 @@ __device__ __forceinline__ void store_out(const vhip_fftm_desc &d, int vb, int v
         }
         return;
     }
-    if (d.acc) {                    // power accumulation in place of the store
+    if (!SOFF && d.acc) {           // power accumulation in place of the store (standalone launches only)
         T *a = (T *)d.acc + (vb + (j + leg) * es);
         *a += (T)d.acc_gain * (v.x * v.x + v.y * v.y);
         return;

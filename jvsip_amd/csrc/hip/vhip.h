@@ -87,8 +87,8 @@ typedef struct {
     int compiled;                  /* the plan may run on its compiled-schedule kernels */
     int pad2_;
     vhip_view in, out;
-    /* fused element work of a pipeline (NULL when unused; only the generic
-     * kernel implements it):
+    /* fused element work of a pipeline (NULL when unused; only the
+     * standalone generic kernel implements it, not the batched one):
      *   tap   real weights applied on load, by element position within the
      *         vector: R2C real sample s times tap[s * tap_s]; C2C element j
      *         times tap[j * tap_s]
