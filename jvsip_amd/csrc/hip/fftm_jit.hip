@@ -47,7 +47,7 @@ bool jit_on() {
 // the template instance of a schedule; empty when the schedule has no
 // compiled form (more than 8 stages, or not a single pass)
 std::string instance(int prec, int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
-    if (nst < 1 || nst > 8 || min < 0 || min > 1 || mout < 0 || mout > 1 || (prec != 4 && prec != 8)) return "";
+    if (nst < 1 || nst > 8 || min < 0 || min > 3 || mout < 0 || mout > 3 || (prec != 4 && prec != 8)) return "";
     int r[8];
     for (int s = 0; s < 8; ++s) r[s] = s < nst ? radix[s] : 1;
     char buf[256];
@@ -159,18 +159,37 @@ extern "C" int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vp
     return (it != g_cache.end() && it->second.fn) ? 1 : 0;
 }
 
+// layouts of a descriptor as template arguments (bit 1: fused taper / power
+// accumulation); false when the compiled kernels cannot run it
+static bool desc_layouts(const vhip_fftm_desc *d, int &min, int &mout) {
+    if ((d->prec != 4 && d->prec != 8) || d->kind != VHIP_C2C) return false;
+    min = side_mode(d->in, d->map_in, d->nvec);
+    mout = side_mode(d->out, d->map_out, d->nvec);
+    if (min < 0 || mout < 0) return false;
+    if (d->tap) min |= 2;
+    if (d->acc) mout |= 2;
+    return true;
+}
+
+// Compile the kernel a descriptor needs (for launches the plan did not
+// prepare at creation, such as fused pipelines); 0 when it is available.
+extern "C" int vhip_jit_prepare_desc(const vhip_fftm_desc *d) {
+    int min, mout;
+    if (!desc_layouts(d, min, mout)) return -1;
+    return vhip_jit_prepare(d->prec, d->n, d->nst, d->radix, d->vpb, d->threads, min, mout);
+}
+
 // Launch d on its compiled kernel when one is loaded and d has the layout
 // the kernel serves; -100 otherwise (the caller uses the generic kernel).
 extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
-    if (!d->compiled || (d->prec != 4 && d->prec != 8) || d->kind != VHIP_C2C || d->tap || d->acc) return -100;
-    const int min = side_mode(d->in, d->map_in, d->nvec), mout = side_mode(d->out, d->map_out, d->nvec);
-    if (min < 0 || mout < 0) return -100;
+    int min, mout;
+    if (!d->compiled || !desc_layouts(d, min, mout)) return -100;
     const long es = 2L * d->prec;
-    if (min == 0) {
+    if ((min & 1) == 0) {
         if (d->nvec > 1 && d->in.vs0 < d->n) return -100;                 // rows must not overlap
         if (((long)(d->vpb - 1) * d->in.vs0 + d->n) * es >= (1L << 31)) return -100;
     }
-    if (mout == 0) {
+    if ((mout & 1) == 0) {
         if (d->nvec > 1 && d->out.vs0 < d->n) return -100;
         if (((long)(d->vpb - 1) * d->out.vs0 + d->n) * es >= (1L << 31)) return -100;
     }

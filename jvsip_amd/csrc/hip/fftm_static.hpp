@@ -15,18 +15,21 @@
 // (j - k) R + k + r Ns (to HBM, scaled, in the last stage).  Inverse
 // transforms conjugate on the way in and out.
 //
-// Global layouts:
+// Global layouts (bit 0 of MIN / MOUT):
 //   ST_CONTIG  rows of unit element stride, one vector stride (vs0); lanes
 //              walk along the vector (v = b / (N/R))
 //   ST_ACROSS  any element stride, two-level vector offsets
 //              (v % nv0) vs0 + (v / nv0) vs1; lanes walk across the group's
 //              vectors (v = b mod VPB), for unit vector stride (columns)
-// Middle stages always walk along the vector in LDS.
+// Middle stages always walk along the vector in LDS.  Bit 1 of MIN fuses a
+// real taper into the load (element j times d.tap[j * d.tap_s]); bit 1 of
+// MOUT replaces the store by d.acc[o] += d.acc_gain |y|^2 (vhip_fftm_desc).
 #pragma once
 #include "fftm_core.hpp"
 
 #define ST_CONTIG 0
 #define ST_ACROSS 1
+#define ST_FUSED 2
 
 namespace vhipk {
 namespace st {
@@ -56,6 +59,7 @@ struct Io {
     const int *voff_in;    // ST_ACROSS: element offset of each vector of the group
     const int *voff_out;
     const int *vidx;       // post-twiddled launches: index of each vector within its row
+    long vbase;            // first vector of the group
     T isgn, osgn, scale;
 };
 
@@ -68,9 +72,11 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
     constexpr int NBT = VPB * NBV;
     constexpr int B = (NBT + NT - 1) / NT;
     constexpr bool FIRST = (S == 0), LAST = (S == SC::NST - 1);
+    constexpr int LIN = MIN & 1, LOUT = MOUT & 1;
+    constexpr bool TAP = (MIN & ST_FUSED) != 0, ACC = (MOUT & ST_FUSED) != 0;
     // lane mapping of this stage: the global layout's for the first and last
     // stage (the load's when one stage does both), along the vector otherwise
-    constexpr int MODE = FIRST ? MIN : (LAST ? MOUT : ST_CONTIG);
+    constexpr int MODE = FIRST ? LIN : (LAST ? LOUT : ST_CONTIG);
     constexpr int LS = lstride<T>(N);
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
@@ -91,11 +97,16 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     cpx<T> y;
-                    if (MIN == ST_CONTIG && (d.stream & 1))
+                    if (LIN == ST_CONTIG && (d.stream & 1))
                         y = Buf<T>::template ldc<2>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
-                    else if (MIN == ST_CONTIG)
+                    else if (LIN == ST_CONTIG)
                         y = Buf<T>::template ldc<kLdData>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
                     else y = Buf<T>::template ldc<kLdData>(io.in, (io.voff_in[v] + (j + r * NBV) * io.ies) * ES, 0);
+                    if (TAP) {
+                        const T tv = ((const T *)d.tap)[(long)(j + r * NBV) * d.tap_s];
+                        y.x *= tv;
+                        y.y *= tv;
+                    }
                     y.y *= io.isgn;
                     x[i][r] = y;
                 }
@@ -129,9 +140,14 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
                         y = cmul(y, cmul(w[t & ((1 << d.ptw_shift) - 1)], w[(1 << d.ptw_shift) + (t >> d.ptw_shift)]));
                     }
                     const cpx<T> z = {y.x * io.scale, y.y * (io.scale * io.osgn)};
-                    if (MOUT == ST_CONTIG && (d.stream & 2))
+                    if (ACC) {
+                        const long o = LOUT == ST_CONTIG ? (io.vbase + v) * io.ovs + j + r * NBV
+                                                         : (long)io.voff_out[v] + (long)(j + r * NBV) * io.oes;
+                        T *a = (T *)d.acc + o;
+                        *a += (T)d.acc_gain * (z.x * z.x + z.y * z.y);
+                    } else if (LOUT == ST_CONTIG && (d.stream & 2))
                         Buf<T>::template stc<18>(io.out, (int)(v * io.ovs + j) * ES, r * NBV * ES, z);
-                    else if (MOUT == ST_CONTIG)
+                    else if (LOUT == ST_CONTIG)
                         Buf<T>::template stc<kStData>(io.out, (int)(v * io.ovs + j) * ES, r * NBV * ES, z);
                     else Buf<T>::template stc<kStData>(io.out, (io.voff_out[v] + (j + r * NBV) * io.oes) * ES, 0, z);
                 }
@@ -168,7 +184,7 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
     // all of the workgroup's LDS without a launch attribute
     __shared__ __attribute__((aligned(16))) cpx<T> slds[VPB * LS];
     __shared__ int voff[3 * VPB];
-    constexpr bool ACROSS = (MIN == ST_ACROSS || MOUT == ST_ACROSS);
+    constexpr bool ACROSS = ((MIN & 1) == ST_ACROSS || (MOUT & 1) == ST_ACROSS);
     const long g = ACROSS ? (long)xcd_group(blockIdx.x, gridDim.x) : (long)blockIdx.x;
     const long vbase = g * VPB;
     long nvl = d.nvec - vbase;
@@ -190,11 +206,12 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
         }
         __syncthreads();
     }
-    if (MIN == ST_CONTIG)
+    io.vbase = vbase;
+    if ((MIN & 1) == ST_CONTIG)
         io.in = make_rsrc((const cpx<T> *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + N) * ES);
     else
         io.in = make_rsrc(d.in.re, d.in.span);
-    if (MOUT == ST_CONTIG)
+    if ((MOUT & 1) == ST_CONTIG)
         io.out = make_rsrc((cpx<T> *)d.out.re + vbase * io.ovs, ((io.nv - 1) * io.ovs + N) * ES);
     else
         io.out = make_rsrc(d.out.re, d.out.span);

@@ -237,13 +237,16 @@ int vhip_rng_fill(const vhip_rng_desc *d, void *stream);
  * descriptor on it when it is loaded and the descriptor's layouts match,
  * else returns -100.  Layouts: VHIP_JIT_CONTIG (interleaved rows of unit
  * element stride, single-level vector stride), VHIP_JIT_ACROSS (interleaved,
- * lanes across vectors, any strides within vhip_view.span). */
+ * lanes across vectors, any strides within vhip_view.span); + VHIP_JIT_FUSED
+ * on the load side fuses vhip_fftm_desc.tap, on the store side .acc. */
 #define VHIP_JIT_CONTIG 0
 #define VHIP_JIT_ACROSS 1
+#define VHIP_JIT_FUSED 2
 int vhip_jit_prepare(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in,
                      int layout_out);
 int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);
 int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream);
+int vhip_jit_prepare_desc(const vhip_fftm_desc *d);
 int vhip_jit_compile_check(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in,
                            int layout_out, const char *arch, size_t *code_bytes);
 const char *vhip_jit_last_error(void);

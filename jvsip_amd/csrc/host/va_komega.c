@@ -92,6 +92,9 @@ int vsip_amd_komega_f(const vsip_fftm_f *rcfftm, const vsip_fftm_f *ccfftm, cons
     }
     d[0].acc = (char *)g->block->dev_r + (size_t)g->offset * sizeof(float);
     d[0].acc_gain = gain;
+    /* the column transform's schedule compiled with the taper and the power
+     * accumulation (once per process; the generic kernel otherwise) */
+    if (pc->jit && vhip_jit_prepare_desc(&d[0]) == 0) d[0].compiled = 1;
     rc = vhip_fftm_launch(&d[0], vhip_stream());
     if (rc) { va_error(fn, vhip_error_string(rc), rc); return -1; }
     return 0;

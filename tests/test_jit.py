@@ -38,7 +38,8 @@ def test_planner_schedules_compile(lib, so, prec, n, nvec):
     s = lib.schedule(prec, n, nvec, 163840)
     assert s["rc"] == 0
     rad = (ctypes.c_int * len(s["radix"]))(*s["radix"])
-    for shape, lin, lout in (("along", 0, 0), ("across", 1, 1), ("across", 0, 1)):
+    for shape, lin, lout in (("along", 0, 0), ("across", 1, 1), ("across", 0, 1), ("along", 2, 2),
+                             ("across", 3, 3)):
         vpb, thr, _ = s[shape]
         size = ctypes.c_size_t()
         rc = so.vhip_jit_compile_check(prec, n, len(s["radix"]), rad, vpb, thr, lin, lout, b"gfx950",
@@ -51,5 +52,5 @@ def test_refuses_schedules_without_a_compiled_form(so):
     rad = (ctypes.c_int * 9)(*([2] * 9))
     assert so.vhip_jit_compile_check(4, 512, 9, rad, 1, 64, 0, 0, b"gfx950", None) != 0
     rad = (ctypes.c_int * 2)(4, 4)
-    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 2, 0, b"gfx950", None) != 0
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 4, 0, b"gfx950", None) != 0
     assert so.vhip_jit_compile_check(2, 16, 2, rad, 1, 64, 0, 0, b"gfx950", None) != 0
