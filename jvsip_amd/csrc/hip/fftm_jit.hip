@@ -47,7 +47,8 @@ bool jit_on() {
 // the template instance of a schedule; empty when the schedule has no
 // compiled form (more than 8 stages, or not a single pass)
 std::string instance(int prec, int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
-    if (nst < 1 || nst > 8 || min < 0 || min > 3 || mout < 0 || mout > 3 || (prec != 4 && prec != 8)) return "";
+    if (nst < 1 || nst > 8 || min < 0 || min > 7 || mout < 0 || mout > 3 || (prec != 4 && prec != 8)) return "";
+    if ((min & VHIP_JIT_REAL) && (mout & VHIP_JIT_FUSED)) return "";   // no power accumulation after a real input
     int r[8];
     for (int s = 0; s < 8; ++s) r[s] = s < nst ? radix[s] : 1;
     char buf[256];
@@ -162,12 +163,14 @@ extern "C" int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vp
 // layouts of a descriptor as template arguments (bit 1: fused taper / power
 // accumulation); false when the compiled kernels cannot run it
 static bool desc_layouts(const vhip_fftm_desc *d, int &min, int &mout) {
-    if ((d->prec != 4 && d->prec != 8) || d->kind != VHIP_C2C) return false;
+    if ((d->prec != 4 && d->prec != 8) || (d->kind != VHIP_C2C && d->kind != VHIP_R2C)) return false;
+    if (d->kind == VHIP_R2C && (d->acc || d->ptw)) return false;
     min = side_mode(d->in, d->map_in, d->nvec);
     mout = side_mode(d->out, d->map_out, d->nvec);
     if (min < 0 || mout < 0) return false;
-    if (d->tap) min |= 2;
-    if (d->acc) mout |= 2;
+    if (d->tap) min |= VHIP_JIT_FUSED;
+    if (d->acc) mout |= VHIP_JIT_FUSED;
+    if (d->kind == VHIP_R2C) min |= VHIP_JIT_REAL;
     return true;
 }
 
@@ -184,14 +187,18 @@ extern "C" int vhip_jit_prepare_desc(const vhip_fftm_desc *d) {
 extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
     int min, mout;
     if (!d->compiled || !desc_layouts(d, min, mout)) return -100;
+    const bool real = (min & VHIP_JIT_REAL) != 0;
     const long es = 2L * d->prec;
+    // contiguous sides: rows must not overlap and a group's rows span < 2 GiB
+    const long in_len = real ? 2L * d->n : d->n, in_esz = real ? d->prec : es;
+    const long out_len = real ? d->n + 1L : d->n;
     if ((min & 1) == 0) {
-        if (d->nvec > 1 && d->in.vs0 < d->n) return -100;                 // rows must not overlap
-        if (((long)(d->vpb - 1) * d->in.vs0 + d->n) * es >= (1L << 31)) return -100;
+        if (d->nvec > 1 && d->in.vs0 < in_len) return -100;
+        if (((long)(d->vpb - 1) * d->in.vs0 + in_len) * in_esz >= (1L << 31)) return -100;
     }
     if ((mout & 1) == 0) {
-        if (d->nvec > 1 && d->out.vs0 < d->n) return -100;
-        if (((long)(d->vpb - 1) * d->out.vs0 + d->n) * es >= (1L << 31)) return -100;
+        if (d->nvec > 1 && d->out.vs0 < out_len) return -100;
+        if (((long)(d->vpb - 1) * d->out.vs0 + out_len) * es >= (1L << 31)) return -100;
     }
     hipFunction_t fn = nullptr;
     {

@@ -24,12 +24,17 @@
 // Middle stages always walk along the vector in LDS.  Bit 1 of MIN fuses a
 // real taper into the load (element j times d.tap[j * d.tap_s]); bit 1 of
 // MOUT replaces the store by d.acc[o] += d.acc_gain |y|^2 (vhip_fftm_desc).
+// Bit 2 of MIN makes the transform real to complex (rcfftmop): the load
+// reads real samples 2m, 2m + 1 as complex element m, the last stage leaves
+// Y in LDS, and the n + 1 outputs are combined from Y[k] and Y[n - k] with
+// the Euler table (r2c_post of the generic kernel).
 #pragma once
 #include "fftm_core.hpp"
 
 #define ST_CONTIG 0
 #define ST_ACROSS 1
 #define ST_FUSED 2
+#define ST_REAL 4
 
 namespace vhipk {
 namespace st {
@@ -74,6 +79,8 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
     constexpr bool FIRST = (S == 0), LAST = (S == SC::NST - 1);
     constexpr int LIN = MIN & 1, LOUT = MOUT & 1;
     constexpr bool TAP = (MIN & ST_FUSED) != 0, ACC = (MOUT & ST_FUSED) != 0;
+    constexpr bool REAL = (MIN & ST_REAL) != 0;
+    constexpr int RS = (int)sizeof(T);
     // lane mapping of this stage: the global layout's for the first and last
     // stage (the load's when one stage does both), along the vector otherwise
     constexpr int MODE = FIRST ? LIN : (LAST ? LOUT : ST_CONTIG);
@@ -97,17 +104,34 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     cpx<T> y;
-                    if (LIN == ST_CONTIG && (d.stream & 1))
-                        y = Buf<T>::template ldc<2>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
-                    else if (LIN == ST_CONTIG)
-                        y = Buf<T>::template ldc<kLdData>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
-                    else y = Buf<T>::template ldc<kLdData>(io.in, (io.voff_in[v] + (j + r * NBV) * io.ies) * ES, 0);
-                    if (TAP) {
-                        const T tv = ((const T *)d.tap)[(long)(j + r * NBV) * d.tap_s];
-                        y.x *= tv;
-                        y.y *= tv;
+                    if (REAL) {             // real samples 2m, 2m + 1 (offsets in real elements)
+                        const int m2 = 2 * (j + r * NBV);
+                        if (LIN == ST_CONTIG) {
+                            y = Buf<T>::template ldc<kLdData>(io.in, (int)(v * io.ivs + m2) * RS, 0);
+                        } else {
+                            const int o = (io.voff_in[v] + m2 * io.ies) * RS;
+                            y.x = Buf<T>::template lds<kLdData>(io.in, o, 0);
+                            y.y = Buf<T>::template lds<kLdData>(io.in, o + io.ies * RS, 0);
+                        }
+                        if (TAP) {
+                            const long s0 = (long)m2 * d.tap_s;
+                            y.x *= ((const T *)d.tap)[s0];
+                            y.y *= ((const T *)d.tap)[s0 + d.tap_s];
+                        }
+                    } else {
+                        if (LIN == ST_CONTIG && (d.stream & 1))
+                            y = Buf<T>::template ldc<2>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
+                        else if (LIN == ST_CONTIG)
+                            y = Buf<T>::template ldc<kLdData>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
+                        else
+                            y = Buf<T>::template ldc<kLdData>(io.in, (io.voff_in[v] + (j + r * NBV) * io.ies) * ES, 0);
+                        if (TAP) {
+                            const T tv = ((const T *)d.tap)[(long)(j + r * NBV) * d.tap_s];
+                            y.x *= tv;
+                            y.y *= tv;
+                        }
+                        y.y *= io.isgn;
                     }
-                    y.y *= io.isgn;
                     x[i][r] = y;
                 }
             } else {
@@ -130,7 +154,11 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
                     x[i][r] = cmul(x[i][r], Buf<T>::template ldc<0>(io.tw, (toff + k + (r - 1) * NS) * ES, 0));
             }
             Dft<T, R>::run(x[i]);
-            if (LAST) {
+            if (LAST && REAL) {     // Y in natural order, for the combine
+                cpx<T> *dst = lds + v * LS;
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[lpad<T>(j + r * NBV)] = x[i][r];
+            } else if (LAST) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     cpx<T> y = x[i][r];
@@ -207,12 +235,16 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
         __syncthreads();
     }
     io.vbase = vbase;
-    if ((MIN & 1) == ST_CONTIG)
+    constexpr bool REAL = (MIN & ST_REAL) != 0;
+    constexpr int RS = (int)sizeof(T);
+    if ((MIN & 1) == ST_CONTIG && REAL)
+        io.in = make_rsrc((const T *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + 2 * N) * RS);
+    else if ((MIN & 1) == ST_CONTIG)
         io.in = make_rsrc((const cpx<T> *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + N) * ES);
     else
         io.in = make_rsrc(d.in.re, d.in.span);
     if ((MOUT & 1) == ST_CONTIG)
-        io.out = make_rsrc((cpx<T> *)d.out.re + vbase * io.ovs, ((io.nv - 1) * io.ovs + N) * ES);
+        io.out = make_rsrc((cpx<T> *)d.out.re + vbase * io.ovs, ((io.nv - 1) * io.ovs + N + (REAL ? 1 : 0)) * ES);
     else
         io.out = make_rsrc(d.out.re, d.out.span);
     io.tw = make_rsrc(d.tw, (long)N * ES);
@@ -220,6 +252,28 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
     io.osgn = d.conj_out ? (T)-1 : (T)1;
     io.scale = (T)d.scale;
     sstages<T, N, VPB, NT, MIN, MOUT, 0, SC>(slds, d, io);
+    if constexpr (REAL) {
+        // the n + 1 outputs of each vector: Z_k = (A + i^-1 W_k D) / 2 with
+        // A = Y_k + conj(Y_{n-k}), D = Y_k - conj(Y_{n-k}), W_k the Euler table
+        __syncthreads();
+        constexpr int PER = N + 1;
+        const cpx<T> *eu = (const cpx<T> *)d.euler;
+        const T half = (T)0.5 * io.scale;
+        for (int t = threadIdx.x; t < io.nv * PER; t += NT) {
+            int v, k;
+            if ((MOUT & 1) == ST_ACROSS) { v = t % io.nv; k = t / io.nv; }
+            else { v = t / PER; k = t - v * PER; }
+            const cpx<T> *y = slds + v * LS;
+            const cpx<T> F = y[lpad<T>(k == N ? 0 : k)];
+            const cpx<T> Bv = y[lpad<T>(k == 0 ? 0 : N - k)];
+            const cpx<T> Bc = {Bv.x, -Bv.y};
+            const cpx<T> A = F + Bc;
+            const cpx<T> WD = cmul(eu[k], F - Bc);
+            const cpx<T> z = {half * (A.x + WD.y), half * (A.y - WD.x)};
+            if ((MOUT & 1) == ST_CONTIG) Buf<T>::template stc<kStData>(io.out, (int)(v * io.ovs + k) * ES, 0, z);
+            else Buf<T>::template stc<kStData>(io.out, (io.voff_out[v] + k * io.oes) * ES, 0, z);
+        }
+    }
 }
 
 }  // namespace st

@@ -242,6 +242,7 @@ int vhip_rng_fill(const vhip_rng_desc *d, void *stream);
 #define VHIP_JIT_CONTIG 0
 #define VHIP_JIT_ACROSS 1
 #define VHIP_JIT_FUSED 2
+#define VHIP_JIT_REAL 4     /* load side: real input, real-to-complex transform */
 int vhip_jit_prepare(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in,
                      int layout_out);
 int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);

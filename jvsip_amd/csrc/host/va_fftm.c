@@ -203,8 +203,9 @@ static int va_jit_lds_ok(const VA_sched *s, const VA_launch *L, int prec)
     return (long)L->vpb * s->lds_stride * 2 * prec + (long)L->vpb * 12 <= lds_max;
 }
 
-/* Complex plans (FP32, FP64) whose passes the schedule-generic kernel would
- * run (every pass except the FP32 power-of-two kernel's contiguous rows)
+/* Complex and real-to-complex plans (FP32, FP64) whose passes the
+ * schedule-generic kernel would run (every pass except the FP32
+ * power-of-two kernels' contiguous rows)
  * and that move at least VSIP_AMD_JIT_MIN_ELEMS elements (default 2^18) get
  * their schedules compiled now, for the layouts the plan's views normally
  * have: contiguous rows (ROW), lanes across columns (COL), and the passes
@@ -215,8 +216,8 @@ static void va_jit_prepare(VA_fftm *p)
     const char *e = getenv("VSIP_AMD_JIT_MIN_ELEMS");
     long min_elems = e ? atol(e) : (1L << 18);
     long n = (long)p->MN;
-    int k;
-    if ((p->type != VA_CCFFTIP && p->type != VA_CCFFTOP) || p->staged) return;
+    int k, real = (p->type == VA_RCFFTOP) ? VHIP_JIT_REAL : 0;
+    if ((p->type != VA_CCFFTIP && p->type != VA_CCFFTOP && p->type != VA_RCFFTOP) || p->staged) return;
     if ((long)p->mN * n < min_elems) return;
     if (getenv("VSIP_AMD_JIT") && getenv("VSIP_AMD_JIT")[0] == '0') return;
     p->jit = 1;
@@ -224,13 +225,13 @@ static void va_jit_prepare(VA_fftm *p)
         const VA_sched *s = &p->pass[0];
         if (p->major == VSIP_ROW) {
             if (p->prec == 4 && (n & (n - 1)) == 0 && n >= 256 && n <= 16384 && !getenv("VSIP_AMD_NO_POW2"))
-                return;                                                   /* the power-of-two kernel */
+                return;                                                   /* the power-of-two kernels */
             if (va_jit_lds_ok(s, &s->along, p->prec))
-                vhip_jit_prepare(p->prec, s->n, s->nst, s->radix, s->along.vpb, s->along.threads, VHIP_JIT_CONTIG,
-                                 VHIP_JIT_CONTIG);
+                vhip_jit_prepare(p->prec, s->n, s->nst, s->radix, s->along.vpb, s->along.threads,
+                                 VHIP_JIT_CONTIG | real, VHIP_JIT_CONTIG);
         } else if (va_jit_lds_ok(s, &s->across, p->prec)) {
-            vhip_jit_prepare(p->prec, s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
-                             VHIP_JIT_ACROSS);
+            vhip_jit_prepare(p->prec, s->n, s->nst, s->radix, s->across.vpb, s->across.threads,
+                             VHIP_JIT_ACROSS | real, VHIP_JIT_ACROSS);
         }
         return;
     }
@@ -248,13 +249,13 @@ static void va_jit_prepare(VA_fftm *p)
 int va_jit_ready(const VA_fftm *p)
 {
     const VA_sched *s = &p->pass[0];
-    int k;
+    int k, real = (p->type == VA_RCFFTOP) ? VHIP_JIT_REAL : 0;
     if (p->npass == 1) {
         if (p->major == VSIP_ROW)
-            return vhip_jit_ready(p->prec, s->n, s->nst, s->radix, s->along.vpb, s->along.threads, VHIP_JIT_CONTIG,
-                                  VHIP_JIT_CONTIG);
-        return vhip_jit_ready(p->prec, s->n, s->nst, s->radix, s->across.vpb, s->across.threads, VHIP_JIT_ACROSS,
-                              VHIP_JIT_ACROSS);
+            return vhip_jit_ready(p->prec, s->n, s->nst, s->radix, s->along.vpb, s->along.threads,
+                                  VHIP_JIT_CONTIG | real, VHIP_JIT_CONTIG);
+        return vhip_jit_ready(p->prec, s->n, s->nst, s->radix, s->across.vpb, s->across.threads,
+                              VHIP_JIT_ACROSS | real, VHIP_JIT_ACROSS);
     }
     if (p->npass < 2) return 0;
     for (k = 0; k < p->npass; k++) {

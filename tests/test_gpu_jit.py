@@ -166,3 +166,32 @@ def test_compiled_schedule_fp64(amd, monkeypatch, m, n, major):
     g = fftm.ccfftm(amd, x.ravel(), v, major=major)
     monkeypatch.delenv("VSIP_AMD_JIT")
     _check("vs generic kernel", fftm.ccfftm(amd, x.ravel(), v, major=major), g, 1e-13)
+
+
+@pytest.mark.parametrize("m,n,major", [(300, 2000, VSIP_ROW), (2000, 300, VSIP_COL), (64, 2 * 2187, VSIP_ROW),
+                                       (1024, 8192, VSIP_COL)])
+def test_compiled_schedule_real_to_complex(amd, monkeypatch, m, n, major):
+    """rcfftmop plans compile the real-input schedule (pack on load, combine
+    in LDS); against the oracle (1e-5) and the generic kernel (1e-6)."""
+    monkeypatch.setenv("VSIP_AMD_JIT_MIN_ELEMS", "0")
+    x = datagen.real_data(102, (m, n))
+    if major == VSIP_ROW:
+        xv, zv, zshape = fftm.View(0, n, m, 1, n), fftm.View(0, n // 2 + 1, m, 1, n // 2 + 1), (m, n // 2 + 1)
+    else:
+        xv, zv, zshape = fftm.View(0, n, m, 1, n), fftm.View(0, n, m // 2 + 1, 1, n), (m // 2 + 1, n)
+    p = amd.rcfftmop_create_f(m, n, 1.0, major, 0, 1)
+    assert p, amd.last_error()
+    d = json.loads(amd.describe(p))
+    amd.fftm_destroy_f(p)
+    print("  passes", [(q["n"], q["radix"]) for q in d["passes"]], "compiled", d.get("compiled_schedule"),
+          flush=True)
+    pow2_row = major == VSIP_ROW and (n // 2) & (n // 2 - 1) == 0
+    assert d.get("compiled_schedule") is (not pow2_row)
+    _case("rcfftmop_f %s %dx%d (compiled schedule)" % ("ROW" if major == VSIP_ROW else "COL", m, n))
+    z0 = np.zeros(zshape[0] * zshape[1], np.complex64)
+    z = fftm.rcfftm(amd, x.ravel(), xv, z0, zv, major=major, scale=0.5)
+    _check("vs oracle", z, po.rcfftm(x.ravel(), xv, z0, zv, major, 0.5), 1e-5)
+    monkeypatch.setenv("VSIP_AMD_JIT", "0")
+    g = fftm.rcfftm(amd, x.ravel(), xv, z0, zv, major=major, scale=0.5)
+    monkeypatch.delenv("VSIP_AMD_JIT")
+    _check("vs generic kernel", z, g, 1e-6)

@@ -39,7 +39,7 @@ def test_planner_schedules_compile(lib, so, prec, n, nvec):
     assert s["rc"] == 0
     rad = (ctypes.c_int * len(s["radix"]))(*s["radix"])
     for shape, lin, lout in (("along", 0, 0), ("across", 1, 1), ("across", 0, 1), ("along", 2, 2),
-                             ("across", 3, 3)):
+                             ("across", 3, 3), ("along", 4, 0), ("across", 5, 1), ("across", 6, 1)):
         vpb, thr, _ = s[shape]
         size = ctypes.c_size_t()
         rc = so.vhip_jit_compile_check(prec, n, len(s["radix"]), rad, vpb, thr, lin, lout, b"gfx950",
@@ -52,5 +52,6 @@ def test_refuses_schedules_without_a_compiled_form(so):
     rad = (ctypes.c_int * 9)(*([2] * 9))
     assert so.vhip_jit_compile_check(4, 512, 9, rad, 1, 64, 0, 0, b"gfx950", None) != 0
     rad = (ctypes.c_int * 2)(4, 4)
-    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 4, 0, b"gfx950", None) != 0
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 8, 0, b"gfx950", None) != 0
     assert so.vhip_jit_compile_check(2, 16, 2, rad, 1, 64, 0, 0, b"gfx950", None) != 0
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 4, 2, b"gfx950", None) != 0   # real input + accumulate
