@@ -28,6 +28,30 @@ __global__ void __launch_bounds__(VPB * (1 << LOG2N) / 16)
     pow2_body<LOG2N, VPB, true, PLAIN>(d, per_wg, (cpx<float> *)smem);
 }
 
+// the pipelined 4096-point kernel: at most 2 workgroups per CU, so the
+// compiler may use up to 256 VGPRs
+template <bool PLAIN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    pow2_pipe_kernel(const vhip_fftm_desc d) {
+    __shared__ __attribute__((aligned(16))) cpx<float> lds[P2<12>::LDSV];
+    pow2_pipe_body<PLAIN>(d, lds);
+}
+
+// 4096-point rows on the pipelined kernel when every workgroup of a grid of
+// 2 per CU gets at least 8 rows (measured: 8 rows per workgroup beat one row
+// per workgroup by 7% at 4096 x 4096; fewer rows leave the pipeline short)
+int p2_pipe_launch(const vhip_fftm_desc *d, void *stream, int cus) {
+    static const bool off = getenv("VSIP_AMD_P2_PIPE") && atoi(getenv("VSIP_AMD_P2_PIPE")) == 0;
+    const long grid = 2L * cus;
+    if (off || d->nvec < 8 * grid || d->nvec > 0x7fffffffL) return -100;
+    if (d->in.vs0 < 4096 || d->out.vs0 < 4096) return -100;   // rows must not overlap
+    const bool plain = !d->conj_in && !d->conj_out && d->scale == 1.0;
+    hipStream_t st = (hipStream_t)stream;
+    if (plain) hipLaunchKernelGGL((pow2_pipe_kernel<true>), dim3((unsigned)grid), dim3(256), 0, st, *d);
+    else hipLaunchKernelGGL((pow2_pipe_kernel<false>), dim3((unsigned)grid), dim3(256), 0, st, *d);
+    return (int)hipGetLastError();
+}
+
 template <int LOG2N>
 int p2_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     constexpr int VPB = p2_vpb<LOG2N>();
@@ -102,7 +126,11 @@ int pow2_try_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     case 512: return p2_launch<9>(d, stream, cus);
     case 1024: return p2_launch<10>(d, stream, cus);
     case 2048: return p2_launch<11>(d, stream, cus);
-    case 4096: return p2_launch<12>(d, stream, cus);
+    case 4096: {
+        if (!p2_radices_match<12>(d)) return -100;
+        const int rc = p2_pipe_launch(d, stream, cus);
+        return rc == -100 ? p2_launch<12>(d, stream, cus) : rc;
+    }
     case 8192: return p2_launch<13>(d, stream, cus);
     case 16384: return p2_launch<14>(d, stream, cus);
     }

@@ -215,6 +215,117 @@ __device__ __forceinline__ void pow2_body(const vhip_fftm_desc &d, int per_wg, c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined persistent form of the 4096-point row kernel (radices 16/16/16,
+// 256 lanes, one row per pass).  Workgroup b of a grid of G transforms rows
+// b, b + G, b + 2G, ...; while it transforms one row the 16 loads of the
+// next are in flight, so HBM stays busy through the LDS stages.  Two things
+// keep the loads of the next row from stalling the current one:
+//   - no vector-memory access inside a row besides the row's own loads and
+//     stores: the stage twiddles of lane t, W_256^{r (t mod 16)} and
+//     W_4096^{rt} (r = 1..15), are loaded once into 60 registers (the 2
+//     workgroups per CU leave 256 per lane), ahead of the first row;
+//   - the prefetch is unconditional (rows past the end use a zero-length
+//     buffer range and read 0), so the wait before stage 0 always counts
+//     exactly the loads and stores issued after the row's own loads.
+// Measured on MI355X, 4096 x 4096 in place: 48.9-49.2 us vs 51.7-52.5 us
+// for one row per workgroup (2 workgroups per CU, 8 rows each).
+
+// stage 1 or 2 of the 4096-point row with the twiddles w[r-1] of this lane
+template <int S, bool PLAIN>
+__device__ __forceinline__ void p2p_stage(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> (&w)[15],
+                                          rsrc_t out, float osgn, float scale) {
+    constexpr int R = 16, NB = 256;
+    constexpr int LNS = 4 * S, NS = 1 << LNS;
+    constexpr bool LAST = (S == 2);
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const int j = t;
+    {
+        const cpx<float> *src = lds + padc(j);
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[r] = src[padc(r * NB)];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 1; r < R; ++r) x[r] = cmul(x[r], w[r - 1]);
+    Dft<float, R>::run(x);
+    if (LAST) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const cpx<float> y = x[r];
+            if (PLAIN) bstore<true>(out, j * 8, r * NB * 8, y);
+            else bstore<true>(out, j * 8, r * NB * 8, {y.x * scale, y.y * (scale * osgn)});
+        }
+    } else {
+        const int base = ((j >> LNS) << (LNS + 4)) + (j & (NS - 1));
+        cpx<float> *dst = lds + padc(base);
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[padc(r * NS)] = x[r];
+        __syncthreads();
+    }
+}
+
+// one 4096-point row held in x (stage 0 from registers, stages 1-2 as above)
+template <bool PLAIN>
+__device__ __forceinline__ void p2p_row(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> (&w1)[15],
+                                        const cpx<float> (&w2)[15], rsrc_t out, float isgn, float osgn,
+                                        float scale) {
+    // the input conjugation is applied here rather than at the load, which
+    // would make the prefetch of the next row wait for its data
+    if (!PLAIN) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) x[e].y *= isgn;
+    }
+    // stage 0 has no twiddles: the table arguments are unused
+    p2_stage<12, 1, 0, PLAIN, false, true>(x, lds, make_rsrc(nullptr, 0u), nullptr, out, 0, osgn, scale);
+    p2p_stage<1, PLAIN>(x, lds, w1, out, osgn, scale);
+    p2p_stage<2, PLAIN>(x, lds, w2, out, osgn, scale);
+}
+
+// lds: P2<12>::LDSV complex elements
+template <bool PLAIN>
+__device__ __forceinline__ void pow2_pipe_body(const vhip_fftm_desc &d, cpx<float> *lds) {
+    constexpr int N = 4096;
+    const cpx<float> *tw = (const cpx<float> *)d.tw;
+    const cpx<float> *in = (const cpx<float> *)d.in.re;
+    cpx<float> *out = (cpx<float> *)d.out.re;
+    const int t = threadIdx.x;
+    const long ivs = d.in.vs0, ovs = d.out.vs0;
+    const float isgn = d.conj_in ? -1.f : 1.f;
+    const float osgn = d.conj_out ? -1.f : 1.f;
+    const float scale = (float)d.scale;
+    const int G = gridDim.x;
+    const int nvec = __builtin_amdgcn_readfirstlane((int)d.nvec);
+    int g = blockIdx.x;
+    if (g >= nvec) return;
+    auto rin = [&](int row) {
+        return make_rsrc(in + (long)(row < nvec ? row : 0) * ivs, row < nvec ? (unsigned)(N * 8) : 0u);
+    };
+    // twiddles first: loads return in order, and the row waits for both
+    cpx<float> w1[15], w2[15];
+#pragma unroll
+    for (int r = 1; r < 16; ++r) {
+        w1[r - 1] = tw[d.tw_off[1] + (r - 1) * 16 + (t & 15)];
+        w2[r - 1] = tw[d.tw_off[2] + (r - 1) * 256 + t];
+    }
+    cpx<float> a[16], b[16];
+    p2_load<12, 1, true, true>(a, rin(g), ivs, 1.f);
+    // vmcnt(0): the first row and the twiddles are in, so no load is pending
+    // where the loop below starts (its waits then count only its own loads)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    for (;;) {
+        p2_load<12, 1, true, true>(b, rin(g + G), ivs, 1.f);
+        p2p_row<PLAIN>(a, lds, w1, w2, make_rsrc(out + (long)g * ovs, (unsigned)(N * 8)), isgn, osgn, scale);
+        g += G;
+        if (g >= nvec) break;
+        p2_load<12, 1, true, true>(a, rin(g + G), ivs, 1.f);
+        p2p_row<PLAIN>(b, lds, w1, w2, make_rsrc(out + (long)g * ovs, (unsigned)(N * 8)), isgn, osgn, scale);
+        g += G;
+        if (g >= nvec) break;
+    }
+}
+
 // buffer range of `len` complex elements per row for the group at row g*VPB
 template <int VPB>
 __device__ __forceinline__ unsigned group_bytes_len(long nvec, long g, long vs, long len) {

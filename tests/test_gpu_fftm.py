@@ -69,6 +69,35 @@ def test_ccfftmip_dir_scale(amd, ref, direction, scale):
     _check("vs jvsip", y, fftm.ccfftm(ref, x.ravel(), v, direction=direction, scale=scale), TOL_F)
 
 
+def _np_rows(x, direction, scale):
+    x = x.astype(np.complex128)
+    n = x.shape[1]
+    return (np.fft.fft(x, axis=1) if direction < 0 else np.fft.ifft(x, axis=1) * n) * scale
+
+
+@pytest.mark.parametrize("m", [4100, 4096 + 3 * 512 + 7])
+def test_pipelined_4096_rows(amd, m):
+    """4096-point rows, at least 8 per workgroup of a grid of 2 per CU
+    (256 CUs), take the pipelined persistent kernel (rows b, b + G, ...);
+    the uneven row counts give some workgroups one row more, and the
+    prefetch past the last row reads nothing.  Forward, inverse with scale,
+    and out of place from rows 2N apart, against numpy's FP64 FFT."""
+    n = 4096
+    x = datagen.complex_data(140, (m, n), dist="uniform")
+    v = fftm.dense_view(m, n)
+    for direction, scale in ((VSIP_FFT_FWD, 1.0), (VSIP_FFT_INV, 1.0 / n)):
+        _case("ccfftmip_f ROW %dx%d dir %+d (pipelined rows)" % (m, n, direction))
+        y = fftm.ccfftm(amd, x.ravel(), v, direction=direction, scale=scale).reshape(m, n)
+        _check("vs numpy fp64", y, _np_rows(x, direction, scale), TOL_F)
+    _case("ccfftmop_f ROW %dx%d from a 2N-wide parent (pipelined rows)" % (m, n))
+    xp = np.zeros((m, 2 * n), np.complex64)
+    xp[:, :n] = x
+    xv = fftm.View(0, 2 * n, m, 1, n)
+    xr, y = fftm.ccfftm(amd, xp.ravel(), xv, out_blk=np.zeros(m * n, np.complex64), out_view=v)
+    np.testing.assert_array_equal(xr, xp.ravel())
+    _check("vs numpy fp64", y.reshape(m, n), _np_rows(x, VSIP_FFT_FWD, 1.0), TOL_F)
+
+
 def test_fwd_inv_roundtrip(amd):
     _case("ccfftmip_f ROW 128x4096 fwd then inv 1/N")
     x = datagen.complex_data(2, (128, 4096), dist="uniform")
