@@ -671,7 +671,7 @@ kernel_fn fftm_kernel_f32(int kind);
 kernel_fn fftm_kernel_d8(int kind);
 int pow2_try_launch(const vhip_fftm_desc *d, void *stream, int cus);
 int pow2_set_lds_limit(int bytes);
-int pow2_real_try_launch(const vhip_fftm_desc *d, void *stream);
+int pow2_real_try_launch(const vhip_fftm_desc *d, void *stream, int cus);
 int pow2_real_set_lds_limit(int bytes);
 int batch_set_lds_limit(int bytes);
 

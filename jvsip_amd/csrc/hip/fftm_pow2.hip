@@ -115,7 +115,7 @@ int p2_set_lds(int bytes) {
 // does not (the caller then uses the generic kernel).
 int pow2_try_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     if (d->prec != 4 || d->ptw || d->tap || d->acc) return -100;
-    if (d->kind != VHIP_C2C) return pow2_real_try_launch(d, stream);
+    if (d->kind != VHIP_C2C) return pow2_real_try_launch(d, stream, cus);
     if (!d->in.contig || !d->out.contig || d->in.split || d->out.split) return -100;
     if (d->map_in != VHIP_MAP_ALONG || d->map_out != VHIP_MAP_ALONG) return -100;
     if (d->in.nv0 < d->nvec || d->out.nv0 < d->nvec) return -100;   // single-level vectors only

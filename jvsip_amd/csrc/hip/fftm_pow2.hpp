@@ -406,18 +406,206 @@ __device__ __forceinline__ void pow2_real_body(const vhip_fftm_desc &d, cpx<floa
     if constexpr (P::NST > 2) p2_stage<LOG2N, VPB, (P::NST > 2 ? 2 : 0), false, TOL, KIND == VHIP_C2R>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
     if constexpr (P::NST > 3) p2_stage<LOG2N, VPB, (P::NST > 3 ? 3 : 0), false, TOL, KIND == VHIP_C2R>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
     if (KIND == VHIP_R2C) {
-        // the n + 1 outputs of each row from Y in LDS (natural order)
+        // The n + 1 outputs of each row from Y in LDS (natural order), in
+        // mirror pairs: item m < n/2 of row v reads Y_m and Y_{n-m} once and
+        // stores z_m and z_{n-m} (m = 0 gives z_0 and z_n, both from Y_0,
+        // and also z_{n/2}).  VPB n/2 items per group = 8 per lane, so the
+        // item index needs no division.  Rows past nv are skipped (with a
+        // single row the launch accepts any row stride, so their offsets may
+        // fall inside the store range).
         const float half = 0.5f * scale;
-        for (int t = threadIdx.x; t < nv * (N + 1); t += T) {
-            const int v = t / (N + 1), k = t - v * (N + 1);
-            const cpx<float> *y = lds + v * P::LDSV;
-            const cpx<float> F = y[padc(k == N ? 0 : k)];
-            const cpx<float> B2 = y[padc(k == 0 ? 0 : N - k)];
+        const rsrc_t ro = make_rsrc(out, group_bytes_len<VPB>(d.nvec, g, ovs, N + 1));
+        auto z = [&](const cpx<float> F, const cpx<float> B2, int k) {
             const cpx<float> Bc = {B2.x, -B2.y};
             const cpx<float> A = F + Bc;
             const cpx<float> WD = cmul(eu[k], F - Bc);
-            out[v * ovs + k] = {half * (A.x + WD.y), half * (A.y - WD.x)};
+            return cpx<float>{half * (A.x + WD.y), half * (A.y - WD.x)};
+        };
+        static_assert((VPB * (N / 2)) % T == 0, "mirror items must divide over the lanes");
+#pragma unroll
+        for (int i = 0; i < VPB * (N / 2) / T; ++i) {
+            const int q = threadIdx.x + i * T;
+            const int v = q / (N / 2), m = q & (N / 2 - 1);
+            if (v >= nv) continue;
+            const cpx<float> *y = lds + v * P::LDSV;
+            const cpx<float> F = y[padc(m)];
+            const cpx<float> B2 = y[padc(m == 0 ? 0 : N - m)];
+            const int vo = (int)(v * ovs) * 8;
+            bstore(ro, vo + m * 8, 0, z(F, B2, m));
+            bstore(ro, vo + (N - m) * 8, 0, z(B2, F, N - m));
+            if (m == 0) {
+                const cpx<float> H = y[padc(N / 2)];
+                bstore(ro, vo + (N / 2) * 8, 0, z(H, H, N / 2));
+            }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined persistent form of the R2C real-row kernel (the same arithmetic
+// as pow2_real_body<LOG2N, VPB, VHIP_R2C>, VPB n = 4096, 256 lanes):
+// workgroup b of a grid of G transforms row groups b, b + G, ... with the
+// loads of its next group in flight, as the 4096-point pipelined kernel does.
+// What keeps vector memory out of the group's own work:
+//   - every stage's twiddles are the same for all butterflies of a lane
+//     (k = t mod NS, as 256 is a multiple of every NS here), loaded once into
+//     registers; so are the Euler factors of the lane's mirror items;
+//   - every store of the combine is issued by every lane; a lane with
+//     nothing to write there stores at an offset past the buffer range,
+//     which the hardware drops, so no store is conditional and the wait
+//     before the next group's stage 0 counts a fixed number of accesses.
+
+// buffer offset no range here reaches (ranges stay below 2 GiB)
+constexpr int kDropOffset = (int)0x80000000u;
+
+// stage S >= 1 with the lane's twiddles w[r-1] (the same for its B butterflies)
+template <int LOG2N, int VPB, int S>
+__device__ __forceinline__ void p2r_stage(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> *w) {
+    using P = P2<LOG2N>;
+    constexpr int N = P::N;
+    constexpr int LR = P::bits(S);
+    constexpr int R = 1 << LR;
+    constexpr int B = 16 / R;
+    constexpr int LNB = LOG2N - LR;
+    constexpr int NB = 1 << LNB;
+    constexpr int LNS = P::log2ns(S);
+    constexpr int NS = 1 << LNS;
+    constexpr int T = VPB * N / 16;
+    static_assert(T % NS == 0, "a lane's twiddles must not depend on its butterfly");
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        const int b = t + i * T;
+        const int v = b >> LNB, j = b & (NB - 1);
+        const cpx<float> *src = lds + v * P::LDSV + padc(j);
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[i * R + r] = src[padc(r * NB)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        const int b = t + i * T;
+        const int v = b >> LNB, j = b & (NB - 1);
+        const int k = j & (NS - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) x[i * R + r] = cmul(x[i * R + r], w[r - 1]);
+        Dft<float, R>::run(&x[i * R]);
+        const int base = ((j >> LNS) << (LNS + LR)) + k;
+        cpx<float> *dst = lds + v * P::LDSV + padc(base);
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[padc(r * NS)] = x[i * R + r];
+    }
+    __syncthreads();
+}
+
+template <int LOG2N>
+struct P2R {
+    static constexpr int N = 1 << LOG2N;
+    static constexpr int VPB = p2_vpb<LOG2N>();
+    static constexpr int T = VPB * N / 16;
+    static constexpr int NST = P2<LOG2N>::NST;
+    static constexpr int NIT = VPB * (N / 2) / T;                      // mirror items per lane
+    static constexpr int NE = (N / 2) > T ? (N / 2) / T : 1;            // distinct m per lane
+    static_assert(NST <= 3 && NIT == 8 && NE <= NIT, "shape of the pipelined R2C kernel");
+};
+
+// one group: the row transforms from x (stage 0 from registers), the
+// combine from LDS, the stores to the group's output range `ro`
+template <int LOG2N>
+__device__ __forceinline__ void p2r_group(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> (&w)[2][15],
+                                          const cpx<float> (&em)[P2R<LOG2N>::NE], const cpx<float> (&en)[P2R<LOG2N>::NE],
+                                          cpx<float> eh, rsrc_t ro, long ovs, int nv, float half) {
+    using P = P2<LOG2N>;
+    using Q = P2R<LOG2N>;
+    constexpr int N = Q::N, VPB = Q::VPB, T = Q::T;
+    p2_stage<LOG2N, VPB, 0, true, true>(x, lds, make_rsrc(nullptr, 0u), nullptr, ro, 0, 1.f, 1.f);
+    if constexpr (Q::NST > 1) p2r_stage<LOG2N, VPB, (Q::NST > 1 ? 1 : 0)>(x, lds, w[0]);
+    if constexpr (Q::NST > 2) p2r_stage<LOG2N, VPB, (Q::NST > 2 ? 2 : 0)>(x, lds, w[1]);
+    auto z = [&](const cpx<float> F, const cpx<float> B2, cpx<float> e) {
+        const cpx<float> Bc = {B2.x, -B2.y};
+        const cpx<float> A = F + Bc;
+        const cpx<float> WD = cmul(e, F - Bc);
+        return cpx<float>{half * (A.x + WD.y), half * (A.y - WD.x)};
+    };
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < Q::NIT; ++i) {
+        const int q = t + i * T;
+        const int v = q / (N / 2), m = q & (N / 2 - 1);
+        const cpx<float> *y = lds + v * P::LDSV;
+        const cpx<float> F = y[padc(m)];
+        const cpx<float> B2 = y[padc(m == 0 ? 0 : N - m)];
+        const int vo = (int)(v * ovs) * 8;
+        const bool live = v < nv;
+        bstore(ro, live ? vo + m * 8 : kDropOffset, 0, z(F, B2, em[i % Q::NE]));
+        bstore(ro, live ? vo + (N - m) * 8 : kDropOffset, 0, z(B2, F, en[i % Q::NE]));
+    }
+    // z_{n/2} of row t, for the first VPB lanes
+    {
+        const int v = t < VPB ? t : 0;
+        const cpx<float> H = lds[v * P::LDSV + padc(N / 2)];
+        bstore(ro, (t < VPB && t < nv) ? (int)(v * ovs) * 8 + (N / 2) * 8 : kDropOffset, 0, z(H, H, eh));
+    }
+    __syncthreads();   // the next group's stage 0 overwrites the LDS rows
+}
+
+template <int LOG2N>
+__device__ __forceinline__ void pow2_r2c_pipe_body(const vhip_fftm_desc &d, cpx<float> *lds) {
+    using P = P2<LOG2N>;
+    using Q = P2R<LOG2N>;
+    constexpr int N = Q::N, VPB = Q::VPB, T = Q::T;
+    const cpx<float> *tw = (const cpx<float> *)d.tw;
+    const cpx<float> *eu = (const cpx<float> *)d.euler;
+    const cpx<float> *in = (const cpx<float> *)d.in.re;
+    cpx<float> *out = (cpx<float> *)d.out.re;
+    const long ivs = d.in.vs0 / 2, ovs = d.out.vs0;   // the real side's stride is in reals
+    const float half = 0.5f * (float)d.scale;
+    const int G = gridDim.x;
+    const int ngroups = __builtin_amdgcn_readfirstlane((int)((d.nvec + VPB - 1) / VPB));
+    const long nvec = d.nvec;
+    int g = blockIdx.x;
+    if (g >= ngroups) return;
+    const int t = threadIdx.x;
+    // per-lane constants first (loads return in order)
+    cpx<float> w[2][15];
+#pragma unroll
+    for (int s = 1; s < Q::NST; ++s) {
+        const int R = 1 << P::bits(s), NS = 1 << P::log2ns(s);
+#pragma unroll
+        for (int r = 1; r < 16; ++r)
+            if (r < R) w[s - 1][r - 1] = tw[d.tw_off[s] + (r - 1) * NS + (t & (NS - 1))];
+    }
+    cpx<float> em[Q::NE], en[Q::NE];
+#pragma unroll
+    for (int i = 0; i < Q::NE; ++i) {
+        const int m = (t + i * T) & (N / 2 - 1);
+        em[i] = eu[m];
+        en[i] = eu[N - m];
+    }
+    const cpx<float> eh = eu[N / 2];
+    auto rin = [&](int gi) {
+        return make_rsrc(in + (long)(gi < ngroups ? gi : 0) * VPB * ivs,
+                         gi < ngroups ? group_bytes<LOG2N, VPB>(nvec, gi, ivs) : 0u);
+    };
+    auto rout = [&](int gi) { return make_rsrc(out + (long)gi * VPB * ovs, group_bytes_len<VPB>(nvec, gi, ovs, N + 1)); };
+    auto nvg = [&](int gi) { const long r = nvec - (long)gi * VPB; return r > VPB ? VPB : (int)r; };
+    cpx<float> a[16], b[16];
+    p2_load<LOG2N, VPB, true>(a, rin(g), ivs, 1.f);
+    p2_load<LOG2N, VPB, true>(b, rin(g + G), ivs, 1.f);
+    p2r_group<LOG2N>(a, lds, w, em, en, eh, rout(g), ovs, nvg(g), half);
+    g += G;
+    if (g >= ngroups) return;
+    p2_load<LOG2N, VPB, true>(a, rin(g + G), ivs, 1.f);
+    for (;;) {
+        p2r_group<LOG2N>(b, lds, w, em, en, eh, rout(g), ovs, nvg(g), half);
+        g += G;
+        if (g >= ngroups) break;
+        p2_load<LOG2N, VPB, true>(b, rin(g + G), ivs, 1.f);
+        p2r_group<LOG2N>(a, lds, w, em, en, eh, rout(g), ovs, nvg(g), half);
+        g += G;
+        if (g >= ngroups) break;
+        p2_load<LOG2N, VPB, true>(a, rin(g + G), ivs, 1.f);
     }
 }
 

@@ -239,6 +239,27 @@ def test_real_pow2_rows(amd, ref, m, n):
     _check("round trip", back, x.ravel(), 1e-6)
 
 
+@pytest.mark.parametrize("m,parent", [(8195, 1), (8192 + 4 * 512 + 1, 2)])
+def test_pipelined_r2c_rows(amd, m, parent):
+    """rcfftmop on rows of 2048 reals with at least 4 groups of 4 rows per
+    workgroup of a grid of 2 per CU (256 CUs) takes the pipelined R2C
+    kernel; the row counts leave a partial last group and uneven groups
+    per workgroup; parent 2 puts the complex output rows 2(n/2+1) apart.
+    Against numpy's FP64 rfft."""
+    n = 2048
+    h = n // 2 + 1
+    _case("rcfftmop_f ROW %dx%d, output parent x%d (pipelined R2C)" % (m, n, parent))
+    x = datagen.real_data(141, (m, n))
+    xv = fftm.dense_view(m, n)
+    zv = fftm.View(0, parent * h, m, 1, h)
+    z0 = np.zeros(m * parent * h, np.complex64)
+    z = fftm.rcfftm(amd, x.ravel(), xv, z0, zv, scale=0.25).reshape(m, parent * h)
+    want = np.fft.rfft(x.astype(np.float64), axis=1) * 0.25
+    _check("vs numpy fp64", z[:, :h], want, TOL_F)
+    if parent > 1:
+        assert not np.any(z[:, h:]), "stores outside the output rows"
+
+
 def test_real_pow2_strided_rows(amd):
     """Real rows 2x apart in their parents (rows of a 2N-wide real matrix,
     complex rows of a 2(N/2+1)-wide one) on the power-of-two real kernel."""
