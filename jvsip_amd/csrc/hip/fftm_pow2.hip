@@ -52,6 +52,30 @@ int p2_pipe_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     return (int)hipGetLastError();
 }
 
+// the pipelined kernel for rows of 256..2048 (groups of 4096 / N rows)
+template <int LOG2N, bool PLAIN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    pow2_pipe_gen_kernel(const vhip_fftm_desc d) {
+    __shared__ __attribute__((aligned(16))) cpx<float> lds[p2_vpb<LOG2N>() * P2<LOG2N>::LDSV];
+    pow2_pipe_gen_body<LOG2N, PLAIN>(d, lds);
+}
+
+// row groups on the pipelined kernel when every workgroup of a grid of 2 per
+// CU gets at least 8 groups (the caller has checked radices, strides, spans)
+template <int LOG2N>
+int p2_pipe_gen_launch(const vhip_fftm_desc *d, void *stream, int cus) {
+    static const bool off = getenv("VSIP_AMD_P2_PIPE") && atoi(getenv("VSIP_AMD_P2_PIPE")) == 0;
+    constexpr int VPB = p2_vpb<LOG2N>();
+    const long grid = 2L * cus;
+    const long ngroups = (d->nvec + VPB - 1) / VPB;
+    if (off || ngroups < 8 * grid || ngroups > 0x7fffffffL) return -100;
+    const bool plain = !d->conj_in && !d->conj_out && d->scale == 1.0;
+    hipStream_t st = (hipStream_t)stream;
+    if (plain) hipLaunchKernelGGL((pow2_pipe_gen_kernel<LOG2N, true>), dim3((unsigned)grid), dim3(256), 0, st, *d);
+    else hipLaunchKernelGGL((pow2_pipe_gen_kernel<LOG2N, false>), dim3((unsigned)grid), dim3(256), 0, st, *d);
+    return (int)hipGetLastError();
+}
+
 template <int LOG2N>
 int p2_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     constexpr int VPB = p2_vpb<LOG2N>();
@@ -62,6 +86,10 @@ int p2_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     const long span_out = ((long)(VPB - 1) * d->out.vs0 + (1 << LOG2N)) * 8;
     if (d->in.vs0 < (1 << LOG2N) || d->out.vs0 < (1 << LOG2N)) return -100;   // rows must not overlap
     if (span_in >= (1L << 31) || span_out >= (1L << 31)) return -100;
+    if constexpr (LOG2N >= 8 && LOG2N <= 11) {
+        const int rc = p2_pipe_gen_launch<LOG2N>(d, stream, cus);
+        if (rc != -100) return rc;
+    }
     const long ngroups = (d->nvec + VPB - 1) / VPB;
     const size_t lds = (size_t)VPB * P2<LOG2N>::LDSV * 8;
     static const int venv = getenv("VSIP_AMD_P2_VARIANT") ? atoi(getenv("VSIP_AMD_P2_VARIANT")) : 1;

@@ -609,6 +609,124 @@ __device__ __forceinline__ void pow2_r2c_pipe_body(const vhip_fftm_desc &d, cpx<
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined persistent c2c kernel for the shorter power-of-two rows
+// (LOG2N 8..11: groups of VPB = 4096 / N rows, 256 lanes), the design of
+// the 4096-point one: workgroup b transforms groups b, b + G, ... with the
+// next group's loads in flight; stage twiddles per lane in registers; the
+// input conjugation applied at use.
+
+// the last stage of a c2c group, stored to HBM (streamed)
+template <int LOG2N, int VPB, int S, bool PLAIN>
+__device__ __forceinline__ void p2g_last(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> *w, rsrc_t out,
+                                         long out_vs, float osgn, float scale) {
+    using P = P2<LOG2N>;
+    constexpr int N = P::N;
+    constexpr int LR = P::bits(S);
+    constexpr int R = 1 << LR;
+    constexpr int B = 16 / R;
+    constexpr int LNB = LOG2N - LR;
+    constexpr int NB = 1 << LNB;
+    constexpr int NS = 1 << P::log2ns(S);
+    constexpr int T = VPB * N / 16;
+    static_assert(S == P::NST - 1 && T % NS == 0, "last stage with per-lane twiddles");
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        const int b = t + i * T;
+        const int v = b >> LNB, j = b & (NB - 1);
+        const cpx<float> *src = lds + v * P::LDSV + padc(j);
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[i * R + r] = src[padc(r * NB)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        const int b = t + i * T;
+        const int v = b >> LNB, j = b & (NB - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) x[i * R + r] = cmul(x[i * R + r], w[r - 1]);
+        Dft<float, R>::run(&x[i * R]);
+        const int vo = (int)(v * out_vs + j) * 8;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const cpx<float> y = x[i * R + r];
+            if (PLAIN) bstore<true>(out, vo, r * NB * 8, y);
+            else bstore<true>(out, vo, r * NB * 8, {y.x * scale, y.y * (scale * osgn)});
+        }
+    }
+}
+
+template <int LOG2N, bool PLAIN>
+__device__ __forceinline__ void p2g_group(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> (&w)[2][15],
+                                          rsrc_t out, long ovs, float isgn, float osgn, float scale) {
+    using P = P2<LOG2N>;
+    constexpr int VPB = p2_vpb<LOG2N>();
+    static_assert(P::NST == 2 || P::NST == 3, "two or three stages");
+    if (!PLAIN) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) x[e].y *= isgn;
+    }
+    p2_stage<LOG2N, VPB, 0, PLAIN, false, true>(x, lds, make_rsrc(nullptr, 0u), nullptr, out, ovs, osgn, scale);
+    if constexpr (P::NST == 3) {
+        p2r_stage<LOG2N, VPB, 1>(x, lds, w[0]);
+        p2g_last<LOG2N, VPB, 2, PLAIN>(x, lds, w[1], out, ovs, osgn, scale);
+    } else {
+        p2g_last<LOG2N, VPB, 1, PLAIN>(x, lds, w[0], out, ovs, osgn, scale);
+    }
+}
+
+template <int LOG2N, bool PLAIN>
+__device__ __forceinline__ void pow2_pipe_gen_body(const vhip_fftm_desc &d, cpx<float> *lds) {
+    using P = P2<LOG2N>;
+    constexpr int VPB = p2_vpb<LOG2N>();
+    static_assert(LOG2N >= 8 && LOG2N <= 11 && VPB * P::N == 4096, "256-lane groups");
+    const cpx<float> *tw = (const cpx<float> *)d.tw;
+    const cpx<float> *in = (const cpx<float> *)d.in.re;
+    cpx<float> *out = (cpx<float> *)d.out.re;
+    const long ivs = d.in.vs0, ovs = d.out.vs0;
+    const float isgn = d.conj_in ? -1.f : 1.f;
+    const float osgn = d.conj_out ? -1.f : 1.f;
+    const float scale = (float)d.scale;
+    const int G = gridDim.x;
+    const long nvec = d.nvec;
+    const int ngroups = __builtin_amdgcn_readfirstlane((int)((nvec + VPB - 1) / VPB));
+    int g = blockIdx.x;
+    if (g >= ngroups) return;
+    const int t = threadIdx.x;
+    cpx<float> w[2][15];
+#pragma unroll
+    for (int s = 1; s < P::NST; ++s) {
+        const int R = 1 << P::bits(s), NS = 1 << P::log2ns(s);
+#pragma unroll
+        for (int r = 1; r < 16; ++r)
+            if (r < R) w[s - 1][r - 1] = tw[d.tw_off[s] + (r - 1) * NS + (t & (NS - 1))];
+    }
+    auto rin = [&](int gi) {
+        return make_rsrc(in + (long)(gi < ngroups ? gi : 0) * VPB * ivs,
+                         gi < ngroups ? group_bytes<LOG2N, VPB>(nvec, gi, ivs) : 0u);
+    };
+    auto rout = [&](int gi) { return make_rsrc(out + (long)gi * VPB * ovs, group_bytes<LOG2N, VPB>(nvec, gi, ovs)); };
+    cpx<float> a[16], b[16];
+    p2_load<LOG2N, VPB, true, true>(a, rin(g), ivs, 1.f);
+    p2_load<LOG2N, VPB, true, true>(b, rin(g + G), ivs, 1.f);
+    p2g_group<LOG2N, PLAIN>(a, lds, w, rout(g), ovs, isgn, osgn, scale);
+    g += G;
+    if (g >= ngroups) return;
+    p2_load<LOG2N, VPB, true, true>(a, rin(g + G), ivs, 1.f);
+    for (;;) {
+        p2g_group<LOG2N, PLAIN>(b, lds, w, rout(g), ovs, isgn, osgn, scale);
+        g += G;
+        if (g >= ngroups) break;
+        p2_load<LOG2N, VPB, true, true>(b, rin(g + G), ivs, 1.f);
+        p2g_group<LOG2N, PLAIN>(a, lds, w, rout(g), ovs, isgn, osgn, scale);
+        g += G;
+        if (g >= ngroups) break;
+        p2_load<LOG2N, VPB, true, true>(a, rin(g + G), ivs, 1.f);
+    }
+}
+
 // One work item of the batched kernel: VPB = 8192 / N rows starting at
 // vector vbase of the transform d, with the batched workgroup of 512 lanes.
 template <int LOG2N>

@@ -98,6 +98,22 @@ def test_pipelined_4096_rows(amd, m):
     _check("vs numpy fp64", y.reshape(m, n), _np_rows(x, VSIP_FFT_FWD, 1.0), TOL_F)
 
 
+@pytest.mark.parametrize("m,n,direction", [(65536 + 5, 256, VSIP_FFT_FWD), (32768 + 3, 512, VSIP_FFT_INV),
+                                           (16384 + 1, 1024, VSIP_FFT_FWD), (16384 + 1, 1024, VSIP_FFT_INV),
+                                           (8192 + 1, 2048, VSIP_FFT_FWD)])
+def test_pipelined_short_rows(amd, m, n, direction):
+    """Rows of 256..2048 in groups of 4096 / n, at least 8 groups per
+    workgroup of a grid of 2 per CU (256 CUs), take the pipelined kernel;
+    the row counts leave a partial last group.  Inverse cases carry a
+    scale.  Against numpy's FP64 FFT."""
+    scale = 1.0 if direction == VSIP_FFT_FWD else 1.0 / n
+    _case("ccfftmip_f ROW %dx%d dir %+d (pipelined groups)" % (m, n, direction))
+    x = datagen.complex_data(142, (m, n), dist="uniform")
+    v = fftm.dense_view(m, n)
+    y = fftm.ccfftm(amd, x.ravel(), v, direction=direction, scale=scale).reshape(m, n)
+    _check("vs numpy fp64", y, _np_rows(x, direction, scale), TOL_F)
+
+
 def test_fwd_inv_roundtrip(amd):
     _case("ccfftmip_f ROW 128x4096 fwd then inv 1/N")
     x = datagen.complex_data(2, (128, 4096), dist="uniform")
