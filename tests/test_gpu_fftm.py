@@ -256,12 +256,13 @@ def test_real_pow2_rows(amd, ref, m, n):
 
 
 @pytest.mark.parametrize("m,parent", [(8195, 1), (8192 + 4 * 512 + 1, 2)])
-def test_pipelined_r2c_rows(amd, m, parent):
+def test_pipelined_real_rows(amd, m, parent):
     """rcfftmop on rows of 2048 reals with at least 4 groups of 4 rows per
     workgroup of a grid of 2 per CU (256 CUs) takes the pipelined R2C
-    kernel; the row counts leave a partial last group and uneven groups
-    per workgroup; parent 2 puts the complex output rows 2(n/2+1) apart.
-    Against numpy's FP64 rfft."""
+    kernel; the row counts leave a partial last group and uneven groups per
+    workgroup; parent 2 puts the complex rows 2(n/2+1) apart.  R2C against
+    numpy's FP64 rfft; crfftmop (one group per workgroup) of a
+    non-Hermitian input against the oracle, and the round trip."""
     n = 2048
     h = n // 2 + 1
     _case("rcfftmop_f ROW %dx%d, output parent x%d (pipelined R2C)" % (m, n, parent))
@@ -274,6 +275,13 @@ def test_pipelined_r2c_rows(amd, m, parent):
     _check("vs numpy fp64", z[:, :h], want, TOL_F)
     if parent > 1:
         assert not np.any(z[:, h:]), "stores outside the output rows"
+    _case("crfftmop_f ROW %dx%d, input parent x%d" % (m, n, parent))
+    zz = datagen.complex_data(143, (m * parent * h,))       # not Hermitian
+    x0 = np.zeros(m * n, np.float32)
+    xb = fftm.crfftm(amd, zz, zv, x0, xv, scale=1.0 / n)
+    _check("vs oracle", xb, po.crfftm(zz, zv, x0, xv, scale=1.0 / n), TOL_F)
+    back = fftm.crfftm(amd, (4 * z).ravel(), zv, x0, xv, scale=1.0 / n)
+    _check("round trip", back, x.ravel(), 1e-6)
 
 
 def test_real_pow2_strided_rows(amd):
