@@ -18,6 +18,10 @@
 //   - a workgroup runs `per_wg` consecutive groups of VPB rows (persistent) and
 //     may issue the HBM loads of its next group before transforming the
 //     current one.
+// For large batches the pipelined forms further down (pow2_pipe_body,
+// pow2_pipe_gen_body, pow2_r2c_pipe_body) keep the next group's loads in
+// flight through the whole transform of the current one, with no other
+// vector-memory access in between.
 #pragma once
 #include <cstdlib>
 #include "fftm_core.hpp"
