@@ -23,9 +23,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 }
 
 // Real rows (R2C / C2R) of half length n = 2^LOG2N, one group of rows per
-// workgroup; R2C with half length 1024 and at least 4 groups per workgroup
-// of a grid of 2 per CU on the pipelined kernel instead (measured at 8192
-// rows of 2048 reals; a pipelined C2R, whose 32 loads per lane per group
+// workgroup; R2C with half length 256..4096 and at least 4 groups per
+// workgroup of a grid of 2 per CU on the pipelined kernel instead (measured
+// at 8192 rows of 2048 reals and the other lengths; a pipelined C2R, whose 32 loads per lane per group
 // plus the prefetch exceed the 63 accesses a wave can have in flight, ran
 // 29.4 us against 24.3).  Row strides of the real side are counted in reals.
 template <int LOG2N, int KIND>
@@ -41,7 +41,7 @@ int p2_real_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     if (((long)(VPB - 1) * ivs + ilen) * 8 >= (1L << 31) || ((long)(VPB - 1) * ovs + olen) * 8 >= (1L << 31))
         return -100;
     const long grid = (d->nvec + VPB - 1) / VPB;
-    if constexpr (KIND == VHIP_R2C && LOG2N == 10) {
+    if constexpr (KIND == VHIP_R2C && LOG2N >= 8 && LOG2N <= 12) {
         static const bool off = getenv("VSIP_AMD_P2_PIPE") && atoi(getenv("VSIP_AMD_P2_PIPE")) == 0;
         const long pgrid = 2L * cus;
         if (!off && grid >= 4 * pgrid && grid <= 0x7fffffffL) {

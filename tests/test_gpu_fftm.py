@@ -284,6 +284,20 @@ def test_pipelined_real_rows(amd, m, parent):
     _check("round trip", back, x.ravel(), 1e-6)
 
 
+@pytest.mark.parametrize("m,n", [(32768 + 3, 512), (16384 + 5, 1024), (4096 + 1, 4096), (2048 + 1, 8192)])
+def test_pipelined_r2c_lengths(amd, m, n):
+    """The pipelined R2C kernel at the other half lengths (256, 512, 2048,
+    4096: groups of 16, 8, 2, 1 rows), at least 4 groups per workgroup of a
+    grid of 2 per CU (256 CUs), partial last group.  Against numpy's FP64
+    rfft."""
+    h = n // 2 + 1
+    _case("rcfftmop_f ROW %dx%d (pipelined R2C)" % (m, n))
+    x = datagen.real_data(144, (m, n))
+    z = fftm.rcfftm(amd, x.ravel(), fftm.dense_view(m, n), np.zeros(m * h, np.complex64),
+                    fftm.dense_view(m, h), scale=2.0).reshape(m, h)
+    _check("vs numpy fp64", z, np.fft.rfft(x.astype(np.float64), axis=1) * 2.0, TOL_F)
+
+
 def test_real_pow2_strided_rows(amd):
     """Real rows 2x apart in their parents (rows of a 2N-wide real matrix,
     complex rows of a 2(N/2+1)-wide one) on the power-of-two real kernel."""
