@@ -24,10 +24,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 
 // Real rows (R2C / C2R) of half length n = 2^LOG2N, one group of rows per
 // workgroup; R2C with half length 256..4096 and at least 4 groups per
-// workgroup of a grid of 2 per CU on the pipelined kernel instead (measured
-// at 8192 rows of 2048 reals and the other lengths; a pipelined C2R, whose 32 loads per lane per group
-// plus the prefetch exceed the 63 accesses a wave can have in flight, ran
-// 29.4 us against 24.3).  Row strides of the real side are counted in reals.
+// workgroup of a grid of 2 per CU on the pipelined kernel instead (faster at
+// every half length measured, 256..4096).  C2R stays on one group per
+// workgroup: pipelined forms measured slower at every length (8192 x 2048:
+// 26.7-29.4 us against 24.3).  Row strides of the real side are counted in
+// reals.
 template <int LOG2N, int KIND>
 int p2_real_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     constexpr int VPB = p2_vpb<LOG2N>();

@@ -288,14 +288,21 @@ def test_pipelined_real_rows(amd, m, parent):
 def test_pipelined_r2c_lengths(amd, m, n):
     """The pipelined R2C kernel at the other half lengths (256, 512, 2048,
     4096: groups of 16, 8, 2, 1 rows), at least 4 groups per workgroup of a
-    grid of 2 per CU (256 CUs), partial last group.  Against numpy's FP64
-    rfft."""
+    grid of 2 per CU (256 CUs), partial last group.  R2C against numpy's
+    FP64 rfft; C2R (one group per workgroup) of a non-Hermitian input
+    against the oracle, and the round trip."""
     h = n // 2 + 1
-    _case("rcfftmop_f ROW %dx%d (pipelined R2C)" % (m, n))
+    _case("rcfftmop_f / crfftmop_f ROW %dx%d (pipelined R2C)" % (m, n))
     x = datagen.real_data(144, (m, n))
-    z = fftm.rcfftm(amd, x.ravel(), fftm.dense_view(m, n), np.zeros(m * h, np.complex64),
-                    fftm.dense_view(m, h), scale=2.0).reshape(m, h)
-    _check("vs numpy fp64", z, np.fft.rfft(x.astype(np.float64), axis=1) * 2.0, TOL_F)
+    xv, zv = fftm.dense_view(m, n), fftm.dense_view(m, h)
+    z = fftm.rcfftm(amd, x.ravel(), xv, np.zeros(m * h, np.complex64), zv, scale=2.0)
+    _check("rc vs numpy fp64", z.reshape(m, h), np.fft.rfft(x.astype(np.float64), axis=1) * 2.0, TOL_F)
+    x0 = np.zeros(m * n, np.float32)
+    zz = datagen.complex_data(145, (m * h,))       # not Hermitian
+    xb = fftm.crfftm(amd, zz, zv, x0, xv, scale=1.0 / n)
+    _check("cr vs oracle", xb, po.crfftm(zz, zv, x0, xv, scale=1.0 / n), TOL_F)
+    back = fftm.crfftm(amd, z, zv, x0, xv, scale=0.5 / n)
+    _check("round trip", back, x.ravel(), 1e-6)
 
 
 def test_real_pow2_strided_rows(amd):
