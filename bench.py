@@ -357,6 +357,61 @@ def config_c2_real(lib, hip):
     return out
 
 
+def config_p2_rows(lib, hip):
+    """large batches of power-of-two rows (the pipelined kernels; set
+    VSIP_AMD_P2_PIPE=0 for the one-group-per-workgroup ones): c2c forward +
+    inverse (scale 1/N) alternating over 4 matrices in rotation, and
+    rcfftmop / crfftmop (scale 1/N) per call; microseconds per launch and
+    algorithmic GB/s.  The first 64 rows hold U[-1,1] data, the rest of each
+    matrix is zero."""
+    from jvsip_amd import datagen
+    out = {"pipelined": os.environ.get("VSIP_AMD_P2_PIPE", "1") != "0"}
+    for m, n in ((65536, 256), (32768, 512), (16384, 1024), (8192, 2048), (65536, 1024)):
+        x = datagen.complex_data(2, (64, n), dist="uniform").ravel()
+        views = []
+        for _ in range(NBUF):
+            v = lib.cmcreate_f(m, n, 0, 0)
+            assert v, lib.last_error()
+            sub = lib.cmsubview_f(v, 0, 0, 64, n)
+            lib.cmcopyfrom_user_f(x.ctypes.data, None, 0, sub)
+            lib.cmdestroy_f(sub)
+            views.append(v)
+        fwd = lib.ccfftmip_create_f(m, n, 1.0, -1, 0, 0, 1)
+        inv = lib.ccfftmip_create_f(m, n, 1.0 / n, 1, 0, 0, 1)
+        k = [0]
+
+        def pair():
+            lib.ccfftmip_f(fwd, views[k[0] % NBUF])
+            lib.ccfftmip_f(inv, views[(k[0] + NBUF // 2) % NBUF])
+            k[0] += 1
+        _, t = _time_calls(lib, hip, pair, 40)
+        out["c2c_%dx%d" % (m, n)] = {"us": round(t / 2 * 1e6, 2), "gb_per_s": round(16 * m * n / (t / 2) / 1e9, 1)}
+        lib.fftm_destroy_f(fwd)
+        lib.fftm_destroy_f(inv)
+        for v in views:
+            lib.cmalldestroy_f(v)
+    for m, n in ((32768, 512), (16384, 1024), (8192, 2048), (4096, 4096), (2048, 8192)):
+        xv = lib.mcreate_f(m, n, 0, 0)
+        zv = lib.cmcreate_f(m, n // 2 + 1, 0, 0)
+        sub = lib.msubview_f(xv, 0, 0, 64, n)
+        xr = datagen.real_data(2, (64, n))
+        lib.mcopyfrom_user_f(np.ascontiguousarray(xr).ctypes.data, 0, sub)
+        lib.mdestroy_f(sub)
+        rc = lib.rcfftmop_create_f(m, n, 1.0, 0, 0, 1)
+        cr = lib.crfftmop_create_f(m, n, 1.0 / n, 0, 0, 1)
+        _, t_rc = _time_calls(lib, hip, lambda: lib.rcfftmop_f(rc, xv, zv), 40)
+        _, t_cr = _time_calls(lib, hip, lambda: lib.crfftmop_f(cr, zv, xv), 40)
+        b = m * n * 4 + m * (n // 2 + 1) * 8
+        out["real_%dx%d" % (m, n)] = {"rc_us": round(t_rc * 1e6, 2), "rc_gb_per_s": round(b / t_rc / 1e9, 1),
+                                      "cr_us": round(t_cr * 1e6, 2), "cr_gb_per_s": round(b / t_cr / 1e9, 1)}
+        lib.fftm_destroy_f(rc)
+        lib.fftm_destroy_f(cr)
+        lib.malldestroy_f(xv)
+        lib.cmalldestroy_f(zv)
+    assert not lib.last_error(), lib.last_error()
+    return out
+
+
 def config_c3(lib, hip):
     """strided / mixed-radix ccfftmop_f (parent rows 2x the row length) and the FP64 2048^2 case"""
     from jvsip_amd import datagen, fftm
@@ -712,6 +767,7 @@ def main():
         for name, fn in (("c1_plain_64x1024", lambda: config_c1(lib, hip)),
                          ("c2_host_buffers_4096x4096", lambda: config_host_buffers(lib)),
                          ("c2_real_8192x2048", lambda: config_c2_real(lib, hip)),
+                         ("p2_rows", lambda: config_p2_rows(lib, hip)),
                          ("c3_strided_mixed_fp64", lambda: config_c3(lib, hip)),
                          ("c4_batched_uneven", lambda: config_c4(lib, hip)),
                          ("c5_served", lambda: config_c5(lib)),
