@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(VPB * (1 << LOG2N) / 16)
 template <bool PLAIN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     pow2_pipe_kernel(const vhip_fftm_desc d) {
-    __shared__ __attribute__((aligned(16))) cpx<float> lds[P2<12>::LDSV];
+    __shared__ __attribute__((aligned(16))) cpx<float> lds[kPipeLdsv];
     pow2_pipe_body<PLAIN>(d, lds);
 }
 
