@@ -56,7 +56,7 @@ int p2_pipe_launch(const vhip_fftm_desc *d, void *stream, int cus) {
 template <int LOG2N, bool PLAIN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     pow2_pipe_gen_kernel(const vhip_fftm_desc d) {
-    __shared__ __attribute__((aligned(16))) cpx<float> lds[p2_vpb<LOG2N>() * P2<LOG2N>::LDSV];
+    __shared__ __attribute__((aligned(16))) cpx<float> lds[p2_vpb<LOG2N>() * P2<LOG2N>::template ldsv<kPipePS>()];
     pow2_pipe_gen_body<LOG2N, PLAIN>(d, lds);
 }
 

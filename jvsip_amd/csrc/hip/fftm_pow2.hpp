@@ -67,13 +67,20 @@ struct P2 {
     // padded LDS vector stride (odd)
     static constexpr int S0 = N + (N >> 5);
     static constexpr int LDSV = (S0 % 2 == 0) ? S0 + 1 : S0;
+    // the same for a padding of one element per 2^PS
+    template <int PS>
+    static constexpr int ldsv() { return (N + (N >> PS)) % 2 == 0 ? N + (N >> PS) + 1 : N + (N >> PS); }
 };
 
 __host__ __device__ constexpr int padc(int i) { return i + (i >> 5); }
+// padding of one element per 2^PS (padc is PS = 5; the pipelined kernels use
+// 4, which leaves stage 0's stride-16 writes free of bank conflicts)
+template <int PS>
+__host__ __device__ constexpr int padx(int i) { return i + (i >> PS); }
 
 // Stage S: 16 elements per thread in x (already loaded for S == 0), twiddle,
 // DFT, write to LDS (or HBM for the last stage).
-template <int LOG2N, int VPB, int S, bool PLAIN, bool LAST_TO_LDS = false, bool NT = false>
+template <int LOG2N, int VPB, int S, bool PLAIN, bool LAST_TO_LDS = false, bool NT = false, int PS = 5>
 __device__ __forceinline__ void p2_stage(cpx<float> (&x)[16], cpx<float> *lds, rsrc_t tw,
                                          const int *tw_off, rsrc_t out, long out_vs, float osgn,
                                          float scale) {
@@ -96,9 +103,9 @@ __device__ __forceinline__ void p2_stage(cpx<float> (&x)[16], cpx<float> *lds, r
         for (int i = 0; i < B; ++i) {
             const int b = t + i * T;
             const int v = b >> LNB, j = b & (NB - 1);
-            const cpx<float> *src = lds + v * P::LDSV + padc(j);
+            const cpx<float> *src = lds + v * P::template ldsv<PS>() + padx<PS>(j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) x[i * R + r] = src[padc(r * NB)];
+            for (int r = 0; r < R; ++r) x[i * R + r] = src[padx<PS>(r * NB)];
         }
         __syncthreads();
     }
@@ -124,9 +131,9 @@ __device__ __forceinline__ void p2_stage(cpx<float> (&x)[16], cpx<float> *lds, r
             }
         } else {
             const int base = ((j >> LNS) << (LNS + LR)) + k;
-            cpx<float> *dst = lds + v * P::LDSV + padc(base);
+            cpx<float> *dst = lds + v * P::template ldsv<PS>() + padx<PS>(base);
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[padc(r * NS)] = x[i * R + r];
+            for (int r = 0; r < R; ++r) dst[padx<PS>(r * NS)] = x[i * R + r];
         }
     }
     if (!LAST) __syncthreads();
@@ -244,6 +251,8 @@ __device__ __forceinline__ void pow2_body(const vhip_fftm_desc &d, int per_wg, c
 // adds a multiple of 16 to a value below 16, so the padding still splits.
 __host__ __device__ constexpr int pad16(int i) { return i + (i >> 4); }
 constexpr int kPipeLdsv = 4096 + 256 + 1;   // > pad16(4095), odd
+// the same padding for the other pipelined kernels (padx<kPipePS>)
+constexpr int kPipePS = 4;
 
 // stage 0 of the 4096-point row from registers: DFT, lane t's outputs to
 // elements 16 t + r
@@ -483,7 +492,7 @@ __device__ __forceinline__ void pow2_real_body(const vhip_fftm_desc &d, cpx<floa
 constexpr int kDropOffset = (int)0x80000000u;
 
 // stage S >= 1 with the lane's twiddles w[r-1] (the same for its B butterflies)
-template <int LOG2N, int VPB, int S>
+template <int LOG2N, int VPB, int S, int PS = 5>
 __device__ __forceinline__ void p2r_stage(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> *w) {
     using P = P2<LOG2N>;
     constexpr int N = P::N;
@@ -502,9 +511,9 @@ __device__ __forceinline__ void p2r_stage(cpx<float> (&x)[16], cpx<float> *lds, 
     for (int i = 0; i < B; ++i) {
         const int b = t + i * T;
         const int v = b >> LNB, j = b & (NB - 1);
-        const cpx<float> *src = lds + v * P::LDSV + padc(j);
+        const cpx<float> *src = lds + v * P::template ldsv<PS>() + padx<PS>(j);
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[i * R + r] = src[padc(r * NB)];
+        for (int r = 0; r < R; ++r) x[i * R + r] = src[padx<PS>(r * NB)];
     }
     __syncthreads();
 #pragma unroll
@@ -516,9 +525,9 @@ __device__ __forceinline__ void p2r_stage(cpx<float> (&x)[16], cpx<float> *lds, 
         for (int r = 1; r < R; ++r) x[i * R + r] = cmul(x[i * R + r], w[r - 1]);
         Dft<float, R>::run(&x[i * R]);
         const int base = ((j >> LNS) << (LNS + LR)) + k;
-        cpx<float> *dst = lds + v * P::LDSV + padc(base);
+        cpx<float> *dst = lds + v * P::template ldsv<PS>() + padx<PS>(base);
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[padc(r * NS)] = x[i * R + r];
+        for (int r = 0; r < R; ++r) dst[padx<PS>(r * NS)] = x[i * R + r];
     }
     __syncthreads();
 }
@@ -543,9 +552,9 @@ __device__ __forceinline__ void p2r_group(cpx<float> (&x)[16], cpx<float> *lds, 
     using P = P2<LOG2N>;
     using Q = P2R<LOG2N>;
     constexpr int N = Q::N, VPB = Q::VPB, T = Q::T;
-    p2_stage<LOG2N, VPB, 0, true, true>(x, lds, make_rsrc(nullptr, 0u), nullptr, ro, 0, 1.f, 1.f);
-    if constexpr (Q::NST > 1) p2r_stage<LOG2N, VPB, (Q::NST > 1 ? 1 : 0)>(x, lds, w[0]);
-    if constexpr (Q::NST > 2) p2r_stage<LOG2N, VPB, (Q::NST > 2 ? 2 : 0)>(x, lds, w[1]);
+    p2_stage<LOG2N, VPB, 0, true, true, false, kPipePS>(x, lds, make_rsrc(nullptr, 0u), nullptr, ro, 0, 1.f, 1.f);
+    if constexpr (Q::NST > 1) p2r_stage<LOG2N, VPB, (Q::NST > 1 ? 1 : 0), kPipePS>(x, lds, w[0]);
+    if constexpr (Q::NST > 2) p2r_stage<LOG2N, VPB, (Q::NST > 2 ? 2 : 0), kPipePS>(x, lds, w[1]);
     auto z = [&](const cpx<float> F, const cpx<float> B2, cpx<float> e) {
         const cpx<float> Bc = {B2.x, -B2.y};
         const cpx<float> A = F + Bc;
@@ -557,9 +566,9 @@ __device__ __forceinline__ void p2r_group(cpx<float> (&x)[16], cpx<float> *lds, 
     for (int i = 0; i < Q::NIT; ++i) {
         const int q = t + i * T;
         const int v = q / (N / 2), m = q & (N / 2 - 1);
-        const cpx<float> *y = lds + v * P::LDSV;
-        const cpx<float> F = y[padc(m)];
-        const cpx<float> B2 = y[padc(m == 0 ? 0 : N - m)];
+        const cpx<float> *y = lds + v * P::template ldsv<kPipePS>();
+        const cpx<float> F = y[padx<kPipePS>(m)];
+        const cpx<float> B2 = y[padx<kPipePS>(m == 0 ? 0 : N - m)];
         const int vo = (int)(v * ovs) * 8;
         const bool live = v < nv;
         bstore(ro, live ? vo + m * 8 : kDropOffset, 0, z(F, B2, em[i % Q::NE]));
@@ -568,7 +577,7 @@ __device__ __forceinline__ void p2r_group(cpx<float> (&x)[16], cpx<float> *lds, 
     // z_{n/2} of row t, for the first VPB lanes
     {
         const int v = t < VPB ? t : 0;
-        const cpx<float> H = lds[v * P::LDSV + padc(N / 2)];
+        const cpx<float> H = lds[v * P::template ldsv<kPipePS>() + padx<kPipePS>(N / 2)];
         bstore(ro, (t < VPB && t < nv) ? (int)(v * ovs) * 8 + (N / 2) * 8 : kDropOffset, 0, z(H, H, eh));
     }
     __syncthreads();   // the next group's stage 0 overwrites the LDS rows
@@ -641,7 +650,7 @@ __device__ __forceinline__ void pow2_r2c_pipe_body(const vhip_fftm_desc &d, cpx<
 // input conjugation applied at use.
 
 // the last stage of a c2c group, stored to HBM (streamed)
-template <int LOG2N, int VPB, int S, bool PLAIN>
+template <int LOG2N, int VPB, int S, bool PLAIN, int PS = 5>
 __device__ __forceinline__ void p2g_last(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> *w, rsrc_t out,
                                          long out_vs, float osgn, float scale) {
     using P = P2<LOG2N>;
@@ -660,9 +669,9 @@ __device__ __forceinline__ void p2g_last(cpx<float> (&x)[16], cpx<float> *lds, c
     for (int i = 0; i < B; ++i) {
         const int b = t + i * T;
         const int v = b >> LNB, j = b & (NB - 1);
-        const cpx<float> *src = lds + v * P::LDSV + padc(j);
+        const cpx<float> *src = lds + v * P::template ldsv<PS>() + padx<PS>(j);
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[i * R + r] = src[padc(r * NB)];
+        for (int r = 0; r < R; ++r) x[i * R + r] = src[padx<PS>(r * NB)];
     }
     __syncthreads();
 #pragma unroll
@@ -692,12 +701,12 @@ __device__ __forceinline__ void p2g_group(cpx<float> (&x)[16], cpx<float> *lds, 
 #pragma unroll
         for (int e = 0; e < 16; ++e) x[e].y *= isgn;
     }
-    p2_stage<LOG2N, VPB, 0, PLAIN, false, true>(x, lds, make_rsrc(nullptr, 0u), nullptr, out, ovs, osgn, scale);
+    p2_stage<LOG2N, VPB, 0, PLAIN, false, true, kPipePS>(x, lds, make_rsrc(nullptr, 0u), nullptr, out, ovs, osgn, scale);
     if constexpr (P::NST == 3) {
-        p2r_stage<LOG2N, VPB, 1>(x, lds, w[0]);
-        p2g_last<LOG2N, VPB, 2, PLAIN>(x, lds, w[1], out, ovs, osgn, scale);
+        p2r_stage<LOG2N, VPB, 1, kPipePS>(x, lds, w[0]);
+        p2g_last<LOG2N, VPB, 2, PLAIN, kPipePS>(x, lds, w[1], out, ovs, osgn, scale);
     } else {
-        p2g_last<LOG2N, VPB, 1, PLAIN>(x, lds, w[0], out, ovs, osgn, scale);
+        p2g_last<LOG2N, VPB, 1, PLAIN, kPipePS>(x, lds, w[0], out, ovs, osgn, scale);
     }
 }
 

@@ -18,7 +18,7 @@ __global__ void __launch_bounds__(VPB * (1 << LOG2N) / 16)
 template <int LOG2N>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     pow2_r2c_pipe_kernel(const vhip_fftm_desc d) {
-    __shared__ __attribute__((aligned(16))) cpx<float> lds[p2_vpb<LOG2N>() * P2<LOG2N>::LDSV];
+    __shared__ __attribute__((aligned(16))) cpx<float> lds[p2_vpb<LOG2N>() * P2<LOG2N>::template ldsv<kPipePS>()];
     pow2_r2c_pipe_body<LOG2N>(d, lds);
 }
 
