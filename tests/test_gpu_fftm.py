@@ -114,6 +114,22 @@ def test_pipelined_short_rows(amd, m, n, direction):
     _check("vs numpy fp64", y, _np_rows(x, direction, scale), TOL_F)
 
 
+@pytest.mark.parametrize("m,n", [(16384 + 3, 1024), (4096 + 7, 4096)])
+def test_pipelined_rows_in_place_strided(amd, m, n):
+    """ccfftmip on rows 2N apart in their parent (the left half of a 2N-wide
+    matrix), in place, on the pipelined kernels; the right half must stay
+    untouched.  Forward then inverse with scale, against numpy's FP64 FFT."""
+    _case("ccfftmip_f ROW %dx%d in place, parent rows 2N (pipelined)" % (m, n))
+    xp = datagen.complex_data(146, (m, 2 * n), dist="uniform")
+    v = fftm.View(0, 2 * n, m, 1, n)
+    y = fftm.ccfftm(amd, xp.ravel(), v).reshape(m, 2 * n)
+    np.testing.assert_array_equal(y[:, n:], xp[:, n:])
+    _check("fwd vs numpy fp64", y[:, :n], _np_rows(xp[:, :n], VSIP_FFT_FWD, 1.0), TOL_F)
+    z = fftm.ccfftm(amd, y.ravel(), v, direction=VSIP_FFT_INV, scale=1.0 / n).reshape(m, 2 * n)
+    np.testing.assert_array_equal(z[:, n:], xp[:, n:])
+    _check("round trip", z[:, :n], xp[:, :n], 1e-6)
+
+
 def test_fwd_inv_roundtrip(amd):
     _case("ccfftmip_f ROW 128x4096 fwd then inv 1/N")
     x = datagen.complex_data(2, (128, 4096), dist="uniform")
