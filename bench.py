@@ -358,15 +358,17 @@ def config_c2_real(lib, hip):
 
 
 def config_p2_rows(lib, hip):
-    """large batches of power-of-two rows (the pipelined kernels; set
-    VSIP_AMD_P2_PIPE=0 for the one-group-per-workgroup ones): c2c forward +
+    """large batches of power-of-two rows (the pipelined kernels up to 4096
+    points, one-row kernels beyond; set VSIP_AMD_P2_PIPE=0 for the
+    one-group-per-workgroup ones throughout): c2c forward +
     inverse (scale 1/N) alternating over 4 matrices in rotation, and
     rcfftmop / crfftmop (scale 1/N) per call; microseconds per launch and
     algorithmic GB/s.  The first 64 rows hold U[-1,1] data, the rest of each
     matrix is zero."""
     from jvsip_amd import datagen
     out = {"pipelined": os.environ.get("VSIP_AMD_P2_PIPE", "1") != "0"}
-    for m, n in ((65536, 256), (32768, 512), (16384, 1024), (8192, 2048), (65536, 1024)):
+    for m, n in ((65536, 256), (32768, 512), (16384, 1024), (8192, 2048), (65536, 1024), (2048, 8192),
+                 (1024, 16384)):
         x = datagen.complex_data(2, (64, n), dist="uniform").ravel()
         views = []
         for _ in range(NBUF):

@@ -114,7 +114,22 @@ def test_pipelined_short_rows(amd, m, n, direction):
     _check("vs numpy fp64", y, _np_rows(x, direction, scale), TOL_F)
 
 
-@pytest.mark.parametrize("m,n", [(16384 + 3, 1024), (4096 + 7, 4096)])
+@pytest.mark.parametrize("m,direction", [(2048 + 5, VSIP_FFT_FWD), (2048 + 5, VSIP_FFT_INV),
+                                         (4096 + 512 + 1, VSIP_FFT_FWD)])
+def test_pipelined_8192_rows(amd, m, direction):
+    """8192-point rows, at least 4 per workgroup of a grid of 2 per CU (256
+    CUs), take the pipelined 8192-point kernel (stage-1 twiddles from LDS,
+    the last stage's second butterfly twiddled by W_16^r times the first's);
+    uneven rows per workgroup.  Against numpy's FP64 FFT."""
+    n = 8192
+    scale = 1.0 if direction == VSIP_FFT_FWD else 1.0 / n
+    _case("ccfftmip_f ROW %dx%d dir %+d (pipelined 8192)" % (m, n, direction))
+    x = datagen.complex_data(147, (m, n), dist="uniform")
+    y = fftm.ccfftm(amd, x.ravel(), fftm.dense_view(m, n), direction=direction, scale=scale).reshape(m, n)
+    _check("vs numpy fp64", y, _np_rows(x, direction, scale), TOL_F)
+
+
+@pytest.mark.parametrize("m,n", [(16384 + 3, 1024), (4096 + 7, 4096), (2048 + 3, 8192)])
 def test_pipelined_rows_in_place_strided(amd, m, n):
     """ccfftmip on rows 2N apart in their parent (the left half of a 2N-wide
     matrix), in place, on the pipelined kernels; the right half must stay
