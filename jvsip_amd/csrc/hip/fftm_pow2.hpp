@@ -243,16 +243,15 @@ __device__ __forceinline__ void pow2_body(const vhip_fftm_desc &d, int per_wg, c
 // row per workgroup (2 workgroups per CU, 8 rows each); issuing the second
 // row's loads before the first row is transformed gained 1.4%.
 
-// The pipelined 4096-point kernel pads its LDS row by one element per 16
-// (i + (i >> 4)) rather than per 32: stage 0 writes each lane's 16 outputs
-// at a stride of 16 elements, which the 32-element padding maps to a 132-byte
-// lane stride and two-way bank conflicts; a 136-byte stride has none (0.4%
-// faster on 4096 x 4096).  Every index split below (base plus leg offset)
-// adds a multiple of 16 to a value below 16, so the padding still splits.
-__host__ __device__ constexpr int pad16(int i) { return i + (i >> 4); }
-constexpr int kPipeLdsv = 4096 + 256 + 1;   // > pad16(4095), odd
-// the same padding for the other pipelined kernels (padx<kPipePS>)
+// The pipelined kernels pad their LDS rows by one element per 16
+// (padx<kPipePS>, i + (i >> 4)) rather than per 32: stage 0 writes each
+// lane's 16 outputs at a stride of 16 elements, which the 32-element padding
+// maps to a 132-byte lane stride and two-way bank conflicts; a 136-byte
+// stride has none (0.4% faster on 4096 x 4096).  Every index split below
+// (base plus leg offset) adds a multiple of 16 to a value below 16, so the
+// padding still splits.
 constexpr int kPipePS = 4;
+constexpr int kPipeLdsv = P2<12>::ldsv<kPipePS>();   // > padx<kPipePS>(4095), odd
 
 // stage 0 of the 4096-point row from registers: DFT, lane t's outputs to
 // elements 16 t + r
@@ -260,7 +259,7 @@ __device__ __forceinline__ void p2p_stage0(cpx<float> (&x)[16], cpx<float> *lds)
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
     Dft<float, 16>::run(x);
-    cpx<float> *dst = lds + pad16(16 * t);
+    cpx<float> *dst = lds + padx<kPipePS>(16 * t);
 #pragma unroll
     for (int r = 0; r < 16; ++r) dst[r] = x[r];
     __syncthreads();
@@ -277,9 +276,9 @@ __device__ __forceinline__ void p2p_stage(cpx<float> (&x)[16], cpx<float> *lds, 
     asm volatile("" : "+v"(t));
     const int j = t;
     {
-        const cpx<float> *src = lds + pad16(j);
+        const cpx<float> *src = lds + padx<kPipePS>(j);
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[r] = src[pad16(r * NB)];
+        for (int r = 0; r < R; ++r) x[r] = src[padx<kPipePS>(r * NB)];
         __syncthreads();
     }
 #pragma unroll
@@ -294,9 +293,9 @@ __device__ __forceinline__ void p2p_stage(cpx<float> (&x)[16], cpx<float> *lds, 
         }
     } else {
         const int base = ((j >> LNS) << (LNS + 4)) + (j & (NS - 1));
-        cpx<float> *dst = lds + pad16(base);
+        cpx<float> *dst = lds + padx<kPipePS>(base);
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[pad16(r * NS)] = x[r];
+        for (int r = 0; r < R; ++r) dst[padx<kPipePS>(r * NS)] = x[r];
         __syncthreads();
     }
 }
