@@ -30,6 +30,13 @@ __global__ void __launch_bounds__(VPB * (1 << LOG2N) / 16)
 
 // the pipelined 4096-point kernel: at most 2 workgroups per CU, so the
 // compiler may use up to 256 VGPRs
+// row groups per workgroup from which the c2c pipelined kernels run
+// (VSIP_AMD_P2_PIPE_MIN overrides the given default; read once per process)
+static int p2_pipe_min(int dflt) {
+    static const int env = getenv("VSIP_AMD_P2_PIPE_MIN") ? atoi(getenv("VSIP_AMD_P2_PIPE_MIN")) : 0;
+    return env > 0 ? env : dflt;
+}
+
 template <bool PLAIN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     pow2_pipe_kernel(const vhip_fftm_desc d) {
@@ -38,12 +45,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 }
 
 // 4096-point rows on the pipelined kernel when every workgroup of a grid of
-// 2 per CU gets at least 8 rows (measured: 8 rows per workgroup beat one row
-// per workgroup by 7% at 4096 x 4096; fewer rows leave the pipeline short)
+// 2 per CU gets at least 4 rows (measured: 8 rows per workgroup beat one row
+// per workgroup by 7% at 4096 x 4096, 4 rows by 6% at 2048 x 4096; from 2
+// rows on it gains nothing more)
 int p2_pipe_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     static const bool off = getenv("VSIP_AMD_P2_PIPE") && atoi(getenv("VSIP_AMD_P2_PIPE")) == 0;
     const long grid = 2L * cus;
-    if (off || d->nvec < 8 * grid || d->nvec > 0x7fffffffL) return -100;
+    if (off || d->nvec < p2_pipe_min(4) * grid || d->nvec > 0x7fffffffL) return -100;
     if (d->in.vs0 < 4096 || d->out.vs0 < 4096) return -100;   // rows must not overlap
     const bool plain = !d->conj_in && !d->conj_out && d->scale == 1.0;
     hipStream_t st = (hipStream_t)stream;
@@ -77,7 +85,7 @@ constexpr size_t kPipe13Lds = (size_t)(P2<13>::ldsv<kPipePS>() + 112) * 8;
 int p2_pipe13_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     static const bool off = getenv("VSIP_AMD_P2_PIPE") && atoi(getenv("VSIP_AMD_P2_PIPE")) == 0;
     const long grid = 2L * cus;
-    if (off || d->nvec < 4 * grid || d->nvec > 0x7fffffffL) return -100;
+    if (off || d->nvec < p2_pipe_min(4) * grid || d->nvec > 0x7fffffffL) return -100;
     if (d->in.vs0 < 8192 || d->out.vs0 < 8192) return -100;   // rows must not overlap
     const bool plain = !d->conj_in && !d->conj_out && d->scale == 1.0;
     hipStream_t st = (hipStream_t)stream;
@@ -87,14 +95,14 @@ int p2_pipe13_launch(const vhip_fftm_desc *d, void *stream, int cus) {
 }
 
 // row groups on the pipelined kernel when every workgroup of a grid of 2 per
-// CU gets at least 8 groups (the caller has checked radices, strides, spans)
+// CU gets at least 4 groups (the caller has checked radices, strides, spans)
 template <int LOG2N>
 int p2_pipe_gen_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     static const bool off = getenv("VSIP_AMD_P2_PIPE") && atoi(getenv("VSIP_AMD_P2_PIPE")) == 0;
     constexpr int VPB = p2_vpb<LOG2N>();
     const long grid = 2L * cus;
     const long ngroups = (d->nvec + VPB - 1) / VPB;
-    if (off || ngroups < 8 * grid || ngroups > 0x7fffffffL) return -100;
+    if (off || ngroups < p2_pipe_min(4) * grid || ngroups > 0x7fffffffL) return -100;
     const bool plain = !d->conj_in && !d->conj_out && d->scale == 1.0;
     hipStream_t st = (hipStream_t)stream;
     if (plain) hipLaunchKernelGGL((pow2_pipe_gen_kernel<LOG2N, true>), dim3((unsigned)grid), dim3(256), 0, st, *d);

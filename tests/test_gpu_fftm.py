@@ -75,9 +75,9 @@ def _np_rows(x, direction, scale):
     return (np.fft.fft(x, axis=1) if direction < 0 else np.fft.ifft(x, axis=1) * n) * scale
 
 
-@pytest.mark.parametrize("m", [4100, 4096 + 3 * 512 + 7])
+@pytest.mark.parametrize("m", [4100, 4096 + 3 * 512 + 7, 2048 + 1])
 def test_pipelined_4096_rows(amd, m):
-    """4096-point rows, at least 8 per workgroup of a grid of 2 per CU
+    """4096-point rows, at least 4 per workgroup of a grid of 2 per CU
     (256 CUs), take the pipelined persistent kernel (rows b, b + G, ...);
     the uneven row counts give some workgroups one row more, and the
     prefetch past the last row reads nothing.  Forward, inverse with scale,
@@ -100,9 +100,10 @@ def test_pipelined_4096_rows(amd, m):
 
 @pytest.mark.parametrize("m,n,direction", [(65536 + 5, 256, VSIP_FFT_FWD), (32768 + 3, 512, VSIP_FFT_INV),
                                            (16384 + 1, 1024, VSIP_FFT_FWD), (16384 + 1, 1024, VSIP_FFT_INV),
-                                           (8192 + 1, 2048, VSIP_FFT_FWD)])
+                                           (8192 + 1, 2048, VSIP_FFT_FWD), (8192 + 3, 1024, VSIP_FFT_INV),
+                                           (4096 + 1, 2048, VSIP_FFT_FWD)])
 def test_pipelined_short_rows(amd, m, n, direction):
-    """Rows of 256..2048 in groups of 4096 / n, at least 8 groups per
+    """Rows of 256..2048 in groups of 4096 / n, at least 4 groups per
     workgroup of a grid of 2 per CU (256 CUs), take the pipelined kernel;
     the row counts leave a partial last group.  Inverse cases carry a
     scale.  Against numpy's FP64 FFT."""

@@ -125,3 +125,4 @@ def test_serve_error_paths(amd, pools):
         lib.serve([(0.0, 4, 4096, -1, 1.0, 0, -1)], inb, None, budget=1 << 16)
     with pytest.raises(RuntimeError):   # no butterfly for 17
         lib.serve([(0.0, 4, 17, -1, 1.0, 0, -1)], inb, None, budget=1 << 26)
+    lib.amd_clear_error()   # the library keeps its last error; later tests check it
