@@ -68,29 +68,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     pow2_pipe_gen_body<LOG2N, PLAIN>(d, lds);
 }
 
-// the pipelined 8192-point kernel: 2 workgroups of 512 lanes per CU, 128
-// VGPRs per lane; its LDS (a padded row and the stage-1 twiddles, 70.5 KiB)
-// is dynamic
-template <bool PLAIN>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4)))
-    pow2_pipe13_kernel(const vhip_fftm_desc d) {
+// the pipelined 8192- and 16384-point kernels: 2 workgroups of 512 lanes or
+// 1 of 1024 per CU, 128 VGPRs per lane; their LDS (a padded row and the
+// stage-1 twiddles, 70.5 or 137.9 KiB) is dynamic
+template <int LOG2N, bool PLAIN>
+__global__ void __launch_bounds__(1 << (LOG2N - 4)) __attribute__((amdgpu_waves_per_eu(4, 4)))
+    pow2_pipe_big_kernel(const vhip_fftm_desc d) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    pow2_pipe13_body<PLAIN>(d, (cpx<float> *)smem);
+    pow2_pipe_big_body<LOG2N, PLAIN>(d, (cpx<float> *)smem);
 }
 
-constexpr size_t kPipe13Lds = (size_t)(P2<13>::ldsv<kPipePS>() + 112) * 8;
-
-// 8192-point rows on the pipelined kernel when every workgroup of a grid of
-// 2 per CU gets at least 4 rows
-int p2_pipe13_launch(const vhip_fftm_desc *d, void *stream, int cus) {
+// 8192- / 16384-point rows on the pipelined kernel when every workgroup of
+// a grid of 2 / 1 per CU gets at least 4 rows
+template <int LOG2N>
+int p2_pipe_big_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     static const bool off = getenv("VSIP_AMD_P2_PIPE") && atoi(getenv("VSIP_AMD_P2_PIPE")) == 0;
-    const long grid = 2L * cus;
+    constexpr int N = 1 << LOG2N;
+    const long grid = (LOG2N == 13 ? 2L : 1L) * cus;
     if (off || d->nvec < p2_pipe_min(4) * grid || d->nvec > 0x7fffffffL) return -100;
-    if (d->in.vs0 < 8192 || d->out.vs0 < 8192) return -100;   // rows must not overlap
+    if (d->in.vs0 < N || d->out.vs0 < N) return -100;   // rows must not overlap
+    const size_t lds = (size_t)P2B<LOG2N>::LDS * 8;
     const bool plain = !d->conj_in && !d->conj_out && d->scale == 1.0;
     hipStream_t st = (hipStream_t)stream;
-    if (plain) hipLaunchKernelGGL((pow2_pipe13_kernel<true>), dim3((unsigned)grid), dim3(512), kPipe13Lds, st, *d);
-    else hipLaunchKernelGGL((pow2_pipe13_kernel<false>), dim3((unsigned)grid), dim3(512), kPipe13Lds, st, *d);
+    if (plain)
+        hipLaunchKernelGGL((pow2_pipe_big_kernel<LOG2N, true>), dim3((unsigned)grid), dim3(N / 16), lds, st, *d);
+    else
+        hipLaunchKernelGGL((pow2_pipe_big_kernel<LOG2N, false>), dim3((unsigned)grid), dim3(N / 16), lds, st, *d);
     return (int)hipGetLastError();
 }
 
@@ -195,17 +198,22 @@ int pow2_try_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     }
     case 8192: {
         if (!p2_radices_match<13>(d)) return -100;
-        const int rc = p2_pipe13_launch(d, stream, cus);
+        const int rc = p2_pipe_big_launch<13>(d, stream, cus);
         return rc == -100 ? p2_launch<13>(d, stream, cus) : rc;
     }
-    case 16384: return p2_launch<14>(d, stream, cus);
+    case 16384: {
+        if (!p2_radices_match<14>(d)) return -100;
+        const int rc = p2_pipe_big_launch<14>(d, stream, cus);
+        return rc == -100 ? p2_launch<14>(d, stream, cus) : rc;
+    }
     }
     return -100;
 }
 
 int pow2_set_lds_limit(int bytes) {
     int rc = 0;
-    for (const void *f : {(const void *)pow2_pipe13_kernel<true>, (const void *)pow2_pipe13_kernel<false>}) {
+    for (const void *f : {(const void *)pow2_pipe_big_kernel<13, true>, (const void *)pow2_pipe_big_kernel<13, false>,
+                          (const void *)pow2_pipe_big_kernel<14, true>, (const void *)pow2_pipe_big_kernel<14, false>}) {
         const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return (int)e;
     }

@@ -19,7 +19,7 @@
 //     may issue the HBM loads of its next group before transforming the
 //     current one.
 // For large batches the pipelined forms further down (pow2_pipe_body,
-// pow2_pipe_gen_body, pow2_pipe13_body, pow2_r2c_pipe_body) keep the next group's loads in
+// pow2_pipe_gen_body, pow2_pipe_big_body, pow2_r2c_pipe_body) keep the next group's loads in
 // flight through the whole transform of the current one, with no other
 // vector-memory access in between.
 #pragma once
@@ -760,21 +760,36 @@ __device__ __forceinline__ void pow2_pipe_gen_body(const vhip_fftm_desc &d, cpx<
 }
 
 // ---------------------------------------------------------------------------
-// Pipelined persistent kernel for 8192-point rows (radices 16/8/8/8, 512
-// lanes, 2 workgroups per CU, so 128 VGPRs per lane), the design of the
-// shorter ones with the twiddles fitted into that budget:
-//   - stage 1 (k = t mod 16): from a 112-entry LDS table;
-//   - stage 2 (k = t mod 128): 7 per lane in registers;
-//   - stage 3 (k = t + 512 i for the lane's butterflies i = 0, 1): the 7 of
-//     i = 0 in registers, those of i = 1 as the same times W_16^r
-//     (W_8192^{r (t + 512)} = W_8192^{rt} W_16^r), the 7 constants loaded
-//     once from the plan's table (uniform, so into SGPRs).
+// Pipelined persistent kernels for 8192- and 16384-point rows (radices
+// 16/8/8/8 and 16/16/8/8; 512 or 1024 lanes; 2 or 1 workgroups per CU, so
+// 128 VGPRs per lane either way), the design of the shorter ones with the
+// twiddles fitted into that budget:
+//   - stage 1 (k = t mod 16): from a 16 (R1 - 1)-entry LDS table;
+//   - stage 2 (k = t mod NS2, a divisor of the lane count): 7 per lane in
+//     registers;
+//   - stage 3 (k = t + T i for the lane's butterflies i = 0, 1, T = N / 16):
+//     the 7 of i = 0 in registers, those of i = 1 as the same times W_16^r
+//     (W_N^{r (t + T)} = W_N^{rt} W_16^r), the 7 constants loaded once from
+//     the plan's table (uniform, so into SGPRs).
+template <int LOG2N>
+struct P2B {
+    using P = P2<LOG2N>;
+    static constexpr int N = 1 << LOG2N;
+    static constexpr int T = N / 16;
+    static constexpr int R1 = 1 << P::bits(1);
+    static constexpr int NS2 = 1 << P::log2ns(2);
+    static constexpr int NS3 = 1 << P::log2ns(3);
+    static constexpr int TW1 = 16 * (R1 - 1);                  // stage-1 table entries
+    static constexpr int LDS = P::template ldsv<kPipePS>() + TW1;
+    static_assert(P::NST == 4 && P::bits(0) == 4 && P::bits(2) == 3 && P::bits(3) == 3 && NS3 * 8 == N &&
+                      T % NS2 == 0 && TW1 <= T,
+                  "8192 = 16 8 8 8, 16384 = 16 16 8 8");
+};
 
-template <bool PLAIN>
+template <int LOG2N, bool PLAIN>
 __device__ __forceinline__ void p2e_last(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> (&w)[7],
                                          const cpx<float> (&c)[7], rsrc_t out, float osgn, float scale) {
-    constexpr int LOG2N = 13, R = 8, NB = 1024, T = 512;
-    using P = P2<LOG2N>;
+    constexpr int R = 8, NB = P2B<LOG2N>::N / 8, T = P2B<LOG2N>::T;
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
 #pragma unroll
@@ -797,15 +812,14 @@ __device__ __forceinline__ void p2e_last(cpx<float> (&x)[16], cpx<float> *lds, c
             else bstore<true>(out, j * 8, r * NB * 8, {y.x * scale, y.y * (scale * osgn)});
         }
     }
-    static_assert(P::NST == 4 && P::bits(3) == 3 && (1 << P::log2ns(3)) == NB, "8192 = 16 8 8 8");
 }
 
-// lds: P2<13>::ldsv<kPipePS>() row elements, then the 112 stage-1 twiddles
-template <bool PLAIN>
-__device__ __forceinline__ void pow2_pipe13_body(const vhip_fftm_desc &d, cpx<float> *lds) {
-    constexpr int LOG2N = 13, N = 8192;
+// lds: P2<LOG2N>::ldsv<kPipePS>() row elements, then the stage-1 twiddles
+template <int LOG2N, bool PLAIN>
+__device__ __forceinline__ void pow2_pipe_big_body(const vhip_fftm_desc &d, cpx<float> *lds) {
     using P = P2<LOG2N>;
-    static_assert(P::NST == 4 && P::bits(0) == 4 && P::bits(1) == 3 && P::bits(2) == 3, "8192 = 16 8 8 8");
+    using Q = P2B<LOG2N>;
+    constexpr int N = Q::N, T = Q::T, R1 = Q::R1;
     cpx<float> *tw1 = lds + P::template ldsv<kPipePS>();
     const cpx<float> *tw = (const cpx<float> *)d.tw;
     const cpx<float> *in = (const cpx<float> *)d.in.re;
@@ -822,23 +836,23 @@ __device__ __forceinline__ void pow2_pipe13_body(const vhip_fftm_desc &d, cpx<fl
     auto rin = [&](int row) {
         return make_rsrc(in + (long)(row < nvec ? row : 0) * ivs, row < nvec ? (unsigned)(N * 8) : 0u);
     };
-    // stage 1 (NS 16): tw1[k * 7 + r - 1] = W_128^{rk}, loaded here by every
-    // lane (lanes past the table load entry 0) and written to LDS with the
-    // first row, whose wait covers the load; stage 2 (NS 128) and stage 3
-    // (NS 1024) per lane; the stage-3 constants W_16^r
+    // stage 1 (NS 16): tw1[k (R1 - 1) + r - 1] = W_{16 R1}^{rk}, loaded here by
+    // every lane (lanes past the table load entry 0) and written to LDS with
+    // the first row, whose wait covers the load; stages 2 and 3 per lane;
+    // the stage-3 constants W_16^r
     cpx<float> t1;
     {
-        const int e = t < 112 ? t : 0, k = e / 7, r = e - 7 * k + 1;
+        const int e = t < Q::TW1 ? t : 0, k = e / (R1 - 1), r = e - (R1 - 1) * k + 1;
         t1 = tw[d.tw_off[1] + (r - 1) * 16 + k];
     }
     cpx<float> w2[7], w3[7], c3[7];
 #pragma unroll
     for (int r = 1; r < 8; ++r) {
-        w2[r - 1] = tw[d.tw_off[2] + (r - 1) * 128 + (t & 127)];
-        w3[r - 1] = tw[d.tw_off[3] + (r - 1) * 1024 + t];
-        c3[r - 1] = tw[d.tw_off[3] + (r - 1) * 1024 + 512];
+        w2[r - 1] = tw[d.tw_off[2] + (r - 1) * Q::NS2 + (t & (Q::NS2 - 1))];
+        w3[r - 1] = tw[d.tw_off[3] + (r - 1) * Q::NS3 + t];
+        c3[r - 1] = tw[d.tw_off[3] + (r - 1) * Q::NS3 + T];
     }
-    const cpx<float> *w1 = tw1 + (t & 15) * 7;
+    const cpx<float> *w1 = tw1 + (t & 15) * (R1 - 1);
     auto row = [&](cpx<float> (&x)[16], int gg) {
         if (!PLAIN) {
 #pragma unroll
@@ -848,12 +862,12 @@ __device__ __forceinline__ void pow2_pipe13_body(const vhip_fftm_desc &d, cpx<fl
         p2_stage<LOG2N, 1, 0, PLAIN, false, true, kPipePS>(x, lds, make_rsrc(nullptr, 0u), nullptr, o, 0, osgn, scale);
         p2r_stage<LOG2N, 1, 1, kPipePS>(x, lds, w1);
         p2r_stage<LOG2N, 1, 2, kPipePS>(x, lds, w2);
-        p2e_last<PLAIN>(x, lds, w3, c3, o, osgn, scale);
+        p2e_last<LOG2N, PLAIN>(x, lds, w3, c3, o, osgn, scale);
     };
     cpx<float> a[16], b[16];
     p2_load<LOG2N, 1, true, true>(a, rin(g), ivs, 1.f);
     p2_load<LOG2N, 1, true, true>(b, rin(g + G), ivs, 1.f);
-    if (t < 112) tw1[t] = t1;   // read from stage 1 on, after stage 0's barrier
+    if (t < Q::TW1) tw1[t] = t1;   // read from stage 1 on, after stage 0's barrier
     row(a, g);
     g += G;
     if (g >= nvec) return;

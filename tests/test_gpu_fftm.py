@@ -130,7 +130,23 @@ def test_pipelined_8192_rows(amd, m, direction):
     _check("vs numpy fp64", y, _np_rows(x, direction, scale), TOL_F)
 
 
-@pytest.mark.parametrize("m,n", [(16384 + 3, 1024), (4096 + 7, 4096), (2048 + 3, 8192)])
+@pytest.mark.parametrize("m,direction", [(1024 + 3, VSIP_FFT_FWD), (1024 + 3, VSIP_FFT_INV),
+                                         (2048 + 1, VSIP_FFT_FWD)])
+def test_pipelined_16384_rows(amd, m, direction):
+    """16384-point rows, at least 4 per workgroup of a grid of 1 per CU (256
+    CUs), take the pipelined 16384-point kernel (radices 16 16 8 8, stage-1
+    twiddles from LDS, the last stage's second butterfly twiddled by W_16^r
+    times the first's); uneven rows per workgroup.  Against numpy's FP64
+    FFT."""
+    n = 16384
+    scale = 1.0 if direction == VSIP_FFT_FWD else 1.0 / n
+    _case("ccfftmip_f ROW %dx%d dir %+d (pipelined 16384)" % (m, n, direction))
+    x = datagen.complex_data(148, (m, n), dist="uniform")
+    y = fftm.ccfftm(amd, x.ravel(), fftm.dense_view(m, n), direction=direction, scale=scale).reshape(m, n)
+    _check("vs numpy fp64", y, _np_rows(x, direction, scale), TOL_F)
+
+
+@pytest.mark.parametrize("m,n", [(16384 + 3, 1024), (4096 + 7, 4096), (2048 + 3, 8192), (1024 + 5, 16384)])
 def test_pipelined_rows_in_place_strided(amd, m, n):
     """ccfftmip on rows 2N apart in their parent (the left half of a 2N-wide
     matrix), in place, on the pipelined kernels; the right half must stay
