@@ -433,10 +433,10 @@ __device__ __forceinline__ void pow2_real_body(const vhip_fftm_desc &d, cpx<floa
     const rsrc_t o = make_rsrc(out, group_bytes<LOG2N, VPB>(d.nvec, g, ovs));
     constexpr bool TOL = (KIND == VHIP_R2C);
     const float osgn = KIND == VHIP_C2R ? -1.f : 1.f;
-    p2_stage<LOG2N, VPB, 0, false, TOL, KIND == VHIP_C2R>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
-    if constexpr (P::NST > 1) p2_stage<LOG2N, VPB, (P::NST > 1 ? 1 : 0), false, TOL, KIND == VHIP_C2R>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
-    if constexpr (P::NST > 2) p2_stage<LOG2N, VPB, (P::NST > 2 ? 2 : 0), false, TOL, KIND == VHIP_C2R>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
-    if constexpr (P::NST > 3) p2_stage<LOG2N, VPB, (P::NST > 3 ? 3 : 0), false, TOL, KIND == VHIP_C2R>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    p2_stage<LOG2N, VPB, 0, false, TOL, KIND == VHIP_C2R, kPipePS>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 1) p2_stage<LOG2N, VPB, (P::NST > 1 ? 1 : 0), false, TOL, KIND == VHIP_C2R, kPipePS>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 2) p2_stage<LOG2N, VPB, (P::NST > 2 ? 2 : 0), false, TOL, KIND == VHIP_C2R, kPipePS>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
+    if constexpr (P::NST > 3) p2_stage<LOG2N, VPB, (P::NST > 3 ? 3 : 0), false, TOL, KIND == VHIP_C2R, kPipePS>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
     if (KIND == VHIP_R2C) {
         // The n + 1 outputs of each row from Y in LDS (natural order), in
         // mirror pairs: item m < n/2 of row v reads Y_m and Y_{n-m} once and
@@ -459,14 +459,14 @@ __device__ __forceinline__ void pow2_real_body(const vhip_fftm_desc &d, cpx<floa
             const int q = threadIdx.x + i * T;
             const int v = q / (N / 2), m = q & (N / 2 - 1);
             if (v >= nv) continue;
-            const cpx<float> *y = lds + v * P::LDSV;
-            const cpx<float> F = y[padc(m)];
-            const cpx<float> B2 = y[padc(m == 0 ? 0 : N - m)];
+            const cpx<float> *y = lds + v * P::template ldsv<kPipePS>();
+            const cpx<float> F = y[padx<kPipePS>(m)];
+            const cpx<float> B2 = y[padx<kPipePS>(m == 0 ? 0 : N - m)];
             const int vo = (int)(v * ovs) * 8;
             bstore(ro, vo + m * 8, 0, z(F, B2, m));
             bstore(ro, vo + (N - m) * 8, 0, z(B2, F, N - m));
             if (m == 0) {
-                const cpx<float> H = y[padc(N / 2)];
+                const cpx<float> H = y[padx<kPipePS>(N / 2)];
                 bstore(ro, vo + (N / 2) * 8, 0, z(H, H, N / 2));
             }
         }

@@ -51,7 +51,7 @@ int p2_real_launch(const vhip_fftm_desc *d, void *stream, int cus) {
             return (int)hipGetLastError();
         }
     }
-    const size_t lds = (size_t)VPB * P2<LOG2N>::LDSV * 8;
+    const size_t lds = (size_t)VPB * P2<LOG2N>::template ldsv<kPipePS>() * 8;   // rows padded one element per 16
     hipLaunchKernelGGL((pow2_real_kernel<LOG2N, VPB, KIND>), dim3((unsigned)grid), dim3(T), lds,
                        (hipStream_t)stream, *d);
     return (int)hipGetLastError();
