@@ -165,8 +165,8 @@ vsip_cscalar_f vsip_cmget_f(const vsip_cmview_f*, vsip_index, vsip_index);      
 vsip_cscalar_d vsip_cmget_d(const vsip_cmview_d*, vsip_index, vsip_index);            /* vsip.h:523 */
 void vsip_cmput_f(const vsip_cmview_f*, vsip_index, vsip_index, vsip_cscalar_f);      /* vsip.h:795 */
 void vsip_cmput_d(const vsip_cmview_d*, vsip_index, vsip_index, vsip_cscalar_d);      /* vsip.h:794 */
-vsip_cmattr_f* vsip_cmgetattrib_f(const vsip_cmview_f*, vsip_cmattr_f*);              /* vsip.h:767 */
-vsip_cmattr_d* vsip_cmgetattrib_d(const vsip_cmview_d*, vsip_cmattr_d*);              /* vsip.h:766 */
+void vsip_cmgetattrib_f(const vsip_cmview_f*, vsip_cmattr_f*);              /* vsip.h:767 */
+void vsip_cmgetattrib_d(const vsip_cmview_d*, vsip_cmattr_d*);              /* vsip.h:766 */
 vsip_cblock_f* vsip_cmgetblock_f(const vsip_cmview_f*);                               /* vsip.h:359 */
 vsip_cblock_d* vsip_cmgetblock_d(const vsip_cmview_d*);                               /* vsip.h:351 */
 void vsip_cmcopyfrom_user_f(vsip_scalar_f* const, vsip_scalar_f* const, vsip_major,
@@ -197,8 +197,8 @@ vsip_scalar_f vsip_mget_f(const vsip_mview_f*, vsip_index, vsip_index);         
 vsip_scalar_d vsip_mget_d(const vsip_mview_d*, vsip_index, vsip_index);               /* vsip.h:590 */
 void vsip_mput_f(const vsip_mview_f*, vsip_index, vsip_index, vsip_scalar_f);         /* vsip.h:1138 */
 void vsip_mput_d(const vsip_mview_d*, vsip_index, vsip_index, vsip_scalar_d);         /* vsip.h:1137 */
-vsip_mattr_f* vsip_mgetattrib_f(const vsip_mview_f*, vsip_mattr_f*);                  /* vsip.h:1072 */
-vsip_mattr_d* vsip_mgetattrib_d(const vsip_mview_d*, vsip_mattr_d*);                  /* vsip.h:1071 */
+void vsip_mgetattrib_f(const vsip_mview_f*, vsip_mattr_f*);                  /* vsip.h:1072 */
+void vsip_mgetattrib_d(const vsip_mview_d*, vsip_mattr_d*);                  /* vsip.h:1071 */
 void vsip_mcopyfrom_user_f(vsip_scalar_f* const, vsip_major, const vsip_mview_f*);     /* vsip.h:1040 */
 void vsip_mcopyfrom_user_d(vsip_scalar_d* const, vsip_major, const vsip_mview_d*);     /* vsip.h:1039 */
 void vsip_mcopyto_user_f(const vsip_mview_f*, vsip_major, vsip_scalar_f* const);       /* vsip.h:1042 */

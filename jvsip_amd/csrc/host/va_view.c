@@ -338,7 +338,7 @@ void vsip_cmput_d(const vsip_cmview_d *v, vsip_index i, vsip_index j, vsip_cscal
     va_put(&v->v, i, j, &z.r, &z.i);
 }
 
-vsip_cmattr_f *vsip_cmgetattrib_f(const vsip_cmview_f *v, vsip_cmattr_f *a)
+void vsip_cmgetattrib_f(const vsip_cmview_f *v, vsip_cmattr_f *a)
 {
     a->offset = v->v.offset;
     a->row_stride = v->v.row_stride;
@@ -346,10 +346,9 @@ vsip_cmattr_f *vsip_cmgetattrib_f(const vsip_cmview_f *v, vsip_cmattr_f *a)
     a->col_stride = v->v.col_stride;
     a->col_length = v->v.col_length;
     a->block = (vsip_cblock_f *)v->v.block;
-    return a;
 }
 
-vsip_cmattr_d *vsip_cmgetattrib_d(const vsip_cmview_d *v, vsip_cmattr_d *a)
+void vsip_cmgetattrib_d(const vsip_cmview_d *v, vsip_cmattr_d *a)
 {
     a->offset = v->v.offset;
     a->row_stride = v->v.row_stride;
@@ -357,7 +356,6 @@ vsip_cmattr_d *vsip_cmgetattrib_d(const vsip_cmview_d *v, vsip_cmattr_d *a)
     a->col_stride = v->v.col_stride;
     a->col_length = v->v.col_length;
     a->block = (vsip_cblock_d *)v->v.block;
-    return a;
 }
 
 vsip_cblock_f *vsip_cmgetblock_f(const vsip_cmview_f *v) { return (vsip_cblock_f *)v->v.block; }
@@ -475,7 +473,7 @@ void vsip_mput_d(const vsip_mview_d *v, vsip_index i, vsip_index j, vsip_scalar_
     va_put(&v->v, i, j, &x, NULL);
 }
 
-vsip_mattr_f *vsip_mgetattrib_f(const vsip_mview_f *v, vsip_mattr_f *a)
+void vsip_mgetattrib_f(const vsip_mview_f *v, vsip_mattr_f *a)
 {
     a->offset = v->v.offset;
     a->row_stride = v->v.row_stride;
@@ -483,10 +481,9 @@ vsip_mattr_f *vsip_mgetattrib_f(const vsip_mview_f *v, vsip_mattr_f *a)
     a->col_stride = v->v.col_stride;
     a->col_length = v->v.col_length;
     a->block = (vsip_block_f *)v->v.block;
-    return a;
 }
 
-vsip_mattr_d *vsip_mgetattrib_d(const vsip_mview_d *v, vsip_mattr_d *a)
+void vsip_mgetattrib_d(const vsip_mview_d *v, vsip_mattr_d *a)
 {
     a->offset = v->v.offset;
     a->row_stride = v->v.row_stride;
@@ -494,7 +491,6 @@ vsip_mattr_d *vsip_mgetattrib_d(const vsip_mview_d *v, vsip_mattr_d *a)
     a->col_stride = v->v.col_stride;
     a->col_length = v->v.col_length;
     a->block = (vsip_block_d *)v->v.block;
-    return a;
 }
 
 void vsip_mcopyfrom_user_f(vsip_scalar_f *const r, vsip_major major, const vsip_mview_f *v)

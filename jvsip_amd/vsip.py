@@ -108,7 +108,7 @@ def _sig(t):
         "cmput": (None, [P, c_vi, c_vi, cs]),
         "cmcopyfrom_user": (None, [P, P, c_bl, P]),
         "cmcopyto_user": (None, [P, c_bl, P, P]),
-        "cmgetattrib": (P, [P, ctypes.POINTER(cmattr)]),
+        "cmgetattrib": (None, [P, ctypes.POINTER(cmattr)]),
         "cmtransview": (P, [P]),
         "mbind": (P, [P, c_vi, c_st, c_vi, c_st, c_vi]),
         "mcreate": (P, [c_vi, c_vi, c_bl, c_bl]),
