@@ -67,6 +67,39 @@ class Hip:
         return t.value
 
 
+def cpu_share():
+    """The host cores this process may use: the CPUs of its affinity mask,
+    capped by the cgroup's CPU quota (cpu.max) when one is set, and at most 16
+    (one GPU's share of the host on a multi-GPU node); with the host's
+    logical CPU count and CPU model, as lscpu reports them."""
+    n_all = os.cpu_count() or 1
+    try:
+        n_aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n_aff = n_all
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    cores = n_aff if quota is None else max(1, min(n_aff, int(quota)))
+    cores = min(cores, 16)
+    model = "unknown"
+    try:
+        import subprocess
+        for line in subprocess.check_output(["lscpu"], text=True).splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return {"cores": cores, "nproc": n_all, "affinity_cpus": n_aff, "cgroup_cpu_quota": quota,
+            "lscpu_model": model,
+            "policy": "threads = the affinity CPUs, capped by the cgroup quota and by 16, "
+                      "one GPU's share of the host"}
+
+
 def cpu_baseline(threads, steps=5, rows_single=256):
     """jvsip's C on the host: fwd+inv of the 4096 x 4096 matrix split into
     `threads` row slabs (one plan per thread; plans are independent), timed
@@ -470,23 +503,13 @@ def config_c3(lib, hip):
     return out
 
 
-def _c4_lengths(count, seed):
-    """lengths log-uniform over {2^a 3^b 5^c} in [64, 65536]"""
-    cands = sorted({2 ** a * 3 ** b * 5 ** c for a in range(17) for b in range(11) for c in range(7)
-                    if 64 <= 2 ** a * 3 ** b * 5 ** c <= 65536})
-    logc = np.log(np.array(cands, dtype=np.float64))
-    rng = np.random.default_rng(seed)
-    u = rng.uniform(np.log(64), np.log(65536), count)
-    return [cands[int(np.argmin(np.abs(logc - v)))] for v in u]
-
-
-def config_c4(lib, hip, njobs=3000):
+def config_c4(lib, hip, njobs=3000, legs="all"):
     """njobs FFTs of log-uniform 2^a 3^b 5^c lengths in [64, 65536], rows so the
-    total is ~2^27 complex elements: one batched graph launch vs plain per-call"""
+    total is ~2^27 complex elements: one batched graph launch vs plain per-call.
+    legs="batched" times the graph replays only (for counter collection, where
+    each of the plain legs' ~4000 launches would be serialised)"""
     from jvsip_amd import datagen, fftm
-    lengths = _c4_lengths(njobs, 4)
-    per_job = (1 << 27) / njobs
-    rows = [max(1, int(round(per_job / n))) for n in lengths]
+    lengths, rows = datagen.c4_jobs(njobs, 4)
     total = int(sum(r * n for r, n in zip(rows, lengths)))
     pool = datagen.complex_data(4, (total,))
     blk = lib.cblockcreate_f(total, 0)
@@ -509,6 +532,19 @@ def config_c4(lib, hip, njobs=3000):
     info = lib.batch_info(b)
     # batched: graph replays (forward in place; the data just keeps transforming)
     wall_b, gpu_b = _time_calls(lib, hip, lambda: lib.amd_batch_run_f(b), 10)
+    bytes_ = 2 * total * 8
+    if legs == "batched":
+        out = {"jobs": njobs, "elements": total, "distinct_lengths": len(set(lengths)), "batch": info,
+               "batch_create_s": round(t_create, 3), "legs": legs,
+               "batched_ms": round(gpu_b * 1e3, 3), "batched_gb_per_s": round(bytes_ / gpu_b / 1e9, 1)}
+        lib.amd_batch_destroy_f(b)
+        for p in plans:
+            lib.fftm_destroy_f(p)
+        for v in views + rviews:
+            lib.cmdestroy_f(v)
+        lib.cmalldestroy_f(whole)
+        lib.cmalldestroy_f(ref_whole)
+        return out
     # plain per-call, back to back (no synchronisation between calls)
     def plain_all():
         for p, v in zip(plans, views):
@@ -539,7 +575,6 @@ def config_c4(lib, hip, njobs=3000):
         if j in sample:
             errs.append(rel_l2(got[sl], fft64(pool[sl].reshape(r, n), axis=1)))
         off += r * n
-    bytes_ = 2 * total * 8
     out = {"jobs": njobs, "elements": total, "distinct_lengths": len(set(lengths)), "batch": info,
            "batch_create_s": round(t_create, 3),
            "batched_ms": round(gpu_b * 1e3, 3), "batched_gb_per_s": round(bytes_ / gpu_b / 1e9, 1),
@@ -641,42 +676,43 @@ def config_f3_komega(lib, hip):
 def config_c5(lib, nreq=2000, rate=20000.0):
     """Poisson requests at `rate`/s: rows U[1,512], lengths {256, 1000, 1024,
     2187, 4096, 16384}, fwd/inv 50/50, N(0,1) inputs from a device pool,
-    64 GB HBM budget; served (continuous batching) vs sequential plain calls"""
+    64 GB HBM budget.  Three forms of the same stream: served (continuous
+    batching over 4 lanes), plain calls with a synchronisation after each,
+    and plain calls back to back (no synchronisation; completion from an
+    event per request).  Run at the workload's rate, at 5x that rate, and
+    with every request present at once (saturation)."""
     from jvsip_amd import datagen
-    lengths = [256, 1000, 1024, 2187, 4096, 16384]
-    pool_n = 512 * 16384 * 2
+    pool_n = datagen.C5_POOL
     x = datagen.complex_data(5, (pool_n,))
     blk = lib.cblockcreate_f(pool_n, 0)
     pv = lib.cmbind_f(blk, 0, pool_n, 1, 1, pool_n)
     lib.cmcopyfrom_user_f(x.ctypes.data, None, 0, pv)
-    rng = np.random.default_rng(5)
-    t = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    reqs = []
-    for i in range(nreq):
-        n = int(rng.choice(lengths))
-        r = int(rng.integers(1, 513))
-        d = -1 if rng.random() < 0.5 else 1
-        reqs.append((float(t[i]), r, n, d, 1.0 if d < 0 else 1.0 / n, int(rng.integers(0, pool_n - r * n)), -1))
+    reqs = datagen.c5_requests(nreq, rate, seed=5)
     budget = 64 << 30
     out = {"requests": nreq, "rate_per_s": rate, "hbm_budget_bytes": budget}
 
-    def lat(res):
-        l = (res["t_done"] - np.array([q[0] for q in reqs])) * 1e6
+    def lat(res, stream):
+        l = (res["t_done"] - np.array([q[0] for q in stream])) * 1e6
         return {"p50_us": round(float(np.percentile(l, 50)), 1), "p99_us": round(float(np.percentile(l, 99)), 1),
                 "mean_us": round(float(l.mean()), 1)}
 
+    def forms(stream):
+        res_s, st_s = lib.serve(stream, blk, None, budget=budget, lanes=4, max_jobs=64)
+        res_p, st_p = lib.serve(stream, blk, None, budget=budget, plain=True, sync_each=True)
+        res_b, st_b = lib.serve(stream, blk, None, budget=budget, plain=True, sync_each=False)
+        return {"served": dict(lat(res_s, stream), **st_s),
+                "plain_sync_each": dict(lat(res_p, stream), **st_p),
+                "plain_back_to_back": dict(lat(res_b, stream), **st_b),
+                "served_vs_plain_sync_each": round(st_s["req_per_s"] / st_p["req_per_s"], 2),
+                "served_vs_plain_back_to_back": round(st_s["req_per_s"] / st_b["req_per_s"], 2)}
+
     lib.serve(reqs[:50], blk, None, budget=1 << 30)            # warm the schedules
-    res_s, st_s = lib.serve(reqs, blk, None, budget=budget, lanes=4, max_jobs=64)
-    res_p, st_p = lib.serve(reqs, blk, None, budget=budget, plain=True)
-    out["served"] = dict(lat(res_s), **st_s)
-    out["plain"] = dict(lat(res_p), **st_p)
-    # saturation: every request present at t = 0
-    sat = [(0.0,) + q[1:] for q in reqs]
-    _, sat_s = lib.serve(sat, blk, None, budget=budget, lanes=4, max_jobs=64)
-    _, sat_p = lib.serve(sat, blk, None, budget=budget, plain=True)
-    out["saturation"] = {"served_req_per_s": sat_s["req_per_s"], "plain_req_per_s": sat_p["req_per_s"],
-                         "served_gb_per_s": sat_s["gb_per_s"], "plain_gb_per_s": sat_p["gb_per_s"],
-                         "speedup": round(sat_s["req_per_s"] / sat_p["req_per_s"], 2)}
+    out["at_rate"] = forms(reqs)
+    fast = [(q[0] / 5.0,) + q[1:] for q in reqs]
+    out["at_5x_rate"] = forms(fast)
+    out["saturation"] = forms([(0.0,) + q[1:] for q in reqs])
+    out["throughput_ratio_basis"] = ("saturation req/s of the served form over each plain form; "
+                                     "at the workload's rate every form keeps up with arrivals")
     lib.cmalldestroy_f(pv)
     return out
 
@@ -693,6 +729,11 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the other workload configurations")
     ap.add_argument("--only", default="", help="comma-separated configuration names (or prefixes, "
                     "e.g. c3,c4) to run besides the headline; default all")
+    ap.add_argument("--c4-legs", default="all", choices=["all", "batched"],
+                    help="c4: 'batched' times the graph replays only (no plain per-call legs)")
+    ap.add_argument("--settle-ms", type=float, default=60.0,
+                    help="milliseconds of headline steps run before the warm-up steps, so the "
+                         "device reaches its sustained clocks whatever --warmup is")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -725,14 +766,28 @@ def main():
         lib.ccfftmip_f(fwd, views[i % NBUF])
         lib.ccfftmip_f(inv, views[(i + NBUF // 2) % NBUF])
 
+    # the device's own in-place read+write and read-only HBM rates (4 x 128 MiB),
+    # measured first: they also start bringing the clocks up
+    rmw, rd = ctypes.c_double(), ctypes.c_double()
+    assert lib.amd_hbm_probe(BYTES_PER_CALL // 2, 20, ctypes.byref(rmw), ctypes.byref(rd)) == 0
+    # clock settling: the device reaches its sustained clocks only after some
+    # milliseconds under load; run the same steps for settle_ms, untimed
+    ts = time.perf_counter()
+    settle_steps = 0
+    while (time.perf_counter() - ts) * 1e3 < args.settle_ms:
+        for _ in range(8):
+            step(settle_steps)
+            settle_steps += 1
+        assert lib.amd_sync() == 0, lib.last_error()
+    settle_ms = (time.perf_counter() - ts) * 1e3
     for i in range(args.warmup):
-        step(i)
+        step(settle_steps + i)
     assert lib.amd_sync() == 0, lib.last_error()
     ev0, ev1 = hip.event(), hip.event()
     t0 = time.perf_counter()
     hip.record(ev0, stream)
     for i in range(args.steps):
-        step(args.warmup + i)
+        step(settle_steps + args.warmup + i)
     hip.record(ev1, stream)
     rc = lib.amd_sync()
     t1 = time.perf_counter()
@@ -754,9 +809,6 @@ def main():
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    # the device's own in-place read+write and read-only HBM rates (4 x 128 MiB)
-    rmw, rd = ctypes.c_double(), ctypes.c_double()
-    assert lib.amd_hbm_probe(BYTES_PER_CALL // 2, 20, ctypes.byref(rmw), ctypes.byref(rd)) == 0
     acc = accuracy(lib, None)
     for v in views:
         lib.cmalldestroy_f(v)
@@ -771,7 +823,7 @@ def main():
                          ("c2_real_8192x2048", lambda: config_c2_real(lib, hip)),
                          ("p2_rows", lambda: config_p2_rows(lib, hip)),
                          ("c3_strided_mixed_fp64", lambda: config_c3(lib, hip)),
-                         ("c4_batched_uneven", lambda: config_c4(lib, hip)),
+                         ("c4_batched_uneven", lambda: config_c4(lib, hip, legs=args.c4_legs)),
                          ("c5_served", lambda: config_c5(lib)),
                          ("f1_fft1d", lambda: config_fft1d(lib, hip)),
                          ("f3_komega", lambda: config_f3_komega(lib, hip))):
@@ -784,8 +836,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        share = cpu_share()
+        threads = args.cpu_threads or share["cores"]
         cpu = cpu_baseline(threads)
+        if cpu:
+            cpu["host"] = share
 
     out = {
         "metric": "multi-FFT HBM GB/s (vsip_ccfftmip_f 4096x4096 c2c fwd+inv)",
@@ -794,6 +849,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "settle_ms": round(settle_ms, 1),
+        "settle_steps": settle_steps,
         "ms_per_step": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",

@@ -85,10 +85,14 @@ int vsip_amd_tune(int prec, long n, char *buf, size_t len);
  * out_offset, or (out_offset < 0) to memory the server takes from its HBM
  * budget and returns when the request completes.  Requests must be sorted by
  * t_arrive (seconds after the run starts).  The server admits arrived
- * requests while the budget and its launch lanes allow, launches each group
- * as one persistent batched kernel on a free lane (lanes overlap on the
- * device) and retires them as they finish; res[i] gets the admission and
- * completion times.  stats receives a JSON summary.  Returns 0 or an error
+ * requests, up to max_jobs per group, while the budget and its launch lanes
+ * allow, and launches each group on a free lane (a stream; lanes overlap on
+ * the device): requests of at least VSIP_AMD_SERVE_DIRECT elements (default
+ * 32768) launch their own length-specialised kernel, and the smaller ones of
+ * the group share one persistent batched kernel when there are more than two
+ * of them.  Finished groups are retired as they complete; res[i] gets the
+ * admission and completion times.  stats receives a JSON summary (with the
+ * counts of directly launched and batched requests).  Returns 0 or an error
  * code. */
 typedef struct {
     double t_arrive;
@@ -110,10 +114,14 @@ typedef struct {
 int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_f *in_pool,
                      const vsip_cblock_f *out_pool, size_t hbm_budget, int lanes, int max_jobs,
                      vsip_amd_result *res, char *stats, size_t stats_len);
-/* the same stream served by sequential plain calls (launch + synchronise per request) */
+/* the same stream served by plain calls in arrival order on the library
+ * stream, each launched when its request arrives: with sync_each != 0 every
+ * call is followed by a synchronisation (a caller that waits for each
+ * result), otherwise the calls go back to back and res[i].t_done is when the
+ * event recorded after request i is seen complete */
 int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_f *in_pool,
-                           const vsip_cblock_f *out_pool, size_t hbm_budget, vsip_amd_result *res,
-                           char *stats, size_t stats_len);
+                           const vsip_cblock_f *out_pool, size_t hbm_budget, int sync_each,
+                           vsip_amd_result *res, char *stats, size_t stats_len);
 
 /* One k-omega beamformer step, the komega() of the reference's example
  * (examples/komegaExamples/kw_c/kw.c:7-25), in two fused launches:

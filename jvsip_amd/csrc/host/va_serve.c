@@ -305,6 +305,7 @@ static void va_lane_fini(VA_lane *l)
 typedef struct {
     size_t out_off, work_off, out_len, work_len;
     int own_out, own_work;
+    void *work, *work2;    /* scratch matrices inside the arena span work_off */
 } VA_reqmem;
 
 static void *va_pool_addr(const vsip_cblock_f *pool, long off)
@@ -357,9 +358,19 @@ static void va_plan_cache_free(VA_plan_cache *c)
     c->n = 0;
 }
 
-/* launch request r directly with its length's plain schedule */
+/* scratch matrices (rows x n each) of length n's plain schedule: 0, or 1
+ * or 2 for the two- and three-pass splits */
+static int va_plain_nwork(const VA_plan_cache *pc, long n)
+{
+    const VA_fftm *base = va_plan_get((VA_plan_cache *)pc, n);
+    return (base && base->npass >= 2) ? base->npass - 1 : 0;
+}
+
+/* launch request r directly with its length's plain schedule; a split
+ * schedule runs on the request's own scratch (work, work2), never on the
+ * cached plan's, since requests of one length run on several lanes at once */
 static int va_direct_launch(const VA_plan_cache *pc, const vsip_amd_request *r, void *in, void *out,
-                            void *stream)
+                            void *work, void *work2, void *stream)
 {
     const VA_fftm *base = va_plan_get((VA_plan_cache *)pc, r->n);
     VA_fftm p;
@@ -367,19 +378,34 @@ static int va_direct_launch(const VA_plan_cache *pc, const vsip_amd_request *r, 
     VA_mview vi, vo;
     vhip_fftm_desc d[VA_MAX_DESC];
     int nd, k, rc = 0;
-    if (!base) return -1;
-    if ((vsip_length)r->rows > base->mN) return -1;     /* beyond the plan's scratch */
+    if (!base || base->staged || base->npass < 1) return -1;
+    if (base->npass >= 2 && !work) return -1;
+    if (base->npass == 3 && !work2) return -1;
     p = *base;
     p.mN = (vsip_length)r->rows;
     p.M = (vsip_length)r->rows;
     p.dir = r->dir;
     p.scale = r->scale;
+    p.dev_work = work;
+    p.dev_work2 = work2;
     va_fake_view(&bi, &vi, in, r->rows, r->n);
     va_fake_view(&bo, &vo, out, r->rows, r->n);
     nd = va_fftm_descs(&p, &vi, &vo, NULL, NULL, NULL, NULL, d);
     if (nd < 0) return -1;
     for (k = 0; k < nd && !rc; k++) rc = vhip_fftm_launch(&d[k], stream);
     return rc;
+}
+
+/* work items of request q's batched launch (descriptors built on dummy
+ * addresses: only their vector counts are used), or -1 */
+static long va_req_items(const vsip_amd_request *q, void *dummy, VA_ptw_cache *pc)
+{
+    vhip_fftm_desc d[VA_MAX_DESC];
+    long items = 0;
+    int k, nd = va_req_descs(q, dummy, dummy, dummy, pc, d);
+    if (nd < 0) return -1;
+    for (k = 0; k < nd; k++) items += (d[k].nvec + d[k].vpb - 1) / d[k].vpb;
+    return items;
 }
 
 int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_f *in_pool,
@@ -392,8 +418,10 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
     VA_ptw_cache pc;
     VA_plan_cache plc;
     void **req_in = NULL, **req_out = NULL;
+    char *req_direct = NULL;        /* 1: the request launches its own kernel */
+    signed char *req_nwork = NULL;  /* scratch matrices the request needs */
     long *fifo = NULL, head = 0, tail = 0, next = 0, done = 0, i, direct = 0;
-    long batches = 0, jobs_launched = 0, max_inflight = 0;
+    long batches = 0, jobs_launched = 0, max_inflight = 0, batched_launches = 0, batched_reqs = 0;
     int L, rc = 0, grid_max = 0;
     long direct_elems = va_direct_elems();
     double t0, tend;
@@ -407,20 +435,40 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
     fifo = (long *)malloc(sizeof(long) * (size_t)nreq);
     req_in = (void **)malloc(sizeof(void *) * (size_t)nreq);
     req_out = (void **)malloc(sizeof(void *) * (size_t)nreq);
-    if (!lanes || !mem || !fifo || !req_in || !req_out) { rc = -1; goto out; }
+    req_direct = (char *)malloc((size_t)nreq);
+    req_nwork = (signed char *)malloc((size_t)nreq);
+    if (!lanes || !mem || !fifo || !req_in || !req_out || !req_direct || !req_nwork) { rc = -1; goto out; }
     for (L = 0; L < nstreams && !rc; L++) rc = va_lane_init(&lanes[L], max_jobs);
     if (!rc) rc = vhip_batch_max_grid(VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS, &grid_max);
-    /* warm the schedule cache for every length before the clock starts, and
-     * refuse requests that could never fit the budget */
+    /* warm the schedule caches for every length before the clock starts */
     for (i = 0; i < nreq && !rc; i++) {
         const VA_sched *s;
-        size_t need = (size_t)reqs[i].rows * (size_t)reqs[i].n * 8;
-        if (va_tune_get(4, (int)reqs[i].n, &s) == -1) rc = -1;
-        if (va_req_two_pass(&reqs[i])) need *= 2;
+        if (reqs[i].rows < 1 || reqs[i].n < 1 || va_tune_get(4, (int)reqs[i].n, &s) == -1) rc = -1;
         if (reqs[i].out_offset >= 0 && !out_pool) rc = -1;
-        if (need + 512 > hbm_budget) rc = -1;
     }
     if (!rc && va_plan_cache_fill(&plc, reqs, nreq)) rc = -1;
+    /* route every request: its own kernel for large requests and for any
+     * whose batched items would not fit one lane's item table (so the FIFO
+     * head can always be admitted by an idle lane), else the lane's batched
+     * launch; refuse requests that could never fit the budget */
+    for (i = 0; i < nreq && !rc; i++) {
+        const vsip_amd_request *q = &reqs[i];
+        const VA_fftm *pl = va_plan_get(&plc, q->n);
+        size_t len = (size_t)q->rows * (size_t)q->n * 8;
+        int nb = 0, np;
+        if (!pl || pl->staged || pl->npass < 1) { rc = -1; break; }
+        np = va_plain_nwork(&plc, q->n);
+        req_direct[i] = ((long)q->rows * q->n >= direct_elems);
+        if (!req_direct[i]) {
+            long items = va_req_items(q, in_pool->b.dev_r, &pc);
+            if (items < 0) { rc = -1; break; }
+            if (items > lanes[0].cap_items) req_direct[i] = 1;
+            nb = va_req_two_pass(q) ? 1 : 0;
+        }
+        /* a batched request may still run on its plain schedule (small groups) */
+        req_nwork[i] = (signed char)(req_direct[i] ? np : (nb > np ? nb : np));
+        if (len * (size_t)(1 + req_nwork[i]) + 1024 > hbm_budget) rc = -1;
+    }
     if (rc) { va_error("serve", "setup failed", rc); goto out; }
     for (i = 0; i < nreq; i++) {
         res[i].t_admit = res[i].t_done = -1.0;
@@ -461,13 +509,14 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
             while (head < tail && l->nreqs < max_jobs) {
                 long r = fifo[head];
                 const vsip_amd_request *q = &reqs[r];
-                void *in, *out, *work = NULL;
-                int k, n2, dir_launch = ((long)q->rows * q->n >= direct_elems);
+                void *in, *out;
+                int k, n2;
                 long need_items = 0;
                 mem[r].out_len = (size_t)q->rows * (size_t)q->n * 8;
                 mem[r].own_out = (q->out_offset < 0);
-                mem[r].own_work = !dir_launch && va_req_two_pass(q);
-                mem[r].work_len = mem[r].own_work ? mem[r].out_len : 0;
+                mem[r].own_work = (req_nwork[r] > 0);
+                mem[r].work_len = mem[r].out_len * (size_t)req_nwork[r];
+                mem[r].work = mem[r].work2 = NULL;
                 if (mem[r].own_out) {
                     mem[r].out_off = va_arena_alloc(&arena, mem[r].out_len);
                     if (mem[r].out_off == (size_t)-1) break;
@@ -478,11 +527,12 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
                         if (mem[r].own_out) va_arena_free(&arena, mem[r].out_off, mem[r].out_len);
                         break;
                     }
-                    work = arena.base + mem[r].work_off;
+                    mem[r].work = arena.base + mem[r].work_off;
+                    if (req_nwork[r] == 2) mem[r].work2 = (char *)mem[r].work + mem[r].out_len;
                 }
                 in = va_pool_addr(in_pool, q->in_offset);
                 out = mem[r].own_out ? (void *)(arena.base + mem[r].out_off) : va_pool_addr(out_pool, q->out_offset);
-                if (dir_launch) {
+                if (req_direct[r]) {
                     req_in[r] = in;
                     req_out[r] = out;
                     res[r].t_admit = va_now() - t0;
@@ -493,11 +543,12 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
                     head++;
                     continue;
                 }
-                n2 = va_req_descs(q, in, out, work, &pc, &l->desc_host[nd]);
+                n2 = va_req_descs(q, in, out, mem[r].work, &pc, &l->desc_host[nd]);
                 if (n2 < 0) { rc = -1; va_error("serve", "request has no schedule", (int)q->n); goto out; }
                 for (k = 0; k < n2; k++)
                     need_items += (l->desc_host[nd + k].nvec + l->desc_host[nd + k].vpb - 1) / l->desc_host[nd + k].vpb;
                 if (ni + need_items > l->cap_items) {
+                    /* the lane is full; routing guarantees an idle lane takes it */
                     if (mem[r].own_out) va_arena_free(&arena, mem[r].out_off, mem[r].out_len);
                     if (mem[r].own_work) va_arena_free(&arena, mem[r].work_off, mem[r].work_len);
                     break;
@@ -532,7 +583,8 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
             for (g = 0; g < l->nreqs && !rc; g++) {
                 long r = l->reqs[g];
                 if (!l->direct[g] && nbatched > VA_DIRECT_MAX) continue;
-                rc = va_direct_launch(&plc, &reqs[r], req_in[r], req_out[r], l->stream);
+                rc = va_direct_launch(&plc, &reqs[r], req_in[r], req_out[r], mem[r].work, mem[r].work2,
+                                      l->stream);
                 direct++;
             }
             if (!rc && nbatched > VA_DIRECT_MAX) {
@@ -547,6 +599,8 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
                 if (!rc) rc = vhip_batch_launch(l->desc_dev, (const vhip_batch_item *)(db + dbytes), n1, n2b,
                                                 l->ctl_dev, VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS,
                                                 grid_max, l->stream);
+                batched_launches++;
+                batched_reqs += nbatched;
             }
             if (!rc) rc = vhip_event_record(l->event, l->stream);
             if (rc) { va_error("serve", vhip_error_string(rc), rc); goto out; }
@@ -559,16 +613,19 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
     }
     tend = va_now() - t0;
     if (stats && stats_len) {
-        char tmp[512];
+        char tmp[768];
         double bytes = 0.0;
         for (i = 0; i < nreq; i++) bytes += 2.0 * (double)reqs[i].rows * (double)reqs[i].n * 8.0;
         sprintf(tmp, "{\"requests\": %ld, \"seconds\": %.6f, \"req_per_s\": %.1f, \"gb_per_s\": %.2f, "
                 "\"batches\": %ld, \"jobs_per_batch\": %.2f, \"lanes\": %d, \"max_jobs\": %d, "
                 "\"max_lanes_in_flight\": %ld, \"hbm_budget\": %lu, \"hbm_peak\": %lu, "
-                "\"direct_launched\": %ld}",
+                "\"direct_launched\": %ld, \"batched_requests\": %ld, \"batched_launches\": %ld, "
+                "\"requests_per_batched_launch\": %.2f, \"direct_threshold_elems\": %ld}",
                 nreq, tend, (double)nreq / tend, bytes / tend / 1e9, batches,
                 batches ? (double)jobs_launched / (double)batches : 0.0, nstreams, max_jobs,
-                max_inflight, (unsigned long)hbm_budget, (unsigned long)arena.peak, direct);
+                max_inflight, (unsigned long)hbm_budget, (unsigned long)arena.peak, direct,
+                batched_reqs, batched_launches,
+                batched_launches ? (double)batched_reqs / (double)batched_launches : 0.0, direct_elems);
         strncpy(stats, tmp, stats_len - 1);
         stats[stats_len - 1] = '\0';
     }
@@ -587,27 +644,40 @@ out:
     free(fifo);
     free(req_in);
     free(req_out);
+    free(req_direct);
+    free(req_nwork);
     return rc;
 }
 
-/* the same request stream as sequential plain calls, each synchronised */
+/* the same request stream as plain calls in arrival order on the library
+ * stream: with sync_each, each call is followed by a synchronisation (a
+ * caller that waits for every result); without, the calls go back to back
+ * and an event per request gives its completion time */
+#define VA_PLAIN_EVENTS 256
+
 int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_f *in_pool,
-                           const vsip_cblock_f *out_pool, size_t hbm_budget, vsip_amd_result *res,
-                           char *stats, size_t stats_len)
+                           const vsip_cblock_f *out_pool, size_t hbm_budget, int sync_each,
+                           vsip_amd_result *res, char *stats, size_t stats_len)
 {
     VA_arena arena;
     VA_fftm *plans = NULL;
-    int nplans = 0, cap = 64, rc = 0;
-    long i;
+    void *ev[VA_PLAIN_EVENTS];
+    size_t *offs = NULL;
+    int nplans = 0, cap = 64, rc = 0, e;
+    long i, oldest = 0;
     double t0, tend;
     if (va_ensure_init()) return -1;
     if (!in_pool || nreq <= 0) return -1;
+    memset(ev, 0, sizeof(ev));
     if (va_arena_init(&arena, hbm_budget)) { va_error("serve", "HBM budget allocation failed", -1); return -1; }
     plans = (VA_fftm *)calloc((size_t)cap, sizeof(VA_fftm));
-    if (!plans) { va_arena_fini(&arena); return -1; }
+    offs = (size_t *)malloc(sizeof(size_t) * (size_t)nreq);
+    if (!plans || !offs) { rc = -1; goto out; }
+    for (e = 0; e < VA_PLAIN_EVENTS && !rc; e++) rc = vhip_event_create(&ev[e]);
     /* plans are made before the clock starts, as a server would keep them */
     for (i = 0; i < nreq && !rc; i++) {
         int k, found = 0;
+        if (reqs[i].out_offset >= 0 && !out_pool) { rc = -1; break; }
         for (k = 0; k < nplans && !found; k++)
             found = (plans[k].mN == (vsip_length)reqs[i].rows && plans[k].MN == (vsip_length)reqs[i].n &&
                      plans[k].dir == reqs[i].dir && plans[k].scale == (double)reqs[i].scale);
@@ -624,22 +694,40 @@ int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_c
     }
     if (rc) { va_error("serve_plain", "plan creation failed", rc); goto out; }
     t0 = va_now();
-    for (i = 0; i < nreq && !rc; i++) {
-        const vsip_amd_request *q = &reqs[i];
+    for (i = 0; i <= nreq && !rc; i++) {
+        const vsip_amd_request *q = (i < nreq) ? &reqs[i] : NULL;
         VA_block bi, bo;
         VA_mview vi, vo;
-        size_t len = (size_t)q->rows * (size_t)q->n * 8, off = 0;
+        size_t len = q ? (size_t)q->rows * (size_t)q->n * 8 : 0;
         void *out;
         int k;
-        while (va_now() - t0 < q->t_arrive) { /* wait for the request to arrive */ }
-        res[i].t_admit = va_now() - t0;
-        if (q->out_offset < 0) {
-            off = va_arena_alloc(&arena, len);
-            if (off == (size_t)-1) { rc = -1; va_error("serve_plain", "HBM budget exhausted", -1); break; }
-            out = arena.base + off;
-        } else {
-            out = va_pool_addr(out_pool, q->out_offset);
+        /* retire finished requests (in order: one stream); wait for the
+         * request's arrival, for a free event, or at the end for everyone */
+        for (;;) {
+            int need = (i == nreq && oldest < i) || (i - oldest >= VA_PLAIN_EVENTS);
+            while (oldest < i) {
+                int qr = vhip_event_query(ev[oldest % VA_PLAIN_EVENTS]);
+                if (qr == 1) break;
+                if (qr != 0) { rc = qr; va_error("serve_plain", vhip_error_string(qr), qr); goto out; }
+                res[oldest].t_done = va_now() - t0;
+                if (reqs[oldest].out_offset < 0) va_arena_free(&arena, offs[oldest], (size_t)reqs[oldest].rows *
+                                                                (size_t)reqs[oldest].n * 8);
+                oldest++;
+            }
+            if (need && (i == nreq ? oldest < i : i - oldest >= VA_PLAIN_EVENTS)) continue;
+            if (q && va_now() - t0 < q->t_arrive) continue;
+            if (q && q->out_offset < 0) {
+                offs[i] = va_arena_alloc(&arena, len);
+                if (offs[i] == (size_t)-1) {
+                    if (oldest == i) { rc = -1; va_error("serve_plain", "HBM budget exhausted", -1); goto out; }
+                    continue;            /* wait for an earlier request's memory */
+                }
+            }
+            break;
         }
+        if (!q) break;
+        res[i].t_admit = va_now() - t0;
+        out = (q->out_offset < 0) ? (void *)(arena.base + offs[i]) : va_pool_addr(out_pool, q->out_offset);
         va_fake_view(&bi, &vi, va_pool_addr(in_pool, q->in_offset), q->rows, q->n);
         va_fake_view(&bo, &vo, out, q->rows, q->n);
         for (k = 0; k < nplans; k++)
@@ -647,26 +735,29 @@ int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_c
                 plans[k].dir == q->dir && plans[k].scale == (double)q->scale)
                 break;
         rc = va_fftm_exec(&plans[k], &vi, &vo, vhip_stream());
-        if (!rc) rc = vhip_sync();
-        res[i].t_done = va_now() - t0;
+        if (!rc) rc = vhip_event_record(ev[i % VA_PLAIN_EVENTS], vhip_stream());
+        if (!rc && sync_each) rc = vhip_sync();
         res[i].batch = -1;
         res[i].stream = 0;
-        if (q->out_offset < 0) va_arena_free(&arena, off, len);
     }
     tend = va_now() - t0;
     if (!rc && stats && stats_len) {
         char tmp[256];
         double bytes = 0.0;
         for (i = 0; i < nreq; i++) bytes += 2.0 * (double)reqs[i].rows * (double)reqs[i].n * 8.0;
-        sprintf(tmp, "{\"requests\": %ld, \"seconds\": %.6f, \"req_per_s\": %.1f, \"gb_per_s\": %.2f, \"plans\": %d}",
-                nreq, tend, (double)nreq / tend, bytes / tend / 1e9, nplans);
+        sprintf(tmp, "{\"requests\": %ld, \"seconds\": %.6f, \"req_per_s\": %.1f, \"gb_per_s\": %.2f, "
+                "\"plans\": %d, \"sync_each\": %d}",
+                nreq, tend, (double)nreq / tend, bytes / tend / 1e9, nplans, sync_each ? 1 : 0);
         strncpy(stats, tmp, stats_len - 1);
         stats[stats_len - 1] = '\0';
     }
 out:
     vhip_sync();
+    for (e = 0; e < VA_PLAIN_EVENTS; e++)
+        if (ev[e]) vhip_event_destroy(ev[e]);
     for (i = 0; i < nplans; i++) va_fftm_release(&plans[i]);
     free(plans);
+    free(offs);
     va_arena_fini(&arena);
     return rc;
 }

@@ -59,3 +59,46 @@ def real_data(seed, shape, dist="normal", dtype=np.float32):
     n = int(np.prod(shape))
     draw = normal if dist == "normal" else uniform
     return draw(seed, n, dtype).reshape(shape)
+
+
+# ---- the batched and served workloads ----------------------------------------
+
+def c4_lengths(count, seed=4):
+    """`count` lengths drawn log-uniform over the 2^a 3^b 5^c values in
+    [64, 65536] (the nearest such value to each log-uniform draw)."""
+    cands = sorted({2 ** a * 3 ** b * 5 ** c for a in range(17) for b in range(11) for c in range(7)
+                    if 64 <= 2 ** a * 3 ** b * 5 ** c <= 65536})
+    logc = np.log(np.array(cands, dtype=np.float64))
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(np.log(64), np.log(65536), count)
+    return [cands[int(np.argmin(np.abs(logc - v)))] for v in u]
+
+
+def c4_jobs(count=3000, seed=4, total=1 << 27):
+    """(lengths, rows) of the uneven batch: each job's row count makes its
+    share of the `total` complex elements about equal (at least one row)."""
+    lengths = c4_lengths(count, seed)
+    per_job = total / count
+    rows = [max(1, int(round(per_job / n))) for n in lengths]
+    return lengths, rows
+
+
+C5_LENGTHS = (256, 1000, 1024, 2187, 4096, 16384)
+C5_POOL = 512 * 16384 * 2        # complex elements of the device input pool
+
+
+def c5_requests(count=2000, rate=20000.0, seed=5, pool=C5_POOL, lengths=C5_LENGTHS, max_rows=512):
+    """A Poisson request stream at `rate` per second: rows uniform in
+    [1, max_rows], length uniform over `lengths`, forward (scale 1) or inverse
+    (scale 1/n) 50/50, input at a uniform offset of the input pool; tuples in
+    vsip_amd_request field order with out_offset -1 (output from the
+    server's budget)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    reqs = []
+    for i in range(count):
+        n = int(rng.choice(lengths))
+        r = int(rng.integers(1, max_rows + 1))
+        d = -1 if rng.random() < 0.5 else 1
+        reqs.append((float(t[i]), r, n, d, 1.0 if d < 0 else 1.0 / n, int(rng.integers(0, pool - r * n)), -1))
+    return reqs

@@ -259,7 +259,7 @@ class AmdLib(VsipLib):
                                                 ctypes.c_int, ctypes.c_int, ctypes.POINTER(amd_result),
                                                 ctypes.c_char_p, ctypes.c_size_t]),
             "vsip_amd_serve_plain_f": (ctypes.c_int, [ctypes.POINTER(amd_request), ctypes.c_long, P, P,
-                                                      ctypes.c_size_t, ctypes.POINTER(amd_result),
+                                                      ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(amd_result),
                                                       ctypes.c_char_p, ctypes.c_size_t]),
             "vsip_amd_komega_f": (ctypes.c_int, [P, P, P, P, P, P, P, ctypes.c_float]),
         }
@@ -297,10 +297,12 @@ class AmdLib(VsipLib):
         self.amd_tune(prec, n, buf, 1024)
         return json.loads(buf.value.decode())
 
-    def serve(self, reqs, in_pool, out_pool=None, budget=64 << 30, lanes=4, max_jobs=64, plain=False):
+    def serve(self, reqs, in_pool, out_pool=None, budget=64 << 30, lanes=4, max_jobs=64, plain=False,
+              sync_each=True):
         """Run a request stream (list of dicts or tuples in amd_request field
-        order) through the server (or the plain baseline); returns
-        (results as a numpy structured array, stats dict)."""
+        order) through the server (or the plain baseline: plain calls with a
+        synchronisation after each, or back to back with sync_each=False);
+        returns (results as a numpy structured array, stats dict)."""
         import json
         import numpy as np
         n = len(reqs)
@@ -310,7 +312,7 @@ class AmdLib(VsipLib):
         res = (amd_result * n)()
         buf = ctypes.create_string_buffer(1024)
         if plain:
-            rc = self.amd_serve_plain_f(arr, n, in_pool, out_pool, budget, res, buf, 1024)
+            rc = self.amd_serve_plain_f(arr, n, in_pool, out_pool, budget, 1 if sync_each else 0, res, buf, 1024)
         else:
             rc = self.amd_serve_f(arr, n, in_pool, out_pool, budget, lanes, max_jobs, res, buf, 1024)
         if rc:

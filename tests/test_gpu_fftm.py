@@ -368,3 +368,69 @@ def test_real_pow2_strided_rows(amd):
     x0 = np.zeros(m * 2 * n, np.float32)
     xb = fftm.crfftm(amd, z, zv, x0, xv, scale=1.0 / n)
     _check("cr vs oracle", xb, po.crfftm(z, zv, x0, xv, scale=1.0 / n), TOL_F)
+
+
+# ---- FP64 real and out-of-place transforms (VI_rcfftmop_d_def.h, VI_crfftmop_d_def.h,
+# VI_ccfftmop_d_def.h) against the oracle and the reference's _d at 1e-12 -------------
+
+@pytest.mark.parametrize("m,n,major", [(16, 64, VSIP_ROW), (8, 2048, VSIP_ROW), (6, 1000, VSIP_ROW),
+                                       (5, 30, VSIP_ROW), (3, 16384, VSIP_ROW), (60, 40, VSIP_COL),
+                                       (32, 18, VSIP_COL), (2048, 3, VSIP_COL)])
+def test_rcfftmop_d(amd, ref, m, n, major):
+    _case("rcfftmop_d major %d %dx%d scale 0.5" % (major, m, n))
+    x = datagen.real_data(31, (m, n), dtype=np.float64)
+    if major == VSIP_ROW:
+        xv, zv = fftm.dense_view(m, n), fftm.dense_view(m, n // 2 + 1)
+        nz = m * (n // 2 + 1)
+    else:
+        xv, zv = fftm.dense_view(m, n, VSIP_COL), fftm.dense_view(m // 2 + 1, n, VSIP_COL)
+        nz = (m // 2 + 1) * n
+    z0 = np.zeros(nz, np.complex128)
+    z = fftm.rcfftm(amd, x.ravel(), xv, z0, zv, major=major, scale=0.5)
+    _check("vs oracle", z, po.rcfftm(x.ravel(), xv, z0, zv, major=major, scale=0.5), TOL_D)
+    _check("vs jvsip _d", z, fftm.rcfftm(ref, x.ravel(), xv, z0, zv, major=major, scale=0.5), TOL_D)
+
+
+@pytest.mark.parametrize("m,n,major", [(16, 64, VSIP_ROW), (8, 2048, VSIP_ROW), (6, 1000, VSIP_ROW),
+                                       (3, 16384, VSIP_ROW), (60, 40, VSIP_COL), (32, 18, VSIP_COL)])
+def test_crfftmop_d(amd, ref, m, n, major):
+    """non-Hermitian input: the literal pre-combine of VI_crfftmop_d_def.h"""
+    _case("crfftmop_d major %d %dx%d (non-Hermitian input)" % (major, m, n))
+    if major == VSIP_ROW:
+        xv, zv = fftm.dense_view(m, n), fftm.dense_view(m, n // 2 + 1)
+        nz = m * (n // 2 + 1)
+    else:
+        xv, zv = fftm.dense_view(m, n, VSIP_COL), fftm.dense_view(m // 2 + 1, n, VSIP_COL)
+        nz = (m // 2 + 1) * n
+    z = datagen.complex_data(32, (nz,), dtype=np.complex128)
+    x0 = np.zeros(m * n, np.float64)
+    x = fftm.crfftm(amd, z, zv, x0, xv, major=major, scale=0.25)
+    _check("vs oracle", x, po.crfftm(z, zv, x0, xv, major=major, scale=0.25), TOL_D)
+    _check("vs jvsip _d", x, fftm.crfftm(ref, z, zv, x0, xv, major=major, scale=0.25), TOL_D)
+
+
+def test_real_d_roundtrip(amd):
+    m, n = 9, 4096
+    _case("rcfftmop_d then crfftmop_d (scale 1/N) ROW %dx%d" % (m, n))
+    x = datagen.real_data(34, (m, n), dtype=np.float64)
+    xv, zv = fftm.dense_view(m, n), fftm.dense_view(m, n // 2 + 1)
+    z = fftm.rcfftm(amd, x.ravel(), xv, np.zeros(m * (n // 2 + 1), np.complex128), zv)
+    back = fftm.crfftm(amd, z, zv, np.zeros(m * n, np.float64), xv, scale=1.0 / n)
+    _check("round trip", back, x.ravel(), 1e-14)
+
+
+@pytest.mark.parametrize("major,m,n", [(VSIP_ROW, 64, 1001), (VSIP_ROW, 16, 2048), (VSIP_ROW, 8, 2187),
+                                       (VSIP_COL, 300, 40), (VSIP_COL, 1920, 12)])
+def test_ccfftmop_d_strided(amd, ref, major, m, n):
+    """out of place from the left half of a 2N-wide parent, inverse with a scale"""
+    _case("ccfftmop_d major %d %dx%d parent row 2N, inverse scale 1/3" % (major, m, n))
+    x = datagen.complex_data(33, (m * 2 * n,), dtype=np.complex128)
+    xv = fftm.View(0, 2 * n, m, 1, n)
+    y0 = np.zeros(m * n, np.complex128)
+    yv = fftm.dense_view(m, n)
+    xr, y = fftm.ccfftm(amd, x, xv, major=major, direction=VSIP_FFT_INV, scale=1.0 / 3, out_blk=y0, out_view=yv)
+    np.testing.assert_array_equal(xr, x)
+    _check("vs oracle", y, po.ccfftm(x, xv, major=major, direction=VSIP_FFT_INV, scale=1.0 / 3,
+                                     out_blk=y0, out_view=yv), TOL_D)
+    _, yr = fftm.ccfftm(ref, x, xv, major=major, direction=VSIP_FFT_INV, scale=1.0 / 3, out_blk=y0, out_view=yv)
+    _check("vs jvsip _d", y, yr, TOL_D)

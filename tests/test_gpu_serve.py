@@ -126,3 +126,96 @@ def test_serve_error_paths(amd, pools):
     with pytest.raises(RuntimeError):   # no butterfly for 17
         lib.serve([(0.0, 4, 17, -1, 1.0, 0, -1)], inb, None, budget=1 << 26)
     lib.amd_clear_error()   # the library keeps its last error; later tests check it
+
+
+def _requests_two_pass(nreq, seed):
+    """lengths that split into two passes (32768, 65536, 40000 = 2^6 5^4) and
+    single-pass ones, many present at once, on a small input pool"""
+    rng = np.random.default_rng(seed)
+    reqs, out_off = [], 0
+    for i in range(nreq):
+        n = int(rng.choice([32768, 65536, 40000, 4096]))
+        rows = int(rng.integers(1, 9))
+        d = -1 if rng.random() < 0.5 else 1
+        scale = 1.0 if d < 0 else 1.0 / n
+        reqs.append((1e-6 * i, rows, n, d, scale, int(rng.integers(0, POOL - rows * n)), out_off))
+        out_off += rows * n
+    return reqs, out_off
+
+
+@pytest.mark.parametrize("direct_elems", [None, "1000000000"])
+def test_serve_two_pass_lengths_on_four_lanes(amd, pools, monkeypatch, direct_elems):
+    """Requests of one two-pass length run on several lanes at once, each on
+    its own scratch: the mix equals each request alone, the plain calls and
+    the oracle.  With the direct-launch threshold raised, the small ones run
+    in the lanes' batched launches (their two-pass form)."""
+    lib = amd
+    x, inb = pools
+    if direct_elems:
+        monkeypatch.setenv("VSIP_AMD_SERVE_DIRECT", direct_elems)
+    reqs, total = _requests_two_pass(32, seed=11)
+    ob, ov = _out_pool(lib, total)
+    _case("serve two-pass lengths, 4 lanes, direct threshold %s" % (direct_elems or "default"))
+    res, stats = lib.serve(reqs, inb, ob, budget=1 << 30, lanes=4, max_jobs=4)
+    print("  stats", stats, flush=True)
+    assert stats["max_lanes_in_flight"] > 1
+    if direct_elems:
+        assert stats["batched_requests"] > 0
+    mix = _read(lib, ov, total)
+    res_a, _ = lib.serve([(0.0,) + r[1:] for r in reqs], inb, ob, budget=1 << 30, lanes=1, max_jobs=1)
+    alone = _read(lib, ov, total)
+    lib.serve(reqs, inb, ob, budget=1 << 30, plain=True)
+    plain = _read(lib, ov, total)
+    for i, (t, rows, n, d, s, io, oo) in enumerate(reqs):
+        sl = slice(oo, oo + rows * n)
+        want = po.ccfftm(x[io:io + rows * n], fftm.dense_view(rows, n), direction=d, scale=s)
+        e_alone, e_plain, e_or = po.rel_l2(mix[sl], alone[sl]), po.rel_l2(mix[sl], plain[sl]), po.rel_l2(mix[sl], want)
+        print("  req %2d %dx%-6d dir %+d: alone %.1e plain %.1e oracle %.1e" % (i, rows, n, d, e_alone, e_plain, e_or),
+              flush=True)
+        assert e_alone <= 1e-6 and e_plain <= 1e-6 and e_or <= 1e-5, i
+    lib.cmalldestroy_f(ov)
+
+
+def test_serve_request_beyond_a_lane_runs_directly(amd, monkeypatch):
+    """A request whose batched work items exceed one lane's item table
+    (max_jobs * 1200; one 8000-point row per item here) launches its own
+    kernel instead of waiting for a lane that could never hold it."""
+    lib = amd
+    monkeypatch.setenv("VSIP_AMD_SERVE_DIRECT", "1000000000")
+    rows, n = 1300, 8000
+    big = rows * n + 3000
+    x = datagen.complex_data(15, (big,))
+    inb = lib.cblockcreate_f(big, 0)
+    inv = lib.cmbind_f(inb, 0, big, 1, 1, big)
+    lib.cmcopyfrom_user_f(x.ctypes.data, None, VSIP_ROW, inv)
+    reqs = [(0.0, rows, n, -1, 1.0, 0, 0), (0.0, 3, 1000, 1, 1e-3, 7, rows * n)]
+    ob, ov = _out_pool(lib, big)
+    _case("serve %dx%d with max_jobs 1" % (rows, n))
+    res, stats = lib.serve(reqs, inb, ob, budget=1 << 30, lanes=2, max_jobs=1)
+    print("  stats", stats, flush=True)
+    assert stats["direct_launched"] >= 1
+    mix = _read(lib, ov, big)
+    for t, r, nn, d, s, io, oo in reqs:
+        want = po.ccfftm(x[io:io + r * nn], fftm.dense_view(r, nn), direction=d, scale=s)
+        assert po.rel_l2(mix[oo:oo + r * nn], want) <= 1e-5
+    lib.cmalldestroy_f(ov)
+    lib.cmalldestroy_f(inv)
+
+
+def test_serve_plain_back_to_back(amd, pools):
+    """The plain baseline without a synchronisation per call gives the same
+    results and completion times after the arrivals."""
+    lib = amd
+    x, inb = pools
+    reqs, total = _requests(40, rate=1e6, seed=9, max_rows=32)
+    ob, ov = _out_pool(lib, total)
+    _case("plain calls back to back")
+    res, stats = lib.serve(reqs, inb, ob, budget=1 << 30, plain=True, sync_each=False)
+    print("  stats", stats, flush=True)
+    assert stats["sync_each"] == 0
+    assert np.all(res["t_done"] >= res["t_admit"]) and np.all(np.diff(res["t_done"]) >= 0)
+    got = _read(lib, ov, total)
+    for t, rows, n, d, s, io, oo in reqs:
+        want = po.ccfftm(x[io:io + rows * n], fftm.dense_view(rows, n), direction=d, scale=s)
+        assert po.rel_l2(got[oo:oo + rows * n], want) <= 1e-5
+    lib.cmalldestroy_f(ov)
