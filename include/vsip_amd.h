@@ -72,6 +72,22 @@ vsip_amd_batch_f *vsip_amd_batch_create_f(vsip_length njobs, vsip_fftm_f *const 
                                           const vsip_cmview_f *const *outs);
 int vsip_amd_batch_run_f(vsip_amd_batch_f *b);
 void vsip_amd_batch_destroy_f(vsip_amd_batch_f *b);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the transforms of vsip_amd_batch_create_f +
+ * vsip_amd_batch_run_f in one call that allocates nothing.  The launch
+ * descriptors, work items and counters live in a static device scratch of
+ * 4 MiB per GPU (held until the process ends), so a call takes at most
+ * (4 MiB - 256 B) / (464 B per job + 16 B per work item) of them.  Every
+ * job must be a complex FP32 ccfftmip/ccfftmop plan whose length is single
+ * pass in the batched shape and already tuned (vsip_amd_tune, or a
+ * vsip_amd_batch_create_f over it, tunes and allocates once beforehand).
+ * Runs asynchronously on the library stream.  Calls share the scratch and
+ * are ordered by the library stream; calls from several host threads at once
+ * race.  Returns 0, or -1 with the reason in vsip_amd_last_error (sizes
+ * beyond the scratch, an untuned or two-pass length) before any launch. */
+int vsip_amd_batch_run_noalloc_f(vsip_length njobs, vsip_fftm_f *const *plans,
+                                 const vsip_cmview_f *const *ins, const vsip_cmview_f *const *outs);
 /* JSON: jobs, descriptors, work items, grid, LDS bytes, elements */
 int vsip_amd_batch_info_f(const vsip_amd_batch_f *b, char *buf, size_t len);
 /* the batched schedule the autotuner picks for length n (tunes on first use;

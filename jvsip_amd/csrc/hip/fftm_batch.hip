@@ -99,6 +99,72 @@ int batch_set_lds_limit(int bytes) {
 
 }  // namespace vhipk
 
+// ---- the no-allocation batched entry point's scratch ------------------------
+//
+// One file-scope device array per GPU holds the control counters, the launch
+// descriptors and the work items of vsip_amd_batch_run_noalloc_f.  Host code
+// never uses the array's name as an address (in host code it names a host-side
+// stand-in): at set-up, with each device current, hipGetSymbolAddress gives
+// that device's own copy, and only that address is passed to kernels and
+// copies.  Two events per device order the reuse of the host staging halves.
+static __device__ __align__(256) unsigned char scratch[VHIP_NOALLOC_BYTES];
+
+namespace {
+constexpr int kMaxDev = 64;
+struct NoallocDev {
+    void *ptr;
+    hipEvent_t ev[2];
+    int recorded[2];
+    int ready;
+};
+NoallocDev g_na[kMaxDev];
+int g_na_dev = -1;   // the device the library's stream runs on
+}  // namespace
+
+extern "C" int vhip_noalloc_setup(void) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= kMaxDev) return (int)hipErrorInvalidDevice;
+    NoallocDev &d = g_na[dev];
+    if (!d.ready) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (p == nullptr || ((size_t)p & 255) != 0) return (int)hipErrorInvalidSymbol;
+        for (int i = 0; i < 2; ++i) {
+            e = hipEventCreateWithFlags(&d.ev[i], hipEventDisableTiming);
+            if (e != hipSuccess) return (int)e;
+            d.recorded[i] = 0;
+        }
+        d.ptr = p;
+        d.ready = 1;
+    }
+    if (g_na_dev < 0) g_na_dev = dev;
+    return 0;
+}
+
+extern "C" int vhip_noalloc_scratch(void **ptr) {
+    if (g_na_dev < 0 || !g_na[g_na_dev].ready) return (int)hipErrorNotInitialized;
+    *ptr = g_na[g_na_dev].ptr;
+    return 0;
+}
+
+extern "C" int vhip_noalloc_stage_wait(int slot) {
+    if (g_na_dev < 0 || slot < 0 || slot > 1) return (int)hipErrorInvalidValue;
+    NoallocDev &d = g_na[g_na_dev];
+    if (!d.recorded[slot]) return 0;
+    return (int)hipEventSynchronize(d.ev[slot]);
+}
+
+extern "C" int vhip_noalloc_stage_done(int slot, void *stream) {
+    if (g_na_dev < 0 || slot < 0 || slot > 1) return (int)hipErrorInvalidValue;
+    NoallocDev &d = g_na[g_na_dev];
+    hipError_t e = hipEventRecord(d.ev[slot], (hipStream_t)stream);
+    if (e == hipSuccess) d.recorded[slot] = 1;
+    return (int)e;
+}
+
 extern "C" int vhip_batch_max_grid(int threads, int maxe, int lds_bytes, int *grid) {
     int per_cu = 0, cus = 0, dev = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(

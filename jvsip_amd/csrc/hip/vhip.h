@@ -270,6 +270,20 @@ int vhip_batch_launch(const vhip_fftm_desc *descs, const vhip_batch_item *items,
                       int grid, void *stream);
 int vhip_batch_max_grid(int threads, int maxe, int lds_bytes, int *grid);
 
+/* The no-allocation batched entry point's scratch: a static device array of
+ * VHIP_NOALLOC_BYTES per GPU (control counters at offset 0, descriptors and
+ * work items from VHIP_NOALLOC_HEAD on).  vhip_noalloc_setup (called by
+ * vhip_init with the library's device current) looks up this device's copy;
+ * vhip_noalloc_scratch returns it.  The stage calls order the reuse of the
+ * caller's two host staging halves: wait before refilling half `slot`, mark
+ * it after queueing its upload on `stream`. */
+#define VHIP_NOALLOC_BYTES (4u << 20)
+#define VHIP_NOALLOC_HEAD 256u
+int vhip_noalloc_setup(void);
+int vhip_noalloc_scratch(void **ptr);
+int vhip_noalloc_stage_wait(int slot);
+int vhip_noalloc_stage_done(int slot, void *stream);
+
 /* graphs: capture everything issued to `stream` between begin and end */
 int vhip_stream_create(void **stream);
 int vhip_stream_destroy(void *stream);

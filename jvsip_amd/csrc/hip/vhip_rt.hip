@@ -45,6 +45,8 @@ extern "C" int vhip_init(void) {
     if (e != hipSuccess) return (int)e;
     int rc = vhip_fftm_set_lds_limit(g_lds_max);
     if (rc != 0) return rc;
+    rc = vhip_noalloc_setup();         // this device's no-allocation scratch
+    if (rc != 0) return rc;
     const char *b = getenv("VSIP_AMD_HBM_BUDGET");
     if (b) g_budget.store((size_t)strtoull(b, nullptr, 10));
     g_ready = true;

@@ -249,3 +249,134 @@ int vsip_amd_batch_info_f(const vsip_amd_batch_f *b, char *buf, size_t len)
     }
     return pos;
 }
+
+/* ---- one-shot batch that allocates nothing -------------------------------
+ *
+ * The same persistent batched kernel as a created batch, for callers that may
+ * not allocate (code under graph capture, tight serving loops): the launch
+ * descriptors, work items and control counters go to the library's static
+ * device scratch (VHIP_NOALLOC_BYTES per GPU, looked up at library set-up)
+ * instead of memory the library allocates.  They are built in one of two
+ * static host staging halves, uploaded with one copy on the library stream,
+ * and the kernel drains them after a small kernel has zeroed the counters.
+ * Every job must be single pass in the batched shape with its schedule
+ * already in the autotuner's cache (a lookup that misses would tune, and
+ * tuning allocates).  All calls share the device scratch: they are ordered
+ * on the library stream, and calls from several host threads at once race.
+ */
+typedef union {
+    double align_;
+    char b[VHIP_NOALLOC_BYTES];
+} VA_stage;
+static VA_stage va_na_stage[2];
+static int va_na_turn = 0;
+static int va_na_grid = 0;
+
+/* items sorted longest transform first, then by descriptor and first vector
+ * (pad_ carries the length while sorting); a shell sort, which needs no memory */
+static int va_na_before(const vhip_batch_item *a, const vhip_batch_item *b)
+{
+    if (a->pad_ != b->pad_) return a->pad_ > b->pad_;
+    if (a->desc != b->desc) return a->desc < b->desc;
+    return a->vbase < b->vbase;
+}
+
+static void va_na_sort(vhip_batch_item *it, long n)
+{
+    static const long gaps[] = {88573, 29524, 9841, 3280, 1093, 364, 121, 40, 13, 4, 1};
+    int g;
+    for (g = 0; g < (int)(sizeof(gaps) / sizeof(gaps[0])); g++) {
+        long h = gaps[g], i, j;
+        if (h >= n) continue;
+        for (i = h; i < n; i++) {
+            vhip_batch_item t = it[i];
+            for (j = i; j >= h && va_na_before(&t, &it[j - h]); j -= h) it[j] = it[j - h];
+            it[j] = t;
+        }
+    }
+}
+
+int vsip_amd_batch_run_noalloc_f(vsip_length njobs, vsip_fftm_f *const *plans,
+                                 const vsip_cmview_f *const *ins, const vsip_cmview_f *const *outs)
+{
+    char *dev = NULL, *host;
+    vhip_fftm_desc *descs;
+    vhip_batch_item *items;
+    size_t max_desc, used;
+    long ni = 0, j, cap_items;
+    int slot, rc;
+    if (va_ensure_init()) return -1;
+    if (njobs == 0) return 0;
+    if (!plans || !ins) { va_error("batch_noalloc", "no plans or input views", -1); return -1; }
+    rc = vhip_noalloc_scratch((void **)&dev);
+    if (rc) { va_error("batch_noalloc", "no device scratch", rc); return -1; }
+    max_desc = (VHIP_NOALLOC_BYTES - VHIP_NOALLOC_HEAD) / sizeof(vhip_fftm_desc);
+    if (njobs > max_desc) {
+        va_error("batch_noalloc", "more jobs than the scratch holds descriptors for", (int)max_desc);
+        return -1;
+    }
+    if (!va_na_grid) {
+        rc = vhip_batch_max_grid(VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS, &va_na_grid);
+        if (rc) { va_na_grid = 0; va_error("batch_noalloc", vhip_error_string(rc), rc); return -1; }
+    }
+    /* this staging half was last uploaded two calls ago: wait for that copy */
+    slot = va_na_turn;
+    rc = vhip_noalloc_stage_wait(slot);
+    if (rc) { va_error("batch_noalloc", vhip_error_string(rc), rc); return -1; }
+    host = va_na_stage[slot].b;
+    descs = (vhip_fftm_desc *)(host + VHIP_NOALLOC_HEAD);
+    items = (vhip_batch_item *)(descs + njobs);
+    cap_items = (long)((VHIP_NOALLOC_BYTES - VHIP_NOALLOC_HEAD - njobs * sizeof(vhip_fftm_desc)) /
+                       sizeof(vhip_batch_item));
+    for (j = 0; j < (long)njobs; j++) {
+        const VA_fftm *p;
+        const VA_mview *in, *out;
+        const VA_sched *s = NULL;
+        long g, ng;
+        int t;
+        if (!plans[j] || !ins[j]) { va_error("batch_noalloc", "missing plan or view", (int)j); return -1; }
+        p = &plans[j]->p;
+        in = &ins[j]->v;
+        out = (outs && outs[j]) ? &outs[j]->v : in;
+        if (p->markings != VA_VALID || p->prec != 4 || (p->type != VA_CCFFTIP && p->type != VA_CCFFTOP)) {
+            va_error("batch_noalloc", "only complex FP32 ccfftmip/ccfftmop plans batch", (int)j);
+            return -1;
+        }
+        t = va_tune_lookup(4, (int)p->MN, &s);
+        if (t == -4) {
+            va_error("batch_noalloc", "length has no cached batched schedule (vsip_amd_tune it first)", (int)p->MN);
+            return -1;
+        }
+        if (t) {
+            va_error("batch_noalloc", "length is not single pass in the batched shape", (int)p->MN);
+            return -1;
+        }
+        if (va_fftm_descs(p, in, out, s, NULL, NULL, NULL, &descs[j]) != 1) return -1;
+        ng = (descs[j].nvec + descs[j].vpb - 1) / descs[j].vpb;
+        if (ni + ng > cap_items) {
+            va_error("batch_noalloc", "more work items than the scratch holds", (int)cap_items);
+            return -1;
+        }
+        for (g = 0; g < ng; g++) {
+            items[ni].desc = (int)j;
+            items[ni].pad_ = (int)p->MN;
+            items[ni].vbase = g * descs[j].vpb;
+            ni++;
+        }
+    }
+    va_na_sort(items, ni);
+    for (j = 0; j < ni; j++) items[j].pad_ = 0;
+    used = njobs * sizeof(vhip_fftm_desc) + (size_t)ni * sizeof(vhip_batch_item);
+    rc = vhip_memcpy_h2d_async(dev + VHIP_NOALLOC_HEAD, host + VHIP_NOALLOC_HEAD, used, vhip_stream());
+    if (!rc) rc = vhip_noalloc_stage_done(slot, vhip_stream());
+    va_na_turn ^= 1;
+    /* the counters start at zero: cleared by a kernel, not a memset node */
+    if (!rc) rc = vhip_batch_clear((unsigned *)dev, vhip_stream());
+    if (!rc && ni)
+        rc = vhip_batch_launch((const vhip_fftm_desc *)(dev + VHIP_NOALLOC_HEAD),
+                               (const vhip_batch_item *)(dev + VHIP_NOALLOC_HEAD + njobs * sizeof(vhip_fftm_desc)),
+                               ni, 0, (unsigned *)dev, VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS, va_na_grid,
+                               vhip_stream());
+    if (rc) va_error("batch_noalloc", vhip_error_string(rc), rc);
+    return rc;
+}

@@ -192,6 +192,7 @@ int va_sched_upload(VA_sched *s, int prec);
 int va_factor_variant(int n, int prec, int maxbits, int odd_first, int *radix);
 int va_factor_pack(int n, int *radix);
 int va_tune_get(int prec, int n, const VA_sched **out);
+int va_tune_lookup(int prec, int n, const VA_sched **out);
 void va_sched_free(VA_sched *s);
 int va_fftm_init(VA_fftm *p, int prec, int type, vsip_length M, vsip_length N, double scale,
                  int dir, vsip_major major, unsigned int ntimes, vsip_alg_hint hint);

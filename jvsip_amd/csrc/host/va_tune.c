@@ -257,6 +257,25 @@ int va_tune_get(int prec, int n, const VA_sched **out)
     return 0;
 }
 
+/*
+ * The cached batched schedule of length n, without tuning or uploading
+ * anything: 0 and *out, -2 cached as not fitting one workgroup, -1 cached as
+ * unsupported, -4 not in the cache yet.
+ */
+int va_tune_lookup(int prec, int n, const VA_sched **out)
+{
+    int i;
+    *out = NULL;
+    for (i = 0; va_tuned_tab && i < va_tuned_n; i++) {
+        if (va_tuned_tab[i].prec == prec && va_tuned_tab[i].n == n) {
+            if (va_tuned_tab[i].rc) return va_tuned_tab[i].rc;
+            *out = &va_tuned_tab[i].s;
+            return 0;
+        }
+    }
+    return -4;
+}
+
 /* describe the tuned schedule of length n as JSON */
 int vsip_amd_tune(int prec, long n, char *buf, size_t len)
 {

@@ -250,6 +250,8 @@ class AmdLib(VsipLib):
             "vsip_amd_fftm_describe_d": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
             "vsip_amd_batch_create_f": (P, [c_vi, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P)]),
             "vsip_amd_batch_run_f": (ctypes.c_int, [P]),
+            "vsip_amd_batch_run_noalloc_f": (ctypes.c_int, [c_vi, ctypes.POINTER(P), ctypes.POINTER(P),
+                                                            ctypes.POINTER(P)]),
             "vsip_amd_batch_destroy_f": (None, [P]),
             "vsip_amd_batch_info_f": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
             "vsip_amd_hbm_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
@@ -284,6 +286,12 @@ class AmdLib(VsipLib):
         n = len(plans)
         arr = lambda xs: (P * n)(*xs)
         return self.amd_batch_create_f(n, arr(plans), arr(ins), arr(outs) if outs else None)
+
+    def batch_noalloc(self, plans, ins, outs=None):
+        """vsip_amd_batch_run_noalloc_f over Python lists of handles."""
+        n = len(plans)
+        arr = lambda xs: (P * n)(*xs)
+        return self.amd_batch_run_noalloc_f(n, arr(plans), arr(ins), arr(outs) if outs else None)
 
     def batch_info(self, b):
         import json
