@@ -292,6 +292,24 @@ def _time_calls(lib, hip, fn, reps, warm_s=0.03):
     return wall, hip.elapsed_ms(e0, e1) * 1e-3 / reps
 
 
+_CEIL = {}
+
+
+def size_ceiling(lib, bytes_moved):
+    """The in-place read+write probe's rate at the same bytes per launch
+    (4 buffers in rotation, back-to-back launches timed with stream events):
+    what any single launch moving this many bytes reaches on this device,
+    launch overhead included.  Small transforms are bounded by it rather than
+    by the 8 TB/s peak."""
+    half = max(32768, (int(bytes_moved) // 2) // 32768 * 32768)
+    if half not in _CEIL:
+        rmw, rd = ctypes.c_double(), ctypes.c_double()
+        lib.amd_hbm_probe(half, 40, ctypes.byref(rmw), ctypes.byref(rd))
+        lib.amd_hbm_probe(half, 40, ctypes.byref(rmw), ctypes.byref(rd))
+        _CEIL[half] = rmw.value
+    return round(_CEIL[half], 1)
+
+
 def config_c1(lib, hip):
     """plain vsip_ccfftmip_f forward, ROW, 64 x 1024, scale 1, N(0,1) seed 1"""
     from jvsip_amd import datagen, fftm
@@ -308,14 +326,32 @@ def config_c1(lib, hip):
     y = np.empty((64, 1024), np.complex64)
     lib.cmcopyto_user_f(v, 0, y.ctypes.data, None)
     f = np.load(os.path.join(ROOT, "tests", "golden", "jvsip_c1_64x1024.npz"))
-    out = {"shape": "64x1024", "us_per_call_back_to_back": round(wall * 1e6, 2),
+    out = {"shape": "64x1024", "us_per_call_back_to_back_python": round(wall * 1e6, 2),
            "us_per_call_kernel_stream": round(gpu * 1e6, 2), "us_call_plus_sync": round(lat * 1e6, 2),
            "gb_per_s": round(2 * 64 * 1024 * 8 / gpu / 1e9, 1),
+           "size_matched_ceiling_gb_per_s": size_ceiling(lib, 2 * 64 * 1024 * 8),
            "rel_l2_vs_jvsip": rel_l2(y, f["out"]),
            "rel_l2_vs_fp64": rel_l2(y, fft64(x, axis=1))}
     lib.fftm_destroy_f(p)
     lib.cmalldestroy_f(v)
+    out["c_host_per_call"] = percall_c()
     return out
+
+
+def percall_c():
+    """The host cost of a plain call from C (jvsip_amd/lib/bench_percall, a
+    child process): vsip_ccfftmip_f 64 x 1024 back to back against launches
+    of an empty kernel with a 464-byte argument on the same stream."""
+    import subprocess
+    exe = os.path.join(ROOT, "jvsip_amd", "lib", "bench_percall")
+    if not os.path.exists(exe):
+        return None
+    r = subprocess.run([exe, "4000"], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        return {"error": "exit %d: %s" % (r.returncode, r.stderr[-200:])}
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    d["call_minus_empty_launch_us"] = round(d["ccfftmip_f_64x1024_host_us"] - d["empty_kernel_host_us"], 3)
+    return d
 
 
 def config_host_buffers(lib, reps=3):
@@ -471,7 +507,10 @@ def config_c3(lib, hip):
         xs = fftm.gather(x, v)
         sel = f["rows"]
         want = fft64(xs[sel] if major == 0 else xs[:, sel].T, axis=1)
+        ceil = size_ceiling(lib, 2 * m * n * 8)
         out[name] = {"us": round(t * 1e6, 2), "gb_per_s": round(2 * m * n * 8 / t / 1e9, 1),
+                     "size_matched_ceiling_gb_per_s": ceil,
+                     "frac_of_size_matched_ceiling": round(2 * m * n * 8 / t / 1e9 / ceil, 3) if ceil else None,
                      "rel_l2_vs_jvsip": rel_l2(vecs[sel], f["out_rows"]),
                      "rel_l2_vs_fp64": rel_l2(vecs[sel], want),
                      "schedule": json.loads(lib.describe(p))["passes"][0]["radix"]}

@@ -37,11 +37,20 @@ static int p2_pipe_min(int dflt) {
     return env > 0 ? env : dflt;
 }
 
-template <bool PLAIN>
+template <bool PLAIN, bool DB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
     pow2_pipe_kernel(const vhip_fftm_desc d) {
-    __shared__ __attribute__((aligned(16))) cpx<float> lds[kPipeLdsv];
-    pow2_pipe_body<PLAIN>(d, lds);
+    __shared__ __attribute__((aligned(16))) cpx<float> lds[(DB ? 2 : 1) * kPipeLdsv];
+    pow2_pipe_body<PLAIN, DB>(d, lds);
+}
+
+// the 4096-point kernel with cross-lane (DPP) exchanges (device code in
+// fftm_pow2.hpp): one LDS transpose per row
+template <bool PLAIN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    pow2_dpp_kernel(const vhip_fftm_desc d) {
+    __shared__ __attribute__((aligned(16))) cpx<float> lds[2 * kDppLds];
+    pow2_dpp_body<PLAIN>(d, lds);
 }
 
 // 4096-point rows on the pipelined kernel when every workgroup of a grid of
@@ -53,10 +62,22 @@ int p2_pipe_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     const long grid = 2L * cus;
     if (off || d->nvec < p2_pipe_min(4) * grid || d->nvec > 0x7fffffffL) return -100;
     if (d->in.vs0 < 4096 || d->out.vs0 < 4096) return -100;   // rows must not overlap
+    // two LDS rows per workgroup (stages 0 and 1 write different rows, two
+    // barriers per row instead of four); VSIP_AMD_P2_PIPE_DB=0 keeps one
+    static const bool db = !(getenv("VSIP_AMD_P2_PIPE_DB") && atoi(getenv("VSIP_AMD_P2_PIPE_DB")) == 0);
     const bool plain = !d->conj_in && !d->conj_out && d->scale == 1.0;
     hipStream_t st = (hipStream_t)stream;
-    if (plain) hipLaunchKernelGGL((pow2_pipe_kernel<true>), dim3((unsigned)grid), dim3(256), 0, st, *d);
-    else hipLaunchKernelGGL((pow2_pipe_kernel<false>), dim3((unsigned)grid), dim3(256), 0, st, *d);
+    // VSIP_AMD_P2_DPP=1: the kernel with cross-lane exchanges (read per call)
+    const char *dpp = getenv("VSIP_AMD_P2_DPP");
+    if (dpp && atoi(dpp) == 1) {
+        if (plain) hipLaunchKernelGGL((pow2_dpp_kernel<true>), dim3((unsigned)grid), dim3(256), 0, st, *d);
+        else hipLaunchKernelGGL((pow2_dpp_kernel<false>), dim3((unsigned)grid), dim3(256), 0, st, *d);
+        return (int)hipGetLastError();
+    }
+    if (plain && db) hipLaunchKernelGGL((pow2_pipe_kernel<true, true>), dim3((unsigned)grid), dim3(256), 0, st, *d);
+    else if (db) hipLaunchKernelGGL((pow2_pipe_kernel<false, true>), dim3((unsigned)grid), dim3(256), 0, st, *d);
+    else if (plain) hipLaunchKernelGGL((pow2_pipe_kernel<true, false>), dim3((unsigned)grid), dim3(256), 0, st, *d);
+    else hipLaunchKernelGGL((pow2_pipe_kernel<false, false>), dim3((unsigned)grid), dim3(256), 0, st, *d);
     return (int)hipGetLastError();
 }
 

@@ -253,8 +253,11 @@ __device__ __forceinline__ void pow2_body(const vhip_fftm_desc &d, int per_wg, c
 constexpr int kPipePS = 4;
 constexpr int kPipeLdsv = P2<12>::ldsv<kPipePS>();   // > padx<kPipePS>(4095), odd
 
-// stage 0 of the 4096-point row from registers: DFT, lane t's outputs to
-// elements 16 t + r
+// Stage 0 of the 4096-point row from registers: DFT, lane t's outputs to
+// elements 16 t + r.  With a double buffer (DB) stage 0 writes the first LDS
+// row and stage 1 the second, so no stage waits for the other lanes to finish
+// reading before it writes: the row's two barriers (after each write) also
+// order the next row's writes after this row's reads.
 __device__ __forceinline__ void p2p_stage0(cpx<float> (&x)[16], cpx<float> *lds) {
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
@@ -265,10 +268,11 @@ __device__ __forceinline__ void p2p_stage0(cpx<float> (&x)[16], cpx<float> *lds)
     __syncthreads();
 }
 
-// stage 1 or 2 of the 4096-point row with the twiddles w[r-1] of this lane
-template <int S, bool PLAIN>
-__device__ __forceinline__ void p2p_stage(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> (&w)[15],
-                                          rsrc_t out, float osgn, float scale) {
+// stage 1 or 2 of the 4096-point row with the twiddles w[r-1] of this lane,
+// reading LDS row `in` and (stage 1) writing `out_lds`
+template <int S, bool PLAIN, bool DB = false>
+__device__ __forceinline__ void p2p_stage(cpx<float> (&x)[16], const cpx<float> *in, cpx<float> *out_lds,
+                                          const cpx<float> (&w)[15], rsrc_t out, float osgn, float scale) {
     constexpr int R = 16, NB = 256;
     constexpr int LNS = 4 * S, NS = 1 << LNS;
     constexpr bool LAST = (S == 2);
@@ -276,10 +280,10 @@ __device__ __forceinline__ void p2p_stage(cpx<float> (&x)[16], cpx<float> *lds, 
     asm volatile("" : "+v"(t));
     const int j = t;
     {
-        const cpx<float> *src = lds + padx<kPipePS>(j);
+        const cpx<float> *src = in + padx<kPipePS>(j);
 #pragma unroll
         for (int r = 0; r < R; ++r) x[r] = src[padx<kPipePS>(r * NB)];
-        __syncthreads();
+        if (!DB) __syncthreads();
     }
 #pragma unroll
     for (int r = 1; r < R; ++r) x[r] = cmul(x[r], w[r - 1]);
@@ -293,7 +297,7 @@ __device__ __forceinline__ void p2p_stage(cpx<float> (&x)[16], cpx<float> *lds, 
         }
     } else {
         const int base = ((j >> LNS) << (LNS + 4)) + (j & (NS - 1));
-        cpx<float> *dst = lds + padx<kPipePS>(base);
+        cpx<float> *dst = out_lds + padx<kPipePS>(base);
 #pragma unroll
         for (int r = 0; r < R; ++r) dst[padx<kPipePS>(r * NS)] = x[r];
         __syncthreads();
@@ -301,10 +305,11 @@ __device__ __forceinline__ void p2p_stage(cpx<float> (&x)[16], cpx<float> *lds, 
 }
 
 // one 4096-point row held in x (stage 0 from registers, stages 1-2 as above)
-template <bool PLAIN>
+template <bool PLAIN, bool DB>
 __device__ __forceinline__ void p2p_row(cpx<float> (&x)[16], cpx<float> *lds, const cpx<float> (&w1)[15],
                                         const cpx<float> (&w2)[15], rsrc_t out, float isgn, float osgn,
                                         float scale) {
+    cpx<float> *lds2 = DB ? lds + kPipeLdsv : lds;
     // the input conjugation is applied here rather than at the load, which
     // would make the prefetch of the next row wait for its data
     if (!PLAIN) {
@@ -312,12 +317,12 @@ __device__ __forceinline__ void p2p_row(cpx<float> (&x)[16], cpx<float> *lds, co
         for (int e = 0; e < 16; ++e) x[e].y *= isgn;
     }
     p2p_stage0(x, lds);
-    p2p_stage<1, PLAIN>(x, lds, w1, out, osgn, scale);
-    p2p_stage<2, PLAIN>(x, lds, w2, out, osgn, scale);
+    p2p_stage<1, PLAIN, DB>(x, lds, lds2, w1, out, osgn, scale);
+    p2p_stage<2, PLAIN, DB>(x, lds2, lds2, w2, out, osgn, scale);
 }
 
-// lds: kPipeLdsv complex elements
-template <bool PLAIN>
+// lds: kPipeLdsv complex elements (twice that with the double buffer DB)
+template <bool PLAIN, bool DB = false>
 __device__ __forceinline__ void pow2_pipe_body(const vhip_fftm_desc &d, cpx<float> *lds) {
     constexpr int N = 4096;
     const cpx<float> *tw = (const cpx<float> *)d.tw;
@@ -349,19 +354,166 @@ __device__ __forceinline__ void pow2_pipe_body(const vhip_fftm_desc &d, cpx<floa
     // accesses, on every path into it
     p2_load<12, 1, true, true>(a, rin(g), ivs, 1.f);
     p2_load<12, 1, true, true>(b, rin(g + G), ivs, 1.f);
-    p2p_row<PLAIN>(a, lds, w1, w2, make_rsrc(out + (long)g * ovs, (unsigned)(N * 8)), isgn, osgn, scale);
+    p2p_row<PLAIN, DB>(a, lds, w1, w2, make_rsrc(out + (long)g * ovs, (unsigned)(N * 8)), isgn, osgn, scale);
     g += G;
     if (g >= nvec) return;
     p2_load<12, 1, true, true>(a, rin(g + G), ivs, 1.f);
     for (;;) {
-        p2p_row<PLAIN>(b, lds, w1, w2, make_rsrc(out + (long)g * ovs, (unsigned)(N * 8)), isgn, osgn, scale);
+        p2p_row<PLAIN, DB>(b, lds, w1, w2, make_rsrc(out + (long)g * ovs, (unsigned)(N * 8)), isgn, osgn, scale);
         g += G;
         if (g >= nvec) break;
         p2_load<12, 1, true, true>(b, rin(g + G), ivs, 1.f);
-        p2p_row<PLAIN>(a, lds, w1, w2, make_rsrc(out + (long)g * ovs, (unsigned)(N * 8)), isgn, osgn, scale);
+        p2p_row<PLAIN, DB>(a, lds, w1, w2, make_rsrc(out + (long)g * ovs, (unsigned)(N * 8)), isgn, osgn, scale);
         g += G;
         if (g >= nvec) break;
         p2_load<12, 1, true, true>(a, rin(g + G), ivs, 1.f);
+    }
+}
+
+// ---- 4096-point rows with cross-lane (DPP) exchanges ----------------------
+//
+// 4096 = 64 x 64.  Each 64-point transform is held by a quad of 4 adjacent
+// lanes, 16 elements per lane: a 16-point DFT in registers, the twiddles
+// W_64^{q k1} of the lane's quad position q, then a 4-point DFT across the
+// quad as two radix-2 butterflies whose partner values come from DPP
+// quad_perm moves (lane q ^ 2, then q ^ 1) -- no LDS.  Lane q ends with the
+// outputs k1 + 16 brev(q) (brev = 0, 2, 1, 3).  A row takes two such
+// super-stages:
+//   1. lane (g, q) = (L >> 2, L & 3) holds x[g + 64 (q + 4 m)], m = 0..15
+//      (each load instruction reads four whole 128-byte lines); after the
+//      quad DFT it holds S[g][k], k = k1 + 16 brev(q), times W_4096^{g k};
+//   2. after one LDS transpose, lane (h, p) holds S[p + 4 m][h] and the same
+//      steps give X[h + 64 (j1 + 16 brev(p))] (whole 128-byte lines again).
+// One LDS round trip per row instead of two, and one barrier: the rows
+// alternate between two LDS buffers, so a row's barrier also orders the next
+// row's writes after the reads of the row before.  The per-lane twiddles are
+// computed once in double precision (sincospi) and rounded to float.
+// LDS layout of the transpose: element (k, g) at 68 k + g + 8 (k >> 4), which
+// spreads both the write (g contiguous, four k blocks) and the read (h = k
+// across lanes) over all banks.
+constexpr int kDppLds = 68 * 63 + 63 + 8 * 3 + 1;   // elements per buffer
+
+__device__ __forceinline__ constexpr int dpp_a(int k, int g) { return 68 * k + g + 8 * (k >> 4); }
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+// forward 4-point DFT over the 4 lanes of each quad, for 16 registers; lane
+// q = lane & 3 keeps output brev(q)
+__device__ __forceinline__ void quad_dft4(cpx<float> (&x)[16], int q) {
+    const float sa = (q < 2) ? 1.f : -1.f;      // lanes 0,1: y + y[q^2]; lanes 2,3: y[q^2] - y
+    const float sb = (q & 1) ? -1.f : 1.f;      // even: t + t[q^1]; odd: t[q^1] - t
+    const bool rot = (q == 3);                  // times -i (W_4^1) before the second butterfly
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const float px = dpp_f<0x4E>(x[r].x), py = dpp_f<0x4E>(x[r].y);
+        float tx = px + sa * x[r].x, ty = py + sa * x[r].y;
+        const float ux = rot ? ty : tx, uy = rot ? -tx : ty;
+        const float qx = dpp_f<0xB1>(ux), qy = dpp_f<0xB1>(uy);
+        x[r] = {qx + sb * ux, qy + sb * uy};
+    }
+}
+
+template <bool PLAIN>
+__device__ __forceinline__ void p2d_row(cpx<float> (&x)[16], cpx<float> *buf, const cpx<float> (&wa)[16],
+                                        const cpx<float> (&wb)[16], rsrc_t out, int g, int q, int bq, float isgn,
+                                        float osgn, float scale) {
+    if (!PLAIN) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) x[e].y *= isgn;
+    }
+    // super-stage 1: DFT over n_hi = q + 4 m
+    Dft<float, 16>::run(x);
+#pragma unroll
+    for (int k1 = 1; k1 < 16; ++k1) x[k1] = cmul(x[k1], wa[k1]);
+    quad_dft4(x, q);
+    {
+        cpx<float> *dst = buf + dpp_a(16 * bq, g);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) dst[68 * k1] = cmul(x[k1], wb[k1]);
+    }
+    __syncthreads();
+    // super-stage 2: lane (h, p) = (g, q) reads S[p + 4 m][h]
+    {
+        const cpx<float> *src = buf + dpp_a(g, q);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) x[m] = src[4 * m];
+    }
+    Dft<float, 16>::run(x);
+#pragma unroll
+    for (int j1 = 1; j1 < 16; ++j1) x[j1] = cmul(x[j1], wa[j1]);
+    quad_dft4(x, q);
+    const int vo = (g + 1024 * bq) * 8;
+#pragma unroll
+    for (int j1 = 0; j1 < 16; ++j1) {
+        const cpx<float> y = x[j1];
+        if (PLAIN) bstore<true>(out, vo, j1 * 512, y);
+        else bstore<true>(out, vo, j1 * 512, {y.x * scale, y.y * (scale * osgn)});
+    }
+}
+
+// loads of one row in super-stage 1 order: lane (g, q), register m reads
+// element g + 64 q + 256 m
+__device__ __forceinline__ void p2d_load(cpx<float> (&x)[16], rsrc_t in, int g, int q) {
+    const int vo = (g + 64 * q) * 8;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) x[m] = dload<true>(in, vo, m * 2048);
+}
+
+// lds: 2 * kDppLds complex elements; 256 lanes, rows b, b + G, ... as the
+// pipelined kernel
+template <bool PLAIN>
+__device__ __forceinline__ void pow2_dpp_body(const vhip_fftm_desc &d, cpx<float> *lds) {
+    constexpr int N = 4096;
+    const cpx<float> *in = (const cpx<float> *)d.in.re;
+    cpx<float> *out = (cpx<float> *)d.out.re;
+    const int t = threadIdx.x, g = t >> 2, q = t & 3;
+    const int bq = (q == 1) ? 2 : (q == 2) ? 1 : q;
+    const long ivs = d.in.vs0, ovs = d.out.vs0;
+    const float isgn = d.conj_in ? -1.f : 1.f;
+    const float osgn = d.conj_out ? -1.f : 1.f;
+    const float scale = (float)d.scale;
+    const int G = gridDim.x;
+    const int nvec = __builtin_amdgcn_readfirstlane((int)d.nvec);
+    int row = blockIdx.x;
+    if (row >= nvec) return;
+    auto rin = [&](int r) {
+        return make_rsrc(in + (long)(r < nvec ? r : 0) * ivs, r < nvec ? (unsigned)(N * 8) : 0u);
+    };
+    cpx<float> a[16], b[16];
+    p2d_load(a, rin(row), g, q);
+    p2d_load(b, rin(row + G), g, q);
+    // per-lane twiddles in double, rounded once: W_64^{q k1} and
+    // W_4096^{g (k1 + 16 brev(q))}
+    cpx<float> wa[16], wb[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) {
+        double sn, cs;
+        sincospi(-2.0 * (double)((q * k1) & 63) / 64.0, &sn, &cs);
+        wa[k1] = {(float)cs, (float)sn};
+        sincospi(-2.0 * (double)((g * (k1 + 16 * bq)) & 4095) / 4096.0, &sn, &cs);
+        wb[k1] = {(float)cs, (float)sn};
+    }
+    int parity = 0;
+    p2d_row<PLAIN>(a, lds, wa, wb, make_rsrc(out + (long)row * ovs, (unsigned)(N * 8)), g, q, bq, isgn, osgn, scale);
+    row += G;
+    if (row >= nvec) return;
+    p2d_load(a, rin(row + G), g, q);
+    for (;;) {
+        parity ^= 1;
+        p2d_row<PLAIN>(b, lds + parity * kDppLds, wa, wb, make_rsrc(out + (long)row * ovs, (unsigned)(N * 8)), g, q,
+                       bq, isgn, osgn, scale);
+        row += G;
+        if (row >= nvec) break;
+        p2d_load(b, rin(row + G), g, q);
+        parity ^= 1;
+        p2d_row<PLAIN>(a, lds + parity * kDppLds, wa, wb, make_rsrc(out + (long)row * ovs, (unsigned)(N * 8)), g, q,
+                       bq, isgn, osgn, scale);
+        row += G;
+        if (row >= nvec) break;
+        p2d_load(a, rin(row + G), g, q);
     }
 }
 

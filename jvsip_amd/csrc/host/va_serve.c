@@ -663,6 +663,7 @@ int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_c
     VA_fftm *plans = NULL;
     void *ev[VA_PLAIN_EVENTS];
     size_t *offs = NULL;
+    int *plan_of = NULL;     /* each request's plan, resolved before the clock starts */
     int nplans = 0, cap = 64, rc = 0, e;
     long i, oldest = 0;
     double t0, tend;
@@ -672,7 +673,8 @@ int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_c
     if (va_arena_init(&arena, hbm_budget)) { va_error("serve", "HBM budget allocation failed", -1); return -1; }
     plans = (VA_fftm *)calloc((size_t)cap, sizeof(VA_fftm));
     offs = (size_t *)malloc(sizeof(size_t) * (size_t)nreq);
-    if (!plans || !offs) { rc = -1; goto out; }
+    plan_of = (int *)malloc(sizeof(int) * (size_t)nreq);
+    if (!plans || !offs || !plan_of) { rc = -1; goto out; }
     for (e = 0; e < VA_PLAIN_EVENTS && !rc; e++) rc = vhip_event_create(&ev[e]);
     /* plans are made before the clock starts, as a server would keep them */
     for (i = 0; i < nreq && !rc; i++) {
@@ -681,7 +683,8 @@ int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_c
         for (k = 0; k < nplans && !found; k++)
             found = (plans[k].mN == (vsip_length)reqs[i].rows && plans[k].MN == (vsip_length)reqs[i].n &&
                      plans[k].dir == reqs[i].dir && plans[k].scale == (double)reqs[i].scale);
-        if (found) continue;
+        if (found) { plan_of[i] = k - 1; continue; }
+        plan_of[i] = nplans;
         if (nplans == cap) {
             VA_fftm *t = (VA_fftm *)realloc(plans, sizeof(VA_fftm) * (size_t)(2 * cap));
             if (!t) { rc = -1; break; }
@@ -700,7 +703,6 @@ int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_c
         VA_mview vi, vo;
         size_t len = q ? (size_t)q->rows * (size_t)q->n * 8 : 0;
         void *out;
-        int k;
         /* retire finished requests (in order: one stream); wait for the
          * request's arrival, for a free event, or at the end for everyone */
         for (;;) {
@@ -730,11 +732,7 @@ int vsip_amd_serve_plain_f(const vsip_amd_request *reqs, long nreq, const vsip_c
         out = (q->out_offset < 0) ? (void *)(arena.base + offs[i]) : va_pool_addr(out_pool, q->out_offset);
         va_fake_view(&bi, &vi, va_pool_addr(in_pool, q->in_offset), q->rows, q->n);
         va_fake_view(&bo, &vo, out, q->rows, q->n);
-        for (k = 0; k < nplans; k++)
-            if (plans[k].mN == (vsip_length)q->rows && plans[k].MN == (vsip_length)q->n &&
-                plans[k].dir == q->dir && plans[k].scale == (double)q->scale)
-                break;
-        rc = va_fftm_exec(&plans[k], &vi, &vo, vhip_stream());
+        rc = va_fftm_exec(&plans[plan_of[i]], &vi, &vo, vhip_stream());
         if (!rc) rc = vhip_event_record(ev[i % VA_PLAIN_EVENTS], vhip_stream());
         if (!rc && sync_each) rc = vhip_sync();
         res[i].batch = -1;
@@ -758,6 +756,7 @@ out:
     for (i = 0; i < nplans; i++) va_fftm_release(&plans[i]);
     free(plans);
     free(offs);
+    free(plan_of);
     va_arena_fini(&arena);
     return rc;
 }

@@ -75,13 +75,16 @@ def _np_rows(x, direction, scale):
     return (np.fft.fft(x, axis=1) if direction < 0 else np.fft.ifft(x, axis=1) * n) * scale
 
 
-@pytest.mark.parametrize("m", [4100, 4096 + 3 * 512 + 7, 2048 + 1])
-def test_pipelined_4096_rows(amd, m):
+@pytest.mark.parametrize("m,dpp", [(4100, "0"), (4096 + 3 * 512 + 7, "0"), (2048 + 1, "0"),
+                                   (4100, "1"), (4096 + 3 * 512 + 7, "1"), (2048 + 1, "1")])
+def test_pipelined_4096_rows(amd, monkeypatch, m, dpp):
     """4096-point rows, at least 4 per workgroup of a grid of 2 per CU
     (256 CUs), take the pipelined persistent kernel (rows b, b + G, ...);
     the uneven row counts give some workgroups one row more, and the
     prefetch past the last row reads nothing.  Forward, inverse with scale,
-    and out of place from rows 2N apart, against numpy's FP64 FFT."""
+    and out of place from rows 2N apart, against numpy's FP64 FFT.  dpp "1"
+    selects the form with cross-lane (DPP) exchanges (VSIP_AMD_P2_DPP)."""
+    monkeypatch.setenv("VSIP_AMD_P2_DPP", dpp)
     n = 4096
     x = datagen.complex_data(140, (m, n), dist="uniform")
     v = fftm.dense_view(m, n)
