@@ -39,6 +39,31 @@ std::mutex g_mu;
 std::map<std::string, Entry> g_cache;
 char g_err[512];
 
+// The launch path looks kernels up by the schedule's fields packed into
+// integers (no instance-name formatting per launch); prepare fills it.
+struct FastKey {
+    unsigned long long a, b;
+    bool operator<(const FastKey &o) const { return a != o.a ? a < o.a : b < o.b; }
+};
+std::map<FastKey, hipFunction_t> g_fast;
+
+// every field in its own bits: n (31), vpb (16), threads (11), prec (4) in a;
+// min, mout, nst (4 each) and radix - 1 of each stage (4 each, radices 1..16)
+// in b; a schedule the fields cannot hold is not keyed (it runs on the
+// generic kernel)
+bool fast_key(int prec, int n, int nst, const int *radix, int vpb, int threads, int min, int mout, FastKey &k) {
+    if (n < 1 || vpb < 1 || vpb > 0xffff || threads < 1 || threads > 2047 || nst < 1 || nst > 8) return false;
+    k.a = (unsigned long long)(unsigned)n | ((unsigned long long)vpb << 31) | ((unsigned long long)threads << 47) |
+          ((unsigned long long)(prec & 0xf) << 58);
+    k.b = (unsigned long long)(min & 0xf) | ((unsigned long long)(mout & 0xf) << 4) |
+          ((unsigned long long)nst << 8);
+    for (int s = 0; s < nst; ++s) {
+        if (radix[s] < 1 || radix[s] > 16) return false;
+        k.b |= (unsigned long long)(radix[s] - 1) << (12 + 4 * s);
+    }
+    return true;
+}
+
 bool jit_on() {
     const char *e = getenv("VSIP_AMD_JIT");
     return !(e && e[0] == '0');
@@ -149,6 +174,8 @@ extern "C" int vhip_jit_prepare(int prec, int n, int nst, const int *radix, int 
     }
     (void)hipGetLastError();
     g_cache[name] = e;
+    FastKey key;
+    if (e.fn && fast_key(prec, n, nst, radix, vpb, threads, min, mout, key)) g_fast[key] = e.fn;
     return e.fn ? 0 : -1;
 }
 
@@ -201,12 +228,13 @@ extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
         if (((long)(d->vpb - 1) * d->out.vs0 + out_len) * es >= (1L << 31)) return -100;
     }
     hipFunction_t fn = nullptr;
+    FastKey key;
+    if (!fast_key(d->prec, d->n, d->nst, d->radix, d->vpb, d->threads, min, mout, key)) return -100;
     {
-        std::string name = instance(d->prec, d->n, d->nst, d->radix, d->vpb, d->threads, min, mout);
         std::lock_guard<std::mutex> lock(g_mu);
-        auto it = g_cache.find(name);
-        if (it == g_cache.end() || !it->second.fn) return -100;
-        fn = it->second.fn;
+        auto it = g_fast.find(key);
+        if (it == g_fast.end()) return -100;
+        fn = it->second;
     }
     const long grid = (d->nvec + d->vpb - 1) / d->vpb;
     if (grid > 0x7fffffffL) return -100;
