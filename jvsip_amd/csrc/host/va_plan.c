@@ -140,11 +140,12 @@ static long va_lds_bytes(const VA_sched *s, int prec, int vpb)
 }
 
 /* smallest thread count (multiple of 64, <= 1024; <= 512 for the 32-element
- * register class) that fits; 0 if none */
-static int va_fit_threads(const VA_sched *s, int vpb, int maxe)
+ * register class), at least thr_min, that fits; 0 if none */
+static int va_fit_threads(const VA_sched *s, int vpb, int maxe, int thr_min)
 {
     long want = ((long)vpb * s->n + maxe - 1) / maxe;
     int thr = (int)(((want + 63) / 64) * 64), tmax = maxe > 16 ? 512 : 1024;
+    if (thr < thr_min) thr = ((thr_min + 63) / 64) * 64;
     if (thr < 64) thr = 64;
     for (; thr <= tmax; thr += 64)
         if (va_shape_ok(s, vpb, thr, maxe)) return thr;
@@ -159,8 +160,16 @@ static int va_env_cap(const char *name, int v)
     return (c > 0 && c < v) ? c : v;
 }
 
+/* a tuning floor on lanes per workgroup from the environment (0: none) */
+static int va_env_floor(const char *name)
+{
+    const char *e = getenv(name);
+    int c = e ? atoi(e) : 0;
+    return (c > 0 && c <= 1024) ? c : 0;
+}
+
 static int va_pick_launch(const VA_sched *s, int prec, int vpb_want, int lds_soft, int lds_max,
-                          VA_launch *out)
+                          int thr_min, VA_launch *out)
 {
     int vpb, thr, m;
     static const int maxes_f[2] = {16, 32};
@@ -172,7 +181,7 @@ static int va_pick_launch(const VA_sched *s, int prec, int vpb_want, int lds_sof
         if (lds > lds_max) continue;
         if (vpb > 1 && lds > lds_soft) continue;
         for (m = 0; m < nmax; m++) {
-            thr = va_fit_threads(s, vpb, maxes[m]);
+            thr = va_fit_threads(s, vpb, maxes[m], thr_min);
             if (thr) {
                 out->vpb = vpb;
                 out->threads = thr;
@@ -216,7 +225,26 @@ int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max)
     vpb_along = va_env_cap("VSIP_AMD_VPB_ALONG", vpb_along);
     if (vpb_along < 1) vpb_along = 1;
     if ((vsip_length)vpb_along > nvec && nvec > 0) vpb_along = (int)nvec;
-    if (va_pick_launch(s, prec, vpb_along, 65536, lds_max, &s->along)) return -2;
+    {
+        /* A launch of at most two workgroups per CU runs each workgroup once
+         * and is bound by one workgroup's latency: FP32 rows then take enough
+         * lanes for one butterfly per lane in every stage (ROW 512 x 1001,
+         * [13, 11, 7]: 128 -> 192 lanes, 6.53 -> 5.86 us; no gain for
+         * larger grids, and FP64 loses) */
+        int thr_min = va_env_floor("VSIP_AMD_THREADS_ALONG");
+        if (prec == 4 && nvec > 0 && !thr_min) {
+            int cus = 256, k;
+            long most = 0;
+            vhip_cu_count(&cus);
+            if (cus < 1) cus = 256;
+            if (((long)nvec + vpb_along - 1) / vpb_along <= 2L * cus) {
+                for (k = 0; k < s->nst; k++)
+                    if ((long)vpb_along * (n / s->radix[k]) > most) most = (long)vpb_along * (n / s->radix[k]);
+                thr_min = most <= 1024 ? (int)most : 0;
+            }
+        }
+        if (va_pick_launch(s, prec, vpb_along, 65536, lds_max, thr_min, &s->along)) return -2;
+    }
 
     /* across: 16 vectors give 128-byte (float) row segments per load when
      * two such workgroups fit a CU; otherwise 4, whose neighbouring groups
@@ -239,7 +267,8 @@ int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max)
     }
     vpb_across = va_env_cap("VSIP_AMD_VPB_ACROSS", vpb_across);
     if ((vsip_length)vpb_across > nvec && nvec > 0) vpb_across = (int)nvec;
-    if (va_pick_launch(s, prec, vpb_across, lds_max, lds_max, &s->across)) return -2;
+    if (va_pick_launch(s, prec, vpb_across, lds_max, lds_max, va_env_floor("VSIP_AMD_THREADS_ACROSS"), &s->across))
+        return -2;
 
     /* stage twiddle offsets */
     off = 0;
