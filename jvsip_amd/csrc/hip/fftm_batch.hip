@@ -10,8 +10,8 @@
 //
 // Power-of-two lengths from 256 to 8192 on contiguous rows run the
 // specialised power-of-two stages (8192 / n rows per item); the rest run the
-// generic body.  Each workgroup fetches its next item index while it
-// transforms the current one.
+// generic body, whose stages take their twiddles from a table in LDS.  Each
+// workgroup fetches its next item index while it transforms the current one.
 //
 // Lengths longer than one workgroup's LDS run as two passes (n = n1 * n2).
 // Their second-pass items go to a second list drained by a second persistent
@@ -31,6 +31,62 @@ __device__ __forceinline__ unsigned next_index(unsigned *head) {
     return __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ long uniform(long v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long)v >> 32));
+    return (long)(((unsigned long)hi << 32) | lo);
+}
+
+// Twiddle table of the generic items, copied into LDS from the one the host
+// appends to the schedule's stage twiddles (d.btw_off): W_n^e = lo[e mod
+// 2^lg] hi[e >> lg], lg = ceil(ceil(log2 n) / 2).  Batched single-pass
+// lengths are at most 8560 (the LDS of one workgroup), so 2^lg <= 128 and the
+// table holds at most 128 + 67 entries.
+constexpr int kTwCap = 256;
+
+__device__ __forceinline__ int load_twiddles(cpx<float> *tab, const vhip_fftm_desc &d) {
+    const int n = d.n;
+    const int cl = n > 1 ? 32 - __builtin_clz((unsigned)(n - 1)) : 0;
+    const int lg = (cl + 1) >> 1;
+    const int cnt = (1 << lg) + ((n + (1 << lg) - 1) >> lg);
+    const rsrc_t src = make_rsrc((const cpx<float> *)d.tw + d.btw_off, (long)cnt * (long)sizeof(cpx<float>));
+    for (int t = threadIdx.x; t < cnt && t < kTwCap; t += blockDim.x) tab[t] = Buf<float>::ldc(src, t * 8, 0);
+    return lg;
+}
+
+// One generic item: the Stockham body of fft_body (C2C) with the stages after
+// the first reading their twiddles from the LDS table `tw` (lg as loaded).
+template <int MAXE>
+__device__ __forceinline__ void batch_generic(const vhip_fftm_desc &d, long vbase, int nv, char *smem,
+                                              const cpx<float> *tw, int lg) {
+    cpx<float> *lds = (cpx<float> *)smem;
+    Shared sh;
+    sh.voff_in = (long *)(smem + (size_t)d.vpb * d.lds_stride * sizeof(cpx<float>));
+    sh.voff_out = sh.voff_in + d.vpb;
+    sh.vidx0 = (int *)(sh.voff_out + d.vpb);
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+        const long v = vbase + t;
+        sh.voff_in[t] = vec_off(d.in, v);
+        sh.voff_out[t] = vec_off(d.out, v);
+        sh.vidx0[t] = (int)(d.ptw_n > 0 ? v % d.ptw_n : v);
+    }
+    __syncthreads();
+    const float scale = (float)d.scale;
+    const float isgn = d.conj_in ? -1.f : 1.f;
+    const float osgn = d.conj_out ? -1.f : 1.f;
+    BatchCtx<float> bc;
+    bc.lo = tw;
+    bc.hi = tw + (1 << lg);
+    bc.lg = lg;
+    const int nst = d.nst;
+    int Ns = 1;
+    for (int s = 0; s < nst; ++s) {
+        run_stage<float, MAXE, VHIP_C2C, true, true>(d, sh, lds, s, Ns, nv, s == 0, s == nst - 1, isgn, osgn,
+                                                     scale, &bc);
+        Ns *= d.radix[s];
+    }
+}
+
 // ctl[0]: next item; ctl[1]: workgroups finished.  The last workgroup to
 // finish clears both (every workgroup has made its final fetch by then), so
 // the next launch on the stream starts from zero without a clearing kernel.
@@ -43,10 +99,12 @@ __global__ void __launch_bounds__(512)
     // SGPRs
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ long s_item;
+    __shared__ cpx<float> s_tw[kTwCap];
+    int twn = 0, lg = 0;
     if (threadIdx.x == 0) s_item = (long)next_index(&ctl[0]);
     for (;;) {
         __syncthreads();  // s_item is set; the previous item is done with LDS
-        const long it = s_item;
+        const long it = uniform(s_item);
         if (it >= nitems) break;
         long nxt = 0;
         if (threadIdx.x == 0) nxt = (long)next_index(&ctl[0]);   // overlaps the item below
@@ -64,7 +122,13 @@ __global__ void __launch_bounds__(512)
         default: {
             const long left = d.nvec - item.vbase;
             const int nv = left < d.vpb ? (int)left : d.vpb;
-            fft_body<float, MAXE, VHIP_C2C, true>(d, item.vbase, nv, smem);
+            // the previous item's last reads of the table came before the
+            // barrier at the loop head; stage 0's barrier publishes the copy
+            if (d.nst > 1 && d.n != twn) {
+                lg = load_twiddles(s_tw, d);
+                twn = d.n;
+            }
+            batch_generic<MAXE>(d, item.vbase, nv, smem, s_tw, lg);
         }
         }
         __syncthreads();  // every lane has read s_item
@@ -81,20 +145,18 @@ __global__ void __launch_bounds__(512)
 
 typedef void (*batch_fn)(const vhip_fftm_desc *, const vhip_batch_item *, long, unsigned *);
 
+// the batched kernel has one register class, 16 elements per lane
 batch_fn pick_batch(int maxe) {
-    return maxe <= 16 ? batch_kernel<16> : batch_kernel<32>;
+    return maxe == 16 ? batch_kernel<16> : nullptr;
 }
 
 }  // namespace
 
 int batch_set_lds_limit(int bytes) {
-    const void *f[2] = {(const void *)batch_kernel<16>, (const void *)batch_kernel<32>};
-    for (int i = 0; i < 2; ++i) {
-        // the kernel's own __shared__ word comes out of the same per-workgroup LDS
-        hipError_t e = hipFuncSetAttribute(f[i], hipFuncAttributeMaxDynamicSharedMemorySize, bytes - 256);
-        if (e != hipSuccess) return (int)e;
-    }
-    return 0;
+    // the kernel's own __shared__ item indices and twiddle table come out of
+    // the same per-workgroup LDS
+    return (int)hipFuncSetAttribute((const void *)batch_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    bytes - 256 - kTwCap * (int)sizeof(cpx<float>));
 }
 
 }  // namespace vhipk
@@ -167,6 +229,7 @@ extern "C" int vhip_noalloc_stage_done(int slot, void *stream) {
 
 extern "C" int vhip_batch_max_grid(int threads, int maxe, int lds_bytes, int *grid) {
     int per_cu = 0, cus = 0, dev = 0;
+    if (!vhipk::pick_batch(maxe)) return -1;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, (const void *)vhipk::pick_batch(maxe), threads, (size_t)lds_bytes);
     if (e != hipSuccess) return (int)e;
@@ -186,7 +249,7 @@ extern "C" int vhip_batch_clear(unsigned *ctl, void *stream) {
 extern "C" int vhip_batch_launch(const vhip_fftm_desc *descs, const vhip_batch_item *items, long nitems1,
                                  long nitems2, unsigned *ctl, int threads, int maxe, int lds_bytes,
                                  int grid, void *stream) {
-    if (threads != 512 || grid < 1 || nitems1 < 0 || nitems2 < 0) return -1;
+    if (threads != 512 || !vhipk::pick_batch(maxe) || grid < 1 || nitems1 < 0 || nitems2 < 0) return -1;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipSuccess;
     long g1 = nitems1 < grid ? nitems1 : grid;

@@ -229,9 +229,12 @@ template <typename T> struct Dft<T, 15> : DftComp<T, 3, 5, Dft> {};
 // indexing helpers
 // --------------------------------------------------------------------------
 
-// q = a / b, r = a % b for 0 <= a, b < 2^24 (float estimate + one correction)
+// q = a / b, r = a % b for 0 <= a < 2^22, 1 <= b: the float estimate a * rcp(b)
+// (v_rcp_f32, within 1 ulp) is off from a / b by less than 2^22 * 1.5 * 2^-23 < 1,
+// so one correction step gives the exact quotient.  Every caller divides an
+// index below the element count of one workgroup.
 __device__ __forceinline__ void divmod(int a, int b, int &q, int &r) {
-    int qq = (int)__fdividef((float)a, (float)b);
+    int qq = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
     int rr = a - qq * b;
     if (rr < 0) { qq -= 1; rr += b; }
     else if (rr >= b) { qq += 1; rr -= b; }
@@ -392,8 +395,13 @@ __device__ __forceinline__ void store_out(const vhip_fftm_desc &d, int vb, int v
     const int ls = SOFF ? leg : 0;
     if (d.ptw) {
         const int t = vi0 * (j + leg);
-        const cpx<T> *w = (const cpx<T> *)d.ptw;
-        v = cmul(v, cmul(w[t & ((1 << d.ptw_shift) - 1)], w[(1 << d.ptw_shift) + (t >> d.ptw_shift)]));
+        // a global-memory pointer: global loads count only in vmcnt, where a
+        // generic (flat) load would also hold up the LDS waits (lgkmcnt)
+        typedef const __attribute__((address_space(1))) T *gptr;
+        const gptr w = (gptr)d.ptw;
+        const int i0 = 2 * (t & ((1 << d.ptw_shift) - 1)), i1 = 2 * ((1 << d.ptw_shift) + (t >> d.ptw_shift));
+        const cpx<T> w0 = {w[i0], w[i0 + 1]}, w1 = {w[i1], w[i1 + 1]};
+        v = cmul(v, cmul(w0, w1));
     }
     v.x *= scale;
     v.y *= scale * osgn;
@@ -426,15 +434,28 @@ __device__ __forceinline__ void store_out(const vhip_fftm_desc &d, int vb, int v
 // one Stockham stage
 // --------------------------------------------------------------------------
 
+// The batched kernel's form of the stages (BATCH = true): the stages after
+// the first take their twiddles from a two-level table of W_n = e^{-2 pi i / n}
+// in LDS, W_n^e = lo[e mod 2^lg] hi[e >> lg], instead of a global load per
+// twiddle.  The stage twiddle W_{Ns R}^{r k} is W_n^e with e = r k n / (Ns R),
+// which is below n.
+template <typename T>
+struct BatchCtx {
+    const cpx<T> *lo;
+    const cpx<T> *hi;
+    int lg;
+};
+
 // butterflies one thread holds in a stage of radix R (the host planner uses the
 // same rule): MAXE / R, at least one; radices other than 1-5, 7, 8 and 16
 // (composites and the primes 11, 13) hold one
 constexpr int stage_maxb(int R, int maxe) {
     return (R == 6 || R >= 9) && R != 16 ? 1 : ((maxe / R) > 0 ? maxe / R : 1);
 }
-template <typename T, int R, int MAXE, int KIND, bool SRC_G, bool DST_G, bool SOFF>
+template <typename T, int R, int MAXE, int KIND, bool SRC_G, bool DST_G, bool SOFF, bool BATCH = false>
 __device__ __forceinline__ void stage(const vhip_fftm_desc &d, const Shared &sh, cpx<T> *lds,
-                                      int s, int Ns, int nv, T isgn, T osgn, T scale) {
+                                      int s, int Ns, int nv, T isgn, T osgn, T scale,
+                                      const BatchCtx<T> *bc = nullptr) {
     // butterflies per thread: the register class allows MAXE elements, except
     // that composite and large prime radices stay at one butterfly per thread
     constexpr int MAXB = stage_maxb(R, MAXE);
@@ -445,8 +466,10 @@ __device__ __forceinline__ void stage(const vhip_fftm_desc &d, const Shared &sh,
     const int S = d.lds_stride;
     const int across = SRC_G ? (d.map_in == VHIP_MAP_ACROSS)
                              : (DST_G ? (d.map_out == VHIP_MAP_ACROSS) : 0);
-    // this stage's twiddles: (R - 1) * Ns entries from tw_off[s]
+    // this stage's twiddles: (R - 1) * Ns entries from tw_off[s] (BATCH: the
+    // powers of W_n, in steps of n / (Ns R))
     const rsrc_t twr = make_rsrc((const cpx<T> *)d.tw + d.tw_off[s], (long)n * (long)sizeof(cpx<T>));
+    const int tstride = BATCH ? n / (Ns * R) : 0;
 
     cpx<T> x[MAXB][R];
     int bv[MAXB], bj[MAXB];
@@ -488,9 +511,18 @@ __device__ __forceinline__ void stage(const vhip_fftm_desc &d, const Shared &sh,
             if (Ns > 1) {
                 int q;
                 divmod(j, Ns, q, k);
+                if (BATCH) {
+                    const int step = k * tstride, mask = (1 << bc->lg) - 1;
 #pragma unroll
-                for (int r = 1; r < R; ++r)
-                    x[i][r] = cmul(x[i][r], Buf<T>::ldc(twr, (k + (r - 1) * Ns) * (int)sizeof(cpx<T>), 0));
+                    for (int r = 1; r < R; ++r) {
+                        const int e = r * step;
+                        x[i][r] = cmul(x[i][r], cmul(bc->lo[e & mask], bc->hi[e >> bc->lg]));
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        x[i][r] = cmul(x[i][r], Buf<T>::ldc(twr, (k + (r - 1) * Ns) * (int)sizeof(cpx<T>), 0));
+                }
             }
             Dft<T, R>::run(x[i]);
             if (DST_G) {
@@ -511,42 +543,43 @@ __device__ __forceinline__ void stage(const vhip_fftm_desc &d, const Shared &sh,
     if (!DST_G) __syncthreads();
 }
 
-template <typename T, int R, int MAXE, int KIND, bool SOFF>
+template <typename T, int R, int MAXE, int KIND, bool SOFF, bool BATCH = false>
 __device__ __forceinline__ void stage_dispatch_pos(const vhip_fftm_desc &d, const Shared &sh,
                                                    cpx<T> *lds, int s, int Ns, int nv, bool first,
-                                                   bool last, T isgn, T osgn, T scale) {
-    if (first && last) stage<T, R, MAXE, KIND, true, true, SOFF>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
-    else if (first) stage<T, R, MAXE, KIND, true, false, SOFF>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
-    else if (last) stage<T, R, MAXE, KIND, false, true, SOFF>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
-    else stage<T, R, MAXE, KIND, false, false, SOFF>(d, sh, lds, s, Ns, nv, isgn, osgn, scale);
+                                                   bool last, T isgn, T osgn, T scale,
+                                                   const BatchCtx<T> *bc = nullptr) {
+    if (first && last) stage<T, R, MAXE, KIND, true, true, SOFF, BATCH>(d, sh, lds, s, Ns, nv, isgn, osgn, scale, bc);
+    else if (first) stage<T, R, MAXE, KIND, true, false, SOFF, BATCH>(d, sh, lds, s, Ns, nv, isgn, osgn, scale, bc);
+    else if (last) stage<T, R, MAXE, KIND, false, true, SOFF, BATCH>(d, sh, lds, s, Ns, nv, isgn, osgn, scale, bc);
+    else stage<T, R, MAXE, KIND, false, false, SOFF, BATCH>(d, sh, lds, s, Ns, nv, isgn, osgn, scale, bc);
 }
 
-template <typename T, int MAXE, int KIND, bool SOFF>
+template <typename T, int MAXE, int KIND, bool SOFF, bool BATCH = false>
 __device__ __forceinline__ void run_stage(const vhip_fftm_desc &d, const Shared &sh, cpx<T> *lds,
                                           int s, int Ns, int nv, bool first, bool last, T isgn,
-                                          T osgn, T scale) {
+                                          T osgn, T scale, const BatchCtx<T> *bc = nullptr) {
     switch (d.radix[s]) {
     case 16:
         // radix 16 only in the float kernels (the planner never emits it for double)
         if constexpr (sizeof(T) == 4)
-            stage_dispatch_pos<T, 16, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale);
+            stage_dispatch_pos<T, 16, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc);
         break;
-    case 8: stage_dispatch_pos<T, 8, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 4: stage_dispatch_pos<T, 4, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 2: stage_dispatch_pos<T, 2, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 3: stage_dispatch_pos<T, 3, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 5: stage_dispatch_pos<T, 5, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 7: stage_dispatch_pos<T, 7, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 11: stage_dispatch_pos<T, 11, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 13: stage_dispatch_pos<T, 13, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 1: stage_dispatch_pos<T, 1, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 8: stage_dispatch_pos<T, 8, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 4: stage_dispatch_pos<T, 4, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 2: stage_dispatch_pos<T, 2, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 3: stage_dispatch_pos<T, 3, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 5: stage_dispatch_pos<T, 5, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 7: stage_dispatch_pos<T, 7, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 11: stage_dispatch_pos<T, 11, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 13: stage_dispatch_pos<T, 13, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 1: stage_dispatch_pos<T, 1, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
     // composite radices (the planner emits them for float only)
-    case 6: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 6, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 9: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 9, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 10: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 10, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 12: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 12, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 14: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 14, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
-    case 15: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 15, MAXE, KIND, SOFF>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale); break;
+    case 6: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 6, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 9: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 9, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 10: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 10, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 12: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 12, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 14: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 14, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
+    case 15: if constexpr (sizeof(T) == 4) stage_dispatch_pos<T, 15, MAXE, KIND, SOFF, BATCH>(d, sh, lds, s, Ns, nv, first, last, isgn, osgn, scale, bc); break;
     default: break;  // the planner only emits the radices above
     }
 }

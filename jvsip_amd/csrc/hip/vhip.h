@@ -85,7 +85,11 @@ typedef struct {
                                       (set by the host for dense, line-aligned rows;
                                       honoured by the compiled-schedule kernels) */
     int compiled;                  /* the plan may run on its compiled-schedule kernels */
-    int pad2_;
+    int btw_off;                   /* batched kernel: offset (complex elements) in tw of
+                                      the powers of W_n = e^{-2 pi i / n} in two levels,
+                                      lo[t] = W_n^t (t < 2^lg), then hi[m] = W_n^{m 2^lg}
+                                      (m < ceil(n / 2^lg)), lg = ceil(ceil(log2 n) / 2);
+                                      -1 when the schedule has one stage */
     vhip_view in, out;
     /* fused element work of a pipeline (NULL when unused; only the
      * standalone generic kernel implements it, not the batched one):
@@ -263,7 +267,8 @@ int vhip_fftm_launch(const vhip_fftm_desc *d, void *stream);
  * second one drains items[nitems1 .. nitems1 + nitems2), the second passes of
  * two-pass transforms.  ctl holds 4 counters that must be zero before the
  * first launch (vhip_batch_clear); every launch leaves them zero again.  All
- * launches go to `stream` (which may be capturing a graph). */
+ * launches go to `stream` (which may be capturing a graph).  threads must be
+ * 512 and maxe 16 (the batched kernel's only shape). */
 int vhip_batch_clear(unsigned *ctl, void *stream);
 int vhip_batch_launch(const vhip_fftm_desc *descs, const vhip_batch_item *items, long nitems1,
                       long nitems2, unsigned *ctl, int threads, int maxe, int lds_bytes,

@@ -434,6 +434,7 @@ static void va_desc_common(vhip_fftm_desc *d, const VA_fftm *p, const VA_sched *
     d->map_in = map_in;
     d->map_out = map_out;
     d->tw = s->dev_tw;
+    d->btw_off = s->btw_off;
     d->scale = 1.0;
     d->compiled = p->jit;
 }
@@ -574,6 +575,12 @@ int va_fftm_descs(const VA_fftm *p, const VA_mview *in, const VA_mview *out, con
     if (!sA && p->npass == 3) {
         if (kind != VHIP_C2C) { va_error("fftm", "three-pass real transform", -1); return -1; }
         return va_descs3(p, &gi, &go, es_i, vs_i, es_o, vs_o, inv, d);
+    }
+    /* schedules given by the caller run in the batched kernel, whose stages
+     * after the first read the two-level twiddle table of the upload */
+    if (sA && ((sA->nst > 1 && sA->btw_off < 0) || (sB && sB->nst > 1 && sB->btw_off < 0))) {
+        va_error("fftm", "batched schedule without its twiddle table", sA->n);
+        return -1;
     }
     if (!sA) {
         sA = &p->pass[0];

@@ -97,6 +97,7 @@ typedef struct {
     int radix[VHIP_MAX_STAGES];
     int tw_off[VHIP_MAX_STAGES];
     int tw_len;              /* complex entries of the stage twiddle table */
+    int btw_off;             /* the batched kernel's two-level table in dev_tw, or -1 */
     int lds_stride;          /* padded LDS vector stride */
     VA_launch along;         /* for unit element stride (rows) */
     VA_launch across;        /* for unit vector stride (columns) */

@@ -326,11 +326,33 @@ int va_sched_shape_fixed(VA_sched *s, int prec, int n, const int *radix, int nst
     return 0;
 }
 
-/* the stage twiddles of a shaped schedule, uploaded to the device */
+/* lg of the batched kernel's two-level twiddle table of length n:
+ * ceil(ceil(log2 n) / 2) (the kernel derives the same value from n) */
+static int va_btw_lg(int n)
+{
+    int cl = 0;
+    while ((1L << cl) < (long)n) cl++;
+    return (cl + 1) / 2;
+}
+
+/* the stage twiddles of a shaped schedule, uploaded to the device; float
+ * schedules of two or more stages also carry the batched kernel's table of
+ * the powers of W_n = e^{-2 pi i / n} in two levels (after the stage
+ * twiddles, at btw_off): lo[t] = W_n^t for t < 2^lg, then
+ * hi[m] = W_n^{m 2^lg} for m < ceil(n / 2^lg) */
 int va_sched_upload(VA_sched *s, int prec)
 {
     int i, r, k, Ns, off;
+    int blg = 0, bl = 0, bh = 0;
     off = s->tw_len;
+    s->btw_off = -1;
+    if (prec == 4 && s->nst > 1) {
+        blg = va_btw_lg(s->n);
+        bl = 1 << blg;
+        bh = (s->n + bl - 1) >> blg;
+        s->btw_off = off;
+        off += bl + bh;
+    }
     if (off > 0) {
         size_t esz = (size_t)(2 * prec);
         char *host = (char *)malloc(esz * (size_t)off);
@@ -360,6 +382,13 @@ int va_sched_upload(VA_sched *s, int prec)
                 }
             }
             Ns *= R;
+        }
+        for (i = 0; i < bl + bh; i++) {
+            long e = i < bl ? (long)i : (long)(i - bl) << blg;
+            double a = -2.0 * VA_PI * (double)e / (double)s->n;
+            float *f = (float *)host + 2L * (s->btw_off + i);
+            f[0] = (float)cos(a);
+            f[1] = (float)sin(a);
         }
         rc = vhip_malloc(&s->dev_tw, esz * (size_t)off);
         if (!rc) rc = vhip_memcpy_h2d(s->dev_tw, host, esz * (size_t)off);

@@ -75,6 +75,7 @@ static float va_time_sched(const VA_sched *s, int prec, void *scratch, long nvec
     d.map_in = d.map_out = VHIP_MAP_ALONG;
     d.scale = 1.0;
     d.tw = s->dev_tw;
+    d.btw_off = s->btw_off;
     d.in.re = d.out.re = scratch;
     d.in.es = d.out.es = 1;
     d.in.vs0 = d.out.vs0 = s->n;
