@@ -100,16 +100,26 @@ int vsip_amd_tune(int prec, long n, char *buf, size_t len);
  * complex element in_offset and written to the output pool block at
  * out_offset, or (out_offset < 0) to memory the server takes from its HBM
  * budget and returns when the request completes.  Requests must be sorted by
- * t_arrive (seconds after the run starts).  The server admits arrived
- * requests, up to max_jobs per group, while the budget and its launch lanes
- * allow, and launches each group on a free lane (a stream; lanes overlap on
- * the device): requests of at least VSIP_AMD_SERVE_DIRECT elements (default
- * 32768) launch their own length-specialised kernel, and the smaller ones of
- * the group share one persistent batched kernel when there are more than two
- * of them.  Finished groups are retired as they complete; res[i] gets the
- * admission and completion times.  stats receives a JSON summary (with the
- * counts of directly launched and batched requests).  Returns 0 or an error
- * code. */
+ * t_arrive (seconds after the run starts).  Arrived requests are admitted
+ * while their output (and scratch) fits the budget, on one of two paths:
+ *   * resident: requests of at most VSIP_AMD_SERVE_RESIDENT elements (default
+ *     131072; 0 turns the path off) whose length runs in one pass of the
+ *     batched workgroup join one persistent kernel that stays on the device
+ *     for the whole stream (VSIP_AMD_SERVE_RESIDENT_WGS workgroups, default
+ *     32): admission writes the request's descriptor to pinned host memory
+ *     and publishes its work items, the running kernel picks them up with no
+ *     launch, and a per-request flag in host memory reports completion;
+ *   * lanes: the other requests, up to max_jobs per group, launch on a free
+ *     lane (a stream; lanes overlap on the device and with the resident
+ *     kernel): requests of at least VSIP_AMD_SERVE_DIRECT elements (default
+ *     32768) launch their own length-specialised kernel, and the smaller ones
+ *     of the group share one persistent batched kernel when there are more
+ *     than two of them.
+ * With lanes == 1 and max_jobs == 1 every request runs alone, in arrival
+ * order.  Finished requests are retired as they complete; res[i] gets the
+ * admission and completion times (res[i].stream == lanes for the resident
+ * path).  stats receives a JSON summary (with the counts of resident,
+ * directly launched and batched requests).  Returns 0 or an error code. */
 typedef struct {
     double t_arrive;
     long rows;

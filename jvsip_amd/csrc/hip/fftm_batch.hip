@@ -87,6 +87,34 @@ __device__ __forceinline__ void batch_generic(const vhip_fftm_desc &d, long vbas
     }
 }
 
+// One work item: the power-of-two stages for rows of 256..8192 points on
+// contiguous rows, else the generic stages with the LDS twiddle table `tw`
+// (reloaded when the item length differs from *twn).
+template <int MAXE>
+__device__ __forceinline__ void run_item(const vhip_fftm_desc &d, long vbase, char *smem, cpx<float> *tw,
+                                         int &twn, int &lg) {
+    const int l2 = __builtin_amdgcn_readfirstlane(p2::pow2_batch_log2(d));
+    switch (l2) {
+    case 8: p2::pow2_batch_item<8>(d, vbase, smem); break;
+    case 9: p2::pow2_batch_item<9>(d, vbase, smem); break;
+    case 10: p2::pow2_batch_item<10>(d, vbase, smem); break;
+    case 11: p2::pow2_batch_item<11>(d, vbase, smem); break;
+    case 12: p2::pow2_batch_item<12>(d, vbase, smem); break;
+    case 13: p2::pow2_batch_item<13>(d, vbase, smem); break;
+    default: {
+        const long left = d.nvec - vbase;
+        const int nv = left < d.vpb ? (int)left : d.vpb;
+        // the previous item's last reads of the table came before the
+        // caller's barrier; stage 0's barrier publishes the copy
+        if (d.nst > 1 && d.n != twn) {
+            lg = load_twiddles(tw, d);
+            twn = d.n;
+        }
+        batch_generic<MAXE>(d, vbase, nv, smem, tw, lg);
+    }
+    }
+}
+
 // ctl[0]: next item; ctl[1]: workgroups finished.  The last workgroup to
 // finish clears both (every workgroup has made its final fetch by then), so
 // the next launch on the stream starts from zero without a clearing kernel.
@@ -110,27 +138,7 @@ __global__ void __launch_bounds__(512)
         if (threadIdx.x == 0) nxt = (long)next_index(&ctl[0]);   // overlaps the item below
         const vhip_batch_item item = items[it];
         const int di = __builtin_amdgcn_readfirstlane(item.desc);
-        const vhip_fftm_desc &d = descs[di];
-        const int l2 = __builtin_amdgcn_readfirstlane(p2::pow2_batch_log2(d));
-        switch (l2) {
-        case 8: p2::pow2_batch_item<8>(d, item.vbase, smem); break;
-        case 9: p2::pow2_batch_item<9>(d, item.vbase, smem); break;
-        case 10: p2::pow2_batch_item<10>(d, item.vbase, smem); break;
-        case 11: p2::pow2_batch_item<11>(d, item.vbase, smem); break;
-        case 12: p2::pow2_batch_item<12>(d, item.vbase, smem); break;
-        case 13: p2::pow2_batch_item<13>(d, item.vbase, smem); break;
-        default: {
-            const long left = d.nvec - item.vbase;
-            const int nv = left < d.vpb ? (int)left : d.vpb;
-            // the previous item's last reads of the table came before the
-            // barrier at the loop head; stage 0's barrier publishes the copy
-            if (d.nst > 1 && d.n != twn) {
-                lg = load_twiddles(s_tw, d);
-                twn = d.n;
-            }
-            batch_generic<MAXE>(d, item.vbase, nv, smem, s_tw, lg);
-        }
-        }
+        run_item<MAXE>(descs[di], item.vbase, smem, s_tw, twn, lg);
         __syncthreads();  // every lane has read s_item
         if (threadIdx.x == 0) s_item = nxt;
     }
@@ -139,6 +147,109 @@ __global__ void __launch_bounds__(512)
         if (fin == gridDim.x - 1) {
             __hip_atomic_store(&ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&ctl[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// ---- the resident kernel of the server --------------------------------------
+//
+// A persistent grid that stays on the device for a whole request stream and
+// takes work as the host publishes it, with no launch per request:
+//   * items[0 .. nitems) lists every work item of the stream's resident
+//     requests in publication order (item.desc = the request's slot); the
+//     host publishes a prefix of it by raising *htail in coherent host memory,
+//     after writing the request's descriptor into slot hdesc + slot *
+//     VHIP_RES_DESC_STRIDE (also coherent host memory, written once per run,
+//     so no cache holds an older copy of it);
+//   * a workgroup claims the next item index from ctl[0] and waits until it is
+//     published: ctl[1] mirrors *htail on the device, and only the waiter that
+//     holds the oldest unpublished index (index == ctl[1]) reads *htail over
+//     the bus, the others sleep and read ctl[1];
+//   * after an item the workgroup's barrier waits for every wave's stores,
+//     one lane counts the item off req_left[slot] with an agent-scope release
+//     (one L2 write-back for the item), and the workgroup that finishes a
+//     request's last item sets hdone[slot] (system scope);
+//   * every wait is bounded: a waiter that spins spin_limit times, or sees
+//     *hstop, sets ctl[2] (abort), and every workgroup then stops claiming, so
+//     the grid always drains.  A request whose items were abandoned never gets
+//     its done flag; the host sees the grid end first and reports an error.
+// ctl = {next claim, published items seen, abort, 0}: zero before the launch.
+__device__ __forceinline__ unsigned resident_claim(unsigned *ctl, const unsigned *htail, const unsigned *hstop,
+                                                   unsigned nitems, unsigned spin_limit) {
+    if (__hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return nitems;
+    const unsigned i = __hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (i >= nitems) return nitems;
+    // polls are relaxed; one acquire once the item is published
+    for (unsigned spins = 0;; ++spins) {
+        const unsigned t = __hip_atomic_load(&ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (i < t) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return i;
+        }
+        if (__hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return nitems;
+        if (spins >= spin_limit) {
+            __hip_atomic_store(&ctl[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return nitems;
+        }
+        if (i == t) {
+            if (__hip_atomic_load(hstop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
+                __hip_atomic_store(&ctl[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return nitems;
+            }
+            unsigned h = __hip_atomic_load(htail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (h > nitems) h = nitems;
+            if (h > t) {
+                // the descriptors the host wrote before raising htail live in
+                // uncached host memory: no device cache holds them, so the
+                // mirror needs no release of this workgroup's writes
+                __hip_atomic_fetch_max(&ctl[1], h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                continue;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        } else {
+            __builtin_amdgcn_s_sleep(8);
+        }
+    }
+}
+
+template <int MAXE>
+__global__ void __launch_bounds__(512)
+    resident_kernel(const char *__restrict__ hdesc, const vhip_batch_item *__restrict__ items,
+                    unsigned *req_left, unsigned *ctl, const unsigned *htail, const unsigned *hstop,
+                    unsigned *hdone, unsigned nitems, unsigned spin_limit) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    __shared__ unsigned s_item;
+    __shared__ cpx<float> s_tw[kTwCap];
+    int twn = 0, lg = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = resident_claim(ctl, htail, hstop, nitems, spin_limit);
+        __syncthreads();
+        const unsigned it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= nitems) break;
+        const vhip_batch_item item = items[it];
+        const int slot = __builtin_amdgcn_readfirstlane(item.desc);
+        const char *sd = hdesc + (size_t)slot * VHIP_RES_DESC_STRIDE;
+        // one bus round trip for the whole descriptor: touch each of its
+        // 64-byte lines before the stages read its fields
+        {
+            unsigned t0 = *(const unsigned *)(sd + 0), t1 = *(const unsigned *)(sd + 64),
+                     t2 = *(const unsigned *)(sd + 128), t3 = *(const unsigned *)(sd + 192),
+                     t4 = *(const unsigned *)(sd + 256), t5 = *(const unsigned *)(sd + 320),
+                     t6 = *(const unsigned *)(sd + 384), t7 = *(const unsigned *)(sd + 448);
+            asm volatile("" ::"s"(t0), "s"(t1), "s"(t2), "s"(t3), "s"(t4), "s"(t5), "s"(t6), "s"(t7));
+        }
+        run_item<MAXE>(*(const vhip_fftm_desc *)sd, item.vbase, smem, s_tw, twn, lg);
+        // every wave's stores have reached L2 at the barrier; the count's
+        // agent-scope release then writes the XCD's L2 back once for all
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned left =
+                __hip_atomic_fetch_sub(&req_left[slot], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (left == 1u) {
+                // the request's other items were released into the count
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __hip_atomic_store(&hdone[slot], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
         }
     }
 }
@@ -155,8 +266,11 @@ batch_fn pick_batch(int maxe) {
 int batch_set_lds_limit(int bytes) {
     // the kernel's own __shared__ item indices and twiddle table come out of
     // the same per-workgroup LDS
-    return (int)hipFuncSetAttribute((const void *)batch_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    bytes - 256 - kTwCap * (int)sizeof(cpx<float>));
+    const int dyn = bytes - 256 - kTwCap * (int)sizeof(cpx<float>);
+    hipError_t e = hipFuncSetAttribute((const void *)batch_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)resident_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    return (int)e;
 }
 
 }  // namespace vhipk
@@ -239,6 +353,17 @@ extern "C" int vhip_batch_max_grid(int threads, int maxe, int lds_bytes, int *gr
     if (per_cu < 1) per_cu = 1;
     *grid = per_cu * cus;
     return 0;
+}
+
+extern "C" int vhip_resident_launch(const vhip_resident_args *a, int grid, int lds_bytes, void *stream) {
+    if (!a || grid < 1 || !a->hdesc || !a->items || !a->req_left || !a->ctl || !a->htail || !a->hstop ||
+        !a->hdone || ((size_t)a->hdesc & (VHIP_RES_DESC_STRIDE - 1)) != 0)
+        return -1;
+    static_assert(sizeof(vhip_fftm_desc) <= VHIP_RES_DESC_STRIDE, "a request's descriptor fits its slot");
+    hipLaunchKernelGGL(vhipk::resident_kernel<16>, dim3((unsigned)grid), dim3(512), (size_t)lds_bytes,
+                       (hipStream_t)stream, (const char *)a->hdesc, a->items, a->req_left, a->ctl, a->htail,
+                       a->hstop, a->hdone, a->nitems, a->spin_limit);
+    return (int)hipGetLastError();
 }
 
 extern "C" int vhip_batch_clear(unsigned *ctl, void *stream) {

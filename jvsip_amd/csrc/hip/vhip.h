@@ -275,6 +275,52 @@ int vhip_batch_launch(const vhip_fftm_desc *descs, const vhip_batch_item *items,
                       int grid, void *stream);
 int vhip_batch_max_grid(int threads, int maxe, int lds_bytes, int *grid);
 
+/* The server's resident kernel: a persistent grid of the batched kernel's
+ * shape (512 lanes, 16 elements per lane, lds_bytes of LDS) that takes the
+ * work items of a request stream as the host publishes them, with no launch
+ * per request.
+ *   hdesc      coherent host memory: one descriptor per request slot,
+ *              VHIP_RES_DESC_STRIDE bytes apart (the base aligned to that);
+ *              slot s is written once, before any item of it is published
+ *   items      device: every item of the stream in publication order,
+ *              items[i].desc = the request's slot
+ *   req_left   device: per slot, the number of its items (counted down)
+ *   ctl        device: 4 counters, zero before the launch
+ *   htail      coherent host memory: items published so far (only grows)
+ *   hstop      coherent host memory: nonzero makes waiting workgroups stop
+ *   hdone      coherent host memory: per slot, set to 1 (system scope, after
+ *              the request's stores are written back) when its last item is
+ *              done
+ * The grid ends once it has claimed every one of nitems items, or when a
+ * wait exceeds spin_limit polls or hstop is set (ctl[2] then reads 1 and the
+ * unfinished requests never get their done flag). */
+#define VHIP_RES_DESC_STRIDE 512
+typedef struct {
+    const vhip_fftm_desc *hdesc;
+    const vhip_batch_item *items;
+    unsigned *req_left;
+    unsigned *ctl;
+    const unsigned *htail;
+    const unsigned *hstop;
+    unsigned *hdone;
+    unsigned nitems;
+    unsigned spin_limit;
+} vhip_resident_args;
+int vhip_resident_launch(const vhip_resident_args *a, int grid, int lds_bytes, void *stream);
+/* CUs that persistent launches leave to a resident kernel (their grids are
+ * sized to the remaining CUs while it runs) */
+int vhip_set_reserved_cus(int n);
+/* coherent pinned host memory (the device reads and writes it over the bus,
+ * uncached), and its device address */
+int vhip_host_alloc_coherent(void **p, size_t bytes);
+int vhip_host_device_ptr(void **dev, void *host);
+int vhip_stream_done(void *stream);   /* 0 idle, 1 work pending, else error */
+int vhip_stream_sync(void *stream);
+/* host-side release store / acquire load of a word the device reads or
+ * writes (coherent host memory) */
+void vhip_host_publish(unsigned *p, unsigned v);
+unsigned vhip_host_peek(const unsigned *p);
+
 /* The no-allocation batched entry point's scratch: a static device array of
  * VHIP_NOALLOC_BYTES per GPU (control counters at offset 0, descriptors and
  * work items from VHIP_NOALLOC_HEAD on).  vhip_noalloc_setup (called by

@@ -21,6 +21,7 @@ bool g_ready = false;
 std::mutex g_init_mu;
 std::atomic<size_t> g_bytes{0};
 std::atomic<size_t> g_budget{0};
+std::atomic<int> g_reserved_cus{0};   // CUs left to a resident kernel
 
 // sizes of live allocations, so frees are charged back against the budget
 std::mutex g_alloc_mu;
@@ -64,7 +65,14 @@ extern "C" int vhip_lds_max(int *bytes) {
 }
 
 extern "C" int vhip_cu_count(int *cus) {
-    *cus = g_cus;
+    const int r = g_reserved_cus.load();
+    *cus = (r > 0 && r < g_cus) ? g_cus - r : g_cus;
+    return 0;
+}
+
+extern "C" int vhip_set_reserved_cus(int n) {
+    if (n < 0) return -1;
+    g_reserved_cus.store(n);
     return 0;
 }
 
@@ -350,6 +358,35 @@ extern "C" int vhip_event_query(void *ev) {
 extern "C" int vhip_host_alloc(void **p, size_t bytes) {
     *p = nullptr;
     return (int)hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault);
+}
+
+extern "C" int vhip_host_alloc_coherent(void **p, size_t bytes) {
+    *p = nullptr;
+    return (int)hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocCoherent | hipHostMallocMapped);
+}
+
+extern "C" int vhip_host_device_ptr(void **dev, void *host) {
+    *dev = nullptr;
+    return (int)hipHostGetDevicePointer(dev, host, 0);
+}
+
+extern "C" void vhip_host_publish(unsigned *p, unsigned v) {
+    __atomic_store_n(p, v, __ATOMIC_RELEASE);
+}
+
+extern "C" unsigned vhip_host_peek(const unsigned *p) {
+    return __atomic_load_n(p, __ATOMIC_ACQUIRE);
+}
+
+extern "C" int vhip_stream_sync(void *stream) {
+    return (int)hipStreamSynchronize((hipStream_t)stream);
+}
+
+extern "C" int vhip_stream_done(void *stream) {
+    const hipError_t e = hipStreamQuery((hipStream_t)stream);
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) return 1;
+    return (int)e;
 }
 
 extern "C" int vhip_host_free(void *p) {

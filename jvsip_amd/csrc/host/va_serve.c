@@ -266,6 +266,7 @@ typedef struct {
     long *reqs;
     char *direct;          /* per admitted request: 1 = own kernel, 0 = batched */
     int nreqs, busy, seq;
+    int blocked;           /* its stream waits behind the resident kernel */
 } VA_lane;
 
 static int va_lane_init(VA_lane *l, int max_jobs)
@@ -408,6 +409,185 @@ static long va_req_items(const vsip_amd_request *q, void *dummy, VA_ptw_cache *p
     return items;
 }
 
+/* ---- the resident path: one persistent kernel for the whole stream ------- */
+
+/* requests of at most this many elements whose length runs in one pass of
+ * the batched workgroup go to the resident kernel; VSIP_AMD_SERVE_RESIDENT
+ * overrides (0 turns the path off) */
+#define VA_RES_ELEMS 131072L
+/* resident workgroups (one per CU; persistent launches of the other requests
+ * size their grids to the remaining CUs); VSIP_AMD_SERVE_RESIDENT_WGS */
+#define VA_RES_WGS 32
+/* polls after which a waiting resident workgroup gives up (a bound, not a
+ * timeout the stream is expected to reach); VSIP_AMD_SERVE_SPIN_LIMIT */
+#define VA_RES_SPIN (1u << 22)
+
+static long va_env_long(const char *name, long dflt)
+{
+    const char *e = getenv(name);
+    return e ? atol(e) : dflt;
+}
+
+typedef struct {
+    long nslots, nitems, grid;
+    long *slot_of;          /* per request: its slot, or -1 */
+    long *req_of;           /* per slot: the request */
+    unsigned *item_end;     /* per slot: items published once it is */
+    char *hblk;             /* coherent host memory: slots, then htail, hstop, hdone[] */
+    vhip_fftm_desc *hdesc;  /* host addresses */
+    unsigned *htail, *hstop, *hdone;
+    vhip_resident_args args;   /* device addresses */
+    void *stream;
+    int launched;
+    int unreserved;         /* its CUs were handed back to the lanes */
+    int lanes_back;         /* the lanes on its queue were taken back */
+} VA_res;
+
+/* Start the resident kernel.  From here until the stream ends, the host must
+ * make no call that can wait for the device (allocation, free, a synchronous
+ * copy, a module load): the kernel waits for the host to publish work, so
+ * such a call would only return when a waiting workgroup gives up.  Every
+ * schedule, twiddle table and buffer of the stream exists before this. */
+static int va_res_start(VA_res *rs)
+{
+    int rc;
+    if (rs->nslots == 0) return 0;
+    /* the persistent launches of the other requests leave these CUs alone */
+    rc = vhip_set_reserved_cus((int)rs->grid);
+    if (!rc) rc = vhip_resident_launch(&rs->args, (int)rs->grid, VA_BATCH_LDS, rs->stream);
+    if (!rc) rs->launched = 1;
+    else vhip_set_reserved_cus(0);
+    return rc;
+}
+
+/* wait up to `seconds` for a stream to go idle: 0 idle, 1 still busy, else error */
+static int va_stream_wait(void *stream, double seconds)
+{
+    double t0 = va_now();
+    struct timespec ts;
+    ts.tv_sec = 0;
+    ts.tv_nsec = 20000;
+    for (;;) {
+        int q = vhip_stream_done(stream);
+        if (q != 1) return q;
+        if (va_now() - t0 > seconds) return 1;
+        nanosleep(&ts, NULL);
+    }
+}
+
+/* stop and release the resident kernel: 0, or -1 when it did not end within
+ * the bound (its memory is then left allocated, since it may still read it) */
+static int va_res_fini(VA_res *rs)
+{
+    if (rs->launched) {
+        vhip_host_publish(rs->hstop, 1u);   /* waiting workgroups stop at once */
+        if (va_stream_wait(rs->stream, 10.0) != 0) {
+            va_error("serve", "the resident kernel did not end", -1);
+            memset(rs, 0, sizeof(*rs));
+            return -1;
+        }
+        vhip_set_reserved_cus(0);
+    }
+    if (rs->stream) vhip_stream_destroy(rs->stream);
+    if (rs->hblk) vhip_host_free(rs->hblk);
+    if (rs->args.items) vhip_free((void *)rs->args.items);
+    if (rs->args.req_left) vhip_free(rs->args.req_left);
+    if (rs->args.ctl) vhip_free(rs->args.ctl);
+    free(rs->slot_of);
+    free(rs->req_of);
+    free(rs->item_end);
+    memset(rs, 0, sizeof(*rs));
+    return 0;
+}
+
+/* route requests to the resident kernel and lay out every item of the
+ * stream in arrival order (the kernel starts later, in va_res_start) */
+static int va_res_setup(VA_res *rs, const vsip_amd_request *reqs, long nreq, void *dummy, VA_ptw_cache *pc,
+                        int grid_max)
+{
+    long max_elems = va_env_long("VSIP_AMD_SERVE_RESIDENT", VA_RES_ELEMS);
+    long wgs = va_env_long("VSIP_AMD_SERVE_RESIDENT_WGS", VA_RES_WGS);
+    unsigned spin = (unsigned)va_env_long("VSIP_AMD_SERVE_SPIN_LIMIT", (long)VA_RES_SPIN);
+    long i, k, ni = 0;
+    size_t slots_bytes, ctl_off, blk_bytes;
+    vhip_batch_item *items = NULL;
+    unsigned *left = NULL;
+    void *dev = NULL;
+    int rc = 0;
+    memset(rs, 0, sizeof(*rs));
+    rs->slot_of = (long *)malloc(sizeof(long) * (size_t)nreq);
+    rs->req_of = (long *)malloc(sizeof(long) * (size_t)nreq);
+    rs->item_end = (unsigned *)malloc(sizeof(unsigned) * (size_t)nreq);
+    if (!rs->slot_of || !rs->req_of || !rs->item_end) return -1;
+    for (i = 0; i < nreq; i++) {
+        const vsip_amd_request *q = &reqs[i];
+        vhip_fftm_desc d[VA_MAX_DESC];
+        int nd;
+        rs->slot_of[i] = -1;
+        if (max_elems <= 0 || wgs <= 0 || (long)q->rows * q->n > max_elems || va_req_two_pass(q)) continue;
+        nd = va_req_descs(q, dummy, dummy, NULL, pc, d);
+        if (nd != 1) continue;
+        ni += (d[0].nvec + d[0].vpb - 1) / d[0].vpb;
+        if (ni >= 0x7fffffffL) return -1;
+        rs->slot_of[i] = rs->nslots;
+        rs->req_of[rs->nslots] = i;
+        rs->item_end[rs->nslots] = (unsigned)ni;
+        rs->nslots++;
+    }
+    rs->nitems = ni;
+    if (rs->nslots == 0) return 0;
+    /* coherent host block: descriptor slots (aligned to their stride), then
+     * htail and hstop on a line of their own, then the done flags */
+    slots_bytes = (size_t)rs->nslots * VHIP_RES_DESC_STRIDE;
+    ctl_off = slots_bytes;
+    blk_bytes = ctl_off + 128 + sizeof(unsigned) * (size_t)rs->nslots;
+    rc = vhip_host_alloc_coherent((void **)&rs->hblk, blk_bytes);
+    if (!rc && ((size_t)rs->hblk & (VHIP_RES_DESC_STRIDE - 1)) != 0) rc = -1;
+    if (rc) return rc ? rc : -1;
+    memset(rs->hblk, 0, blk_bytes);
+    rs->hdesc = (vhip_fftm_desc *)rs->hblk;
+    rs->htail = (unsigned *)(rs->hblk + ctl_off);
+    rs->hstop = rs->htail + 1;
+    rs->hdone = (unsigned *)(rs->hblk + ctl_off + 128);
+    rc = vhip_host_device_ptr(&dev, rs->hblk);
+    if (rc) return rc;
+    rs->args.hdesc = (const vhip_fftm_desc *)dev;
+    rs->args.htail = (const unsigned *)((char *)dev + ctl_off);
+    rs->args.hstop = rs->args.htail + 1;
+    rs->args.hdone = (unsigned *)((char *)dev + ctl_off + 128);
+    /* every item of the stream, slot by slot, and each slot's item count */
+    items = (vhip_batch_item *)malloc(sizeof(vhip_batch_item) * (size_t)ni);
+    left = (unsigned *)malloc(sizeof(unsigned) * (size_t)rs->nslots);
+    if (!items || !left) { rc = -1; goto done; }
+    for (k = 0; k < rs->nslots; k++) {
+        unsigned first = k ? rs->item_end[k - 1] : 0u, g;
+        vhip_fftm_desc d[VA_MAX_DESC];
+        if (va_req_descs(&reqs[rs->req_of[k]], dummy, dummy, NULL, pc, d) != 1) { rc = -1; goto done; }
+        left[k] = rs->item_end[k] - first;
+        for (g = 0; g < left[k]; g++) {
+            items[first + g].desc = (int)k;
+            items[first + g].pad_ = 0;
+            items[first + g].vbase = (long)g * d[0].vpb;
+        }
+    }
+    rc = vhip_malloc((void **)&rs->args.items, sizeof(vhip_batch_item) * (size_t)ni);
+    if (!rc) rc = vhip_malloc((void **)&rs->args.req_left, sizeof(unsigned) * (size_t)rs->nslots);
+    if (!rc) rc = vhip_malloc((void **)&rs->args.ctl, sizeof(unsigned) * 4);
+    if (!rc) rc = vhip_memcpy_h2d((void *)rs->args.items, items, sizeof(vhip_batch_item) * (size_t)ni);
+    if (!rc) rc = vhip_memcpy_h2d(rs->args.req_left, left, sizeof(unsigned) * (size_t)rs->nslots);
+    if (!rc) rc = vhip_memset(rs->args.ctl, 0, sizeof(unsigned) * 4);
+    if (!rc) rc = vhip_stream_create(&rs->stream);
+    if (rc) goto done;
+    rs->args.nitems = (unsigned)ni;
+    rs->args.spin_limit = spin ? spin : 1u;
+    rs->grid = wgs < grid_max ? wgs : grid_max;
+    if (rs->grid > ni) rs->grid = ni;
+done:
+    free(items);
+    free(left);
+    return rc;
+}
+
 int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_f *in_pool,
                      const vsip_cblock_f *out_pool, size_t hbm_budget, int nstreams, int max_jobs,
                      vsip_amd_result *res, char *stats, size_t stats_len)
@@ -422,13 +602,22 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
     signed char *req_nwork = NULL;  /* scratch matrices the request needs */
     long *fifo = NULL, head = 0, tail = 0, next = 0, done = 0, i, direct = 0;
     long batches = 0, jobs_launched = 0, max_inflight = 0, batched_launches = 0, batched_reqs = 0;
-    int L, rc = 0, grid_max = 0;
+    /* resident requests: arrived (rfifo), published and not yet done (rlive) */
+    long *rfifo = NULL, rhead = 0, rtail = 0, *rlive = NULL, nlive = 0, res_left = 0, polls = 0;
+    long last_done = 0;
+    double t_prog;
+    VA_res rs;
+    /* one lane and one job per group: every request runs alone, in arrival
+     * order, on either path */
+    const int serial = (nstreams == 1 && max_jobs == 1);
+    int L, rc = 0, grid_max = 0, lanes_usable = nstreams;
     long direct_elems = va_direct_elems();
     double t0, tend;
     if (va_ensure_init()) return -1;
     if (!in_pool || !in_pool->b.dev_r || nreq <= 0 || nstreams < 1 || max_jobs < 1) return -1;
     memset(&pc, 0, sizeof(pc));
     memset(&plc, 0, sizeof(plc));
+    memset(&rs, 0, sizeof(rs));
     if (va_arena_init(&arena, hbm_budget)) { va_error("serve", "HBM budget allocation failed", -1); return -1; }
     lanes = (VA_lane *)calloc((size_t)nstreams, sizeof(VA_lane));
     mem = (VA_reqmem *)calloc((size_t)nreq, sizeof(VA_reqmem));
@@ -437,7 +626,12 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
     req_out = (void **)malloc(sizeof(void *) * (size_t)nreq);
     req_direct = (char *)malloc((size_t)nreq);
     req_nwork = (signed char *)malloc((size_t)nreq);
-    if (!lanes || !mem || !fifo || !req_in || !req_out || !req_direct || !req_nwork) { rc = -1; goto out; }
+    rfifo = (long *)malloc(sizeof(long) * (size_t)nreq);
+    rlive = (long *)malloc(sizeof(long) * (size_t)nreq);
+    if (!lanes || !mem || !fifo || !req_in || !req_out || !req_direct || !req_nwork || !rfifo || !rlive) {
+        rc = -1;
+        goto out;
+    }
     for (L = 0; L < nstreams && !rc; L++) rc = va_lane_init(&lanes[L], max_jobs);
     if (!rc) rc = vhip_batch_max_grid(VA_BATCH_THREADS, VA_BATCH_MAXE, VA_BATCH_LDS, &grid_max);
     /* warm the schedule caches for every length before the clock starts */
@@ -447,6 +641,9 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
         if (reqs[i].out_offset >= 0 && !out_pool) rc = -1;
     }
     if (!rc && va_plan_cache_fill(&plc, reqs, nreq)) rc = -1;
+    /* the small single-pass requests go to the resident kernel, started here */
+    if (!rc) rc = va_res_setup(&rs, reqs, nreq, in_pool->b.dev_r, &pc, grid_max);
+    res_left = rs.nslots;
     /* route every request: its own kernel for large requests and for any
      * whose batched items would not fit one lane's item table (so the FIFO
      * head can always be admitted by an idle lane), else the lane's batched
@@ -457,6 +654,12 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
         size_t len = (size_t)q->rows * (size_t)q->n * 8;
         int nb = 0, np;
         if (!pl || pl->staged || pl->npass < 1) { rc = -1; break; }
+        if (rs.slot_of && rs.slot_of[i] >= 0) {
+            req_direct[i] = 0;
+            req_nwork[i] = 0;
+            if (len + 1024 > hbm_budget) rc = -1;
+            continue;
+        }
         np = va_plain_nwork(&plc, q->n);
         req_direct[i] = ((long)q->rows * q->n >= direct_elems);
         if (!req_direct[i]) {
@@ -475,17 +678,124 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
         res[i].batch = -1;
         res[i].stream = -1;
     }
+    rc = va_res_start(&rs);
+    if (rc) { va_error("serve", "resident kernel launch failed", rc); goto out; }
+    /* Streams beyond the hardware queues of the process share them, and a
+     * launch on a queue shared with the resident kernel would wait for the
+     * whole stream.  A tiny kernel on every lane finds such lanes, which then
+     * stay unused (their probe runs when the resident kernel ends). */
+    if (rs.launched) {
+        long lane_reqs = 0;
+        int usable = 0;
+        for (L = 0; L < nstreams && !rc; L++) {
+            rc = vhip_batch_clear(lanes[L].ctl_dev, lanes[L].stream);
+            if (!rc) rc = vhip_event_record(lanes[L].event, lanes[L].stream);
+        }
+        if (rc) { va_error("serve", vhip_error_string(rc), rc); goto out; }
+        for (L = 0; L < nstreams; L++) {
+            double tq = va_now();
+            int q;
+            while ((q = vhip_event_query(lanes[L].event)) == 1 && va_now() - tq < 0.02) {
+            }
+            if (q != 0 && q != 1) { rc = q; va_error("serve", vhip_error_string(q), q); goto out; }
+            lanes[L].blocked = (q == 1);
+            usable += !lanes[L].blocked;
+        }
+        for (i = 0; i < nreq; i++) lane_reqs += (rs.slot_of[i] < 0);
+        lanes_usable = usable;
+        if (usable == 0 && lane_reqs > 0) {
+            rc = -1;
+            va_error("serve", "every lane shares a hardware queue with the resident kernel "
+                     "(GPU_MAX_HW_QUEUES too small)", nstreams);
+            goto out;
+        }
+    }
 
     t0 = va_now();
+    t_prog = t0;
     while (done < nreq) {
         double now = va_now() - t0;
         int inflight = 0;
-        while (next < nreq && reqs[next].t_arrive <= now) fifo[tail++] = next++;
+        while (next < nreq && reqs[next].t_arrive <= now) {
+            if (rs.nslots && rs.slot_of[next] >= 0) rfifo[rtail++] = next++;
+            else fifo[tail++] = next++;
+        }
+        /* resident requests: retire the finished, then publish the arrived
+         * ones in order while their output fits the budget */
+        for (i = 0; i < nlive; i++) {
+            long r = rlive[i];
+            if (!vhip_host_peek(&rs.hdone[rs.slot_of[r]])) continue;
+            res[r].t_done = va_now() - t0;
+            if (mem[r].own_out) va_arena_free(&arena, mem[r].out_off, mem[r].out_len);
+            done++;
+            res_left--;
+            rlive[i--] = rlive[--nlive];
+        }
+        while (rhead < rtail) {
+            long r = rfifo[rhead], slot = rs.slot_of[r];
+            const vsip_amd_request *q = &reqs[r];
+            vhip_fftm_desc *sd = (vhip_fftm_desc *)(rs.hblk + (size_t)slot * VHIP_RES_DESC_STRIDE);
+            void *out;
+            if (serial && (nlive > 0 || lanes[0].busy || (head < tail && fifo[head] < r))) break;
+            mem[r].out_len = (size_t)q->rows * (size_t)q->n * 8;
+            mem[r].own_out = (q->out_offset < 0);
+            mem[r].own_work = 0;
+            if (mem[r].own_out) {
+                mem[r].out_off = va_arena_alloc(&arena, mem[r].out_len);
+                if (mem[r].out_off == (size_t)-1) break;
+            }
+            out = mem[r].own_out ? (void *)(arena.base + mem[r].out_off) : va_pool_addr(out_pool, q->out_offset);
+            if (va_req_descs(q, va_pool_addr(in_pool, q->in_offset), out, NULL, &pc, sd) != 1) {
+                rc = -1;
+                va_error("serve", "resident request has no single-pass schedule", (int)q->n);
+                goto out;
+            }
+            vhip_host_publish(rs.htail, rs.item_end[slot]);
+            res[r].t_admit = va_now() - t0;
+            res[r].batch = -1;
+            res[r].stream = nstreams;   /* the resident kernel, after the lanes */
+            rlive[nlive++] = r;
+            rhead++;
+        }
+        /* every resident request published and done: the kernel is ending,
+         * and the lanes' persistent launches may use every CU again */
+        if (rs.launched && !rs.unreserved && rhead == rs.nslots && nlive == 0) {
+            vhip_set_reserved_cus(0);
+            rs.unreserved = 1;
+        }
+        /* once it has ended, the lanes that shared its queue run again */
+        if (rs.unreserved && !rs.lanes_back && (polls & 63) == 0 && vhip_stream_done(rs.stream) == 0) {
+            int still = 0;
+            for (L = 0; L < nstreams; L++) {
+                if (lanes[L].blocked && vhip_event_query(lanes[L].event) == 0) lanes[L].blocked = 0;
+                still += lanes[L].blocked;
+            }
+            rs.lanes_back = (still == 0);
+        }
+        /* the resident kernel only ends once it has claimed every item, so it
+         * must not be idle while resident requests are unfinished */
+        ++polls;
+        if (res_left > 0 && (polls & 1023) == 0) {
+            int q = vhip_stream_done(rs.stream);
+            if (q == 0) {
+                for (i = 0; i < nlive; i++)
+                    if (!vhip_host_peek(&rs.hdone[rs.slot_of[rlive[i]]])) break;
+                if (i < nlive || rhead < rs.nslots) {
+                    rc = -1;
+                    va_error("serve", "the resident kernel stopped with requests unfinished", (int)res_left);
+                    goto out;
+                }
+            } else if (q != 1) {
+                rc = q;
+                va_error("serve", vhip_error_string(q), q);
+                goto out;
+            }
+        }
         /* retire */
         for (L = 0; L < nstreams; L++) {
             VA_lane *l = &lanes[L];
             int q;
-            if (!l->busy) continue;
+            if (!l->busy || l->blocked) continue;
             q = vhip_event_query(l->event);
             if (q == 1) continue;
             if (q != 0) { rc = q; va_error("serve", vhip_error_string(q), q); goto out; }
@@ -505,7 +815,8 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
             VA_lane *l = &lanes[L];
             int nd = 0, nbatched = 0;
             long ni = 0, n1 = 0, n2b = 0, g;
-            if (l->busy) continue;
+            if (serial && (nlive > 0 || (rhead < rtail && rfifo[rhead] < fifo[head]))) break;
+            if (l->busy || l->blocked) continue;
             while (head < tail && l->nreqs < max_jobs) {
                 long r = fifo[head];
                 const vsip_amd_request *q = &reqs[r];
@@ -610,26 +921,40 @@ int vsip_amd_serve_f(const vsip_amd_request *reqs, long nreq, const vsip_cblock_
         }
         for (inflight = 0, L = 0; L < nstreams; L++) inflight += lanes[L].busy;
         if (inflight > max_inflight) max_inflight = inflight;
+        /* a bound, not a schedule: work in flight with no completion for
+         * this long means the device stopped making progress */
+        if (done != last_done || (inflight == 0 && nlive == 0)) {
+            last_done = done;
+            t_prog = va_now();
+        } else if (va_now() - t_prog > 10.0) {
+            rc = -1;
+            va_error("serve", "no request completed for 10 s", (int)(nreq - done));
+            goto out;
+        }
     }
     tend = va_now() - t0;
     if (stats && stats_len) {
-        char tmp[768];
+        char tmp[1280];
         double bytes = 0.0;
         for (i = 0; i < nreq; i++) bytes += 2.0 * (double)reqs[i].rows * (double)reqs[i].n * 8.0;
         sprintf(tmp, "{\"requests\": %ld, \"seconds\": %.6f, \"req_per_s\": %.1f, \"gb_per_s\": %.2f, "
                 "\"batches\": %ld, \"jobs_per_batch\": %.2f, \"lanes\": %d, \"max_jobs\": %d, "
                 "\"max_lanes_in_flight\": %ld, \"hbm_budget\": %lu, \"hbm_peak\": %lu, "
                 "\"direct_launched\": %ld, \"batched_requests\": %ld, \"batched_launches\": %ld, "
-                "\"requests_per_batched_launch\": %.2f, \"direct_threshold_elems\": %ld}",
+                "\"requests_per_batched_launch\": %.2f, \"direct_threshold_elems\": %ld, "
+                "\"resident_requests\": %ld, \"resident_items\": %ld, \"resident_workgroups\": %ld, "
+                "\"resident_max_elems\": %ld, \"lanes_usable\": %d}",
                 nreq, tend, (double)nreq / tend, bytes / tend / 1e9, batches,
                 batches ? (double)jobs_launched / (double)batches : 0.0, nstreams, max_jobs,
                 max_inflight, (unsigned long)hbm_budget, (unsigned long)arena.peak, direct,
                 batched_reqs, batched_launches,
-                batched_launches ? (double)batched_reqs / (double)batched_launches : 0.0, direct_elems);
+                batched_launches ? (double)batched_reqs / (double)batched_launches : 0.0, direct_elems,
+                rs.nslots, rs.nitems, rs.grid, va_env_long("VSIP_AMD_SERVE_RESIDENT", VA_RES_ELEMS), lanes_usable);
         strncpy(stats, tmp, stats_len - 1);
         stats[stats_len - 1] = '\0';
     }
 out:
+    if (va_res_fini(&rs) && !rc) rc = -1;
     vhip_sync();
     for (L = 0; lanes && L < nstreams; L++) {
         if (lanes[L].stream) vhip_stream_destroy(lanes[L].stream);
@@ -646,6 +971,8 @@ out:
     free(req_out);
     free(req_direct);
     free(req_nwork);
+    free(rfifo);
+    free(rlive);
     return rc;
 }
 

@@ -318,11 +318,11 @@ class AmdLib(VsipLib):
         for i, r in enumerate(reqs):
             arr[i] = amd_request(*r) if not isinstance(r, dict) else amd_request(**r)
         res = (amd_result * n)()
-        buf = ctypes.create_string_buffer(1024)
+        buf = ctypes.create_string_buffer(2048)
         if plain:
-            rc = self.amd_serve_plain_f(arr, n, in_pool, out_pool, budget, 1 if sync_each else 0, res, buf, 1024)
+            rc = self.amd_serve_plain_f(arr, n, in_pool, out_pool, budget, 1 if sync_each else 0, res, buf, 2048)
         else:
-            rc = self.amd_serve_f(arr, n, in_pool, out_pool, budget, lanes, max_jobs, res, buf, 1024)
+            rc = self.amd_serve_f(arr, n, in_pool, out_pool, budget, lanes, max_jobs, res, buf, 2048)
         if rc:
             raise RuntimeError("serve failed (%d): %s" % (rc, self.last_error()))
         out = np.array([(r.t_admit, r.t_done, r.batch, r.stream) for r in res],

@@ -202,6 +202,107 @@ def test_serve_request_beyond_a_lane_runs_directly(amd, monkeypatch):
     lib.cmalldestroy_f(inv)
 
 
+def _small_requests(nreq, rate, seed):
+    """requests of the serving workload's lengths small enough for the
+    resident kernel (rows * n <= 131072)"""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    reqs, out_off = [], 0
+    for i in range(nreq):
+        n = int(rng.choice([256, 1000, 1024, 2187, 4096, 6000]))
+        rows = int(rng.integers(1, min(64, 131072 // n) + 1))
+        d = -1 if rng.random() < 0.5 else 1
+        scale = 1.0 if d < 0 else 1.0 / n
+        reqs.append((float(t[i]), rows, n, d, scale, int(rng.integers(0, POOL - rows * n)), out_off))
+        out_off += rows * n
+    return reqs, out_off
+
+
+@pytest.mark.parametrize("rate", [2000.0, 1e6])
+def test_serve_resident_mix_equals_alone_plain_and_oracle(amd, pools, rate):
+    """Every request small enough runs on the resident kernel, which stays on
+    the device for the stream: at 2000 req/s its workgroups wait between
+    arrivals (the host publishes each request while the kernel runs); at 1e6
+    req/s many requests are in flight on it at once.  Each result equals the
+    request served alone (a one-request stream on the same path), the plain
+    calls and the oracle."""
+    lib = amd
+    x, inb = pools
+    reqs, total = _small_requests(40, rate, seed=21)
+    ob, ov = _out_pool(lib, total)
+    _case("resident mix at %g req/s: %d requests" % (rate, len(reqs)))
+    res, stats = lib.serve(reqs, inb, ob, budget=1 << 30, lanes=4, max_jobs=16)
+    print("  stats", stats, flush=True)
+    assert stats["resident_requests"] == len(reqs) and stats["direct_launched"] == 0
+    assert np.all(res["stream"] == 4) and np.all(res["t_done"] >= res["t_admit"])
+    mix = _read(lib, ov, total)
+    _case("each request alone")
+    for r in reqs:
+        _, st1 = lib.serve([(0.0,) + tuple(r[1:])], inb, ob, budget=1 << 28, lanes=1, max_jobs=1)
+        assert st1["resident_requests"] == 1
+    alone = _read(lib, ov, total)
+    lib.serve(reqs, inb, ob, budget=1 << 30, plain=True)
+    plain = _read(lib, ov, total)
+    for i, (t, rows, n, d, s, io, oo) in enumerate(reqs):
+        sl = slice(oo, oo + rows * n)
+        want = po.ccfftm(x[io:io + rows * n], fftm.dense_view(rows, n), direction=d, scale=s)
+        e_alone, e_plain, e_or = po.rel_l2(mix[sl], alone[sl]), po.rel_l2(mix[sl], plain[sl]), po.rel_l2(mix[sl], want)
+        print("  req %2d %3dx%-5d dir %+d: alone %.1e plain %.1e oracle %.1e" % (i, rows, n, d, e_alone, e_plain, e_or),
+              flush=True)
+        assert e_alone == 0.0 and e_plain <= 1e-6 and e_or <= 1e-5, i
+    lib.cmalldestroy_f(ov)
+
+
+def test_serve_resident_and_lanes_together(amd, pools, monkeypatch):
+    """Small requests on the resident kernel while large ones (and two-pass
+    lengths) launch on the lanes beside it, with outputs from the budget; the
+    same stream with the resident path off gives the same results."""
+    lib = amd
+    x, inb = pools
+    reqs, total = _requests(48, rate=50000.0, seed=23, max_rows=48)
+    reqs += [(reqs[-1][0] + 1e-5 * (k + 1), 3, 32768, -1, 1.0, 11 * k, total + 3 * 32768 * k) for k in range(4)]
+    total += 4 * 3 * 32768
+    ob, ov = _out_pool(lib, total)
+    _case("resident and lanes together: %d requests" % len(reqs))
+    _, st = lib.serve(reqs, inb, ob, budget=1 << 30, lanes=4, max_jobs=16)
+    print("  stats", st, flush=True)
+    assert st["resident_requests"] > 0 and st["direct_launched"] > 0
+    both = _read(lib, ov, total)
+    monkeypatch.setenv("VSIP_AMD_SERVE_RESIDENT", "0")
+    _, st0 = lib.serve(reqs, inb, ob, budget=1 << 30, lanes=4, max_jobs=16)
+    assert st0["resident_requests"] == 0
+    off = _read(lib, ov, total)
+    for i, (t, rows, n, d, s, io, oo) in enumerate(reqs):
+        sl = slice(oo, oo + rows * n)
+        want = po.ccfftm(x[io:io + rows * n], fftm.dense_view(rows, n), direction=d, scale=s)
+        assert po.rel_l2(both[sl], off[sl]) <= 1e-6 and po.rel_l2(both[sl], want) <= 1e-5, i
+    lib.cmalldestroy_f(ov)
+
+
+def test_serve_resident_spin_limit_stops_cleanly(amd, pools, monkeypatch):
+    """A waiting resident workgroup gives up after VSIP_AMD_SERVE_SPIN_LIMIT
+    polls: with a limit of 1 and requests 2 ms apart the kernel stops before
+    the stream is over, the server reports an error instead of waiting for
+    it, and the next stream runs normally."""
+    lib = amd
+    x, inb = pools
+    reqs, total = _small_requests(6, 500.0, seed=25)
+    ob, ov = _out_pool(lib, total)
+    monkeypatch.setenv("VSIP_AMD_SERVE_SPIN_LIMIT", "1")
+    _case("resident kernel with a spin limit of 1")
+    with pytest.raises(RuntimeError, match="resident"):
+        lib.serve(reqs, inb, ob, budget=1 << 30, lanes=2, max_jobs=4)
+    lib.amd_clear_error()
+    monkeypatch.delenv("VSIP_AMD_SERVE_SPIN_LIMIT")
+    _, st = lib.serve(reqs, inb, ob, budget=1 << 30, lanes=2, max_jobs=4)
+    assert st["resident_requests"] == len(reqs)
+    got = _read(lib, ov, total)
+    for t, rows, n, d, s, io, oo in reqs:
+        want = po.ccfftm(x[io:io + rows * n], fftm.dense_view(rows, n), direction=d, scale=s)
+        assert po.rel_l2(got[oo:oo + rows * n], want) <= 1e-5
+    lib.cmalldestroy_f(ov)
+
+
 def test_serve_plain_back_to_back(amd, pools):
     """The plain baseline without a synchronisation per call gives the same
     results and completion times after the arrivals."""
