@@ -58,7 +58,7 @@ __device__ __forceinline__ int load_twiddles(cpx<float> *tab, const vhip_fftm_de
 // the first reading their twiddles from the LDS table `tw` (lg as loaded).
 template <int MAXE>
 __device__ __forceinline__ void batch_generic(const vhip_fftm_desc &d, long vbase, int nv, char *smem,
-                                              const cpx<float> *tw, int lg) {
+                                              const cpx<float> *tw, int lg, long &nxt) {
     cpx<float> *lds = (cpx<float> *)smem;
     Shared sh;
     sh.voff_in = (long *)(smem + (size_t)d.vpb * d.lds_stride * sizeof(cpx<float>));
@@ -79,9 +79,12 @@ __device__ __forceinline__ void batch_generic(const vhip_fftm_desc &d, long vbas
     bc.hi = tw + (1 << lg);
     bc.lg = lg;
     const int nst = d.nst;
-    int Ns = 1;
-    for (int s = 0; s < nst; ++s) {
-        run_stage<float, MAXE, VHIP_C2C, true, true>(d, sh, lds, s, Ns, nv, s == 0, s == nst - 1, isgn, osgn,
+    // stage 0 outside the loop: the claim is taken on every path after it
+    run_stage<float, MAXE, VHIP_C2C, true, true>(d, sh, lds, 0, 1, nv, true, nst == 1, isgn, osgn, scale, &bc);
+    nxt = uniform(nxt);
+    int Ns = d.radix[0];
+    for (int s = 1; s < nst; ++s) {
+        run_stage<float, MAXE, VHIP_C2C, true, true>(d, sh, lds, s, Ns, nv, false, s == nst - 1, isgn, osgn,
                                                      scale, &bc);
         Ns *= d.radix[s];
     }
@@ -90,17 +93,21 @@ __device__ __forceinline__ void batch_generic(const vhip_fftm_desc &d, long vbas
 // One work item: the power-of-two stages for rows of 256..8192 points on
 // contiguous rows, else the generic stages with the LDS twiddle table `tw`
 // (reloaded when the item length differs from *twn).
+// nxt: lane 0's claim of the next item (issued before this item's loads);
+// every wave takes its value right after the item's first stage, where the
+// wait for it is already covered by the wait for the item's own data (memory
+// operations retire in issue order) and no later access is held up by it.
 template <int MAXE>
 __device__ __forceinline__ void run_item(const vhip_fftm_desc &d, long vbase, char *smem, cpx<float> *tw,
-                                         int &twn, int &lg) {
+                                         int &twn, int &lg, long &nxt) {
     const int l2 = __builtin_amdgcn_readfirstlane(p2::pow2_batch_log2(d));
     switch (l2) {
-    case 8: p2::pow2_batch_item<8>(d, vbase, smem); break;
-    case 9: p2::pow2_batch_item<9>(d, vbase, smem); break;
-    case 10: p2::pow2_batch_item<10>(d, vbase, smem); break;
-    case 11: p2::pow2_batch_item<11>(d, vbase, smem); break;
-    case 12: p2::pow2_batch_item<12>(d, vbase, smem); break;
-    case 13: p2::pow2_batch_item<13>(d, vbase, smem); break;
+    case 8: p2::pow2_batch_item<8>(d, vbase, smem, nxt); break;
+    case 9: p2::pow2_batch_item<9>(d, vbase, smem, nxt); break;
+    case 10: p2::pow2_batch_item<10>(d, vbase, smem, nxt); break;
+    case 11: p2::pow2_batch_item<11>(d, vbase, smem, nxt); break;
+    case 12: p2::pow2_batch_item<12>(d, vbase, smem, nxt); break;
+    case 13: p2::pow2_batch_item<13>(d, vbase, smem, nxt); break;
     default: {
         const long left = d.nvec - vbase;
         const int nv = left < d.vpb ? (int)left : d.vpb;
@@ -110,9 +117,22 @@ __device__ __forceinline__ void run_item(const vhip_fftm_desc &d, long vbase, ch
             lg = load_twiddles(tw, d);
             twn = d.n;
         }
-        batch_generic<MAXE>(d, vbase, nv, smem, tw, lg);
+        batch_generic<MAXE>(d, vbase, nv, smem, tw, lg, nxt);
     }
     }
+}
+
+// Read every 64-byte line of a descriptor at once: the stages read its fields
+// as scalars in several dependent steps, which would otherwise each wait for
+// a line of their own.
+__device__ __forceinline__ void touch_desc(const vhip_fftm_desc *__restrict__ d) {
+    const char *p = (const char *)d;
+    static_assert(sizeof(vhip_fftm_desc) <= 512, "eight lines");
+    unsigned t0 = *(const unsigned *)(p + 0), t1 = *(const unsigned *)(p + 64), t2 = *(const unsigned *)(p + 128),
+             t3 = *(const unsigned *)(p + 192), t4 = *(const unsigned *)(p + 256),
+             t5 = *(const unsigned *)(p + 320), t6 = *(const unsigned *)(p + 384),
+             t7 = *(const unsigned *)(p + sizeof(vhip_fftm_desc) - 4);
+    asm volatile("" ::"s"(t0), "s"(t1), "s"(t2), "s"(t3), "s"(t4), "s"(t5), "s"(t6), "s"(t7));
 }
 
 // ctl[0]: next item; ctl[1]: workgroups finished.  The last workgroup to
@@ -138,7 +158,8 @@ __global__ void __launch_bounds__(512)
         if (threadIdx.x == 0) nxt = (long)next_index(&ctl[0]);   // overlaps the item below
         const vhip_batch_item item = items[it];
         const int di = __builtin_amdgcn_readfirstlane(item.desc);
-        run_item<MAXE>(descs[di], item.vbase, smem, s_tw, twn, lg);
+        touch_desc(&descs[di]);
+        run_item<MAXE>(descs[di], item.vbase, smem, s_tw, twn, lg, nxt);
         __syncthreads();  // every lane has read s_item
         if (threadIdx.x == 0) s_item = nxt;
     }
@@ -238,7 +259,8 @@ __global__ void __launch_bounds__(512)
                      t6 = *(const unsigned *)(sd + 384), t7 = *(const unsigned *)(sd + 448);
             asm volatile("" ::"s"(t0), "s"(t1), "s"(t2), "s"(t3), "s"(t4), "s"(t5), "s"(t6), "s"(t7));
         }
-        run_item<MAXE>(*(const vhip_fftm_desc *)sd, item.vbase, smem, s_tw, twn, lg);
+        long none = 0;
+        run_item<MAXE>(*(const vhip_fftm_desc *)sd, item.vbase, smem, s_tw, twn, lg, none);
         // every wave's stores have reached L2 at the barrier; the count's
         // agent-scope release then writes the XCD's L2 back once for all
         __syncthreads();
