@@ -31,11 +31,6 @@ __device__ __forceinline__ unsigned next_index(unsigned *head) {
     return __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ long uniform(long v) {
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long)v >> 32));
-    return (long)(((unsigned long)hi << 32) | lo);
-}
 
 // Twiddle table of the generic items, copied into LDS from the one the host
 // appends to the schedule's stage twiddles (d.btw_off): W_n^e = lo[e mod
@@ -58,7 +53,7 @@ __device__ __forceinline__ int load_twiddles(cpx<float> *tab, const vhip_fftm_de
 // the first reading their twiddles from the LDS table `tw` (lg as loaded).
 template <int MAXE>
 __device__ __forceinline__ void batch_generic(const vhip_fftm_desc &d, long vbase, int nv, char *smem,
-                                              const cpx<float> *tw, int lg, long &nxt) {
+                                              const cpx<float> *tw, int lg, unsigned &nxt) {
     cpx<float> *lds = (cpx<float> *)smem;
     Shared sh;
     sh.voff_in = (long *)(smem + (size_t)d.vpb * d.lds_stride * sizeof(cpx<float>));
@@ -81,7 +76,7 @@ __device__ __forceinline__ void batch_generic(const vhip_fftm_desc &d, long vbas
     const int nst = d.nst;
     // stage 0 outside the loop: the claim is taken on every path after it
     run_stage<float, MAXE, VHIP_C2C, true, true>(d, sh, lds, 0, 1, nv, true, nst == 1, isgn, osgn, scale, &bc);
-    nxt = uniform(nxt);
+    nxt = __builtin_amdgcn_readfirstlane(nxt);
     int Ns = d.radix[0];
     for (int s = 1; s < nst; ++s) {
         run_stage<float, MAXE, VHIP_C2C, true, true>(d, sh, lds, s, Ns, nv, false, s == nst - 1, isgn, osgn,
@@ -97,9 +92,12 @@ __device__ __forceinline__ void batch_generic(const vhip_fftm_desc &d, long vbas
 // every wave takes its value right after the item's first stage, where the
 // wait for it is already covered by the wait for the item's own data (memory
 // operations retire in issue order) and no later access is held up by it.
+// Item indices are 32-bit: with a 64-bit index the compiler gave the claim's
+// register to the next item's index and waited for every store of the item
+// at the loop head.
 template <int MAXE>
 __device__ __forceinline__ void run_item(const vhip_fftm_desc &d, long vbase, char *smem, cpx<float> *tw,
-                                         int &twn, int &lg, long &nxt) {
+                                         int &twn, int &lg, unsigned &nxt) {
     const int l2 = __builtin_amdgcn_readfirstlane(p2::pow2_batch_log2(d));
     switch (l2) {
     case 8: p2::pow2_batch_item<8>(d, vbase, smem, nxt); break;
@@ -146,16 +144,16 @@ __global__ void __launch_bounds__(512)
     // fields load as scalars, so buffer descriptors built from them stay in
     // SGPRs
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    __shared__ long s_item;
+    __shared__ unsigned s_item;
     __shared__ cpx<float> s_tw[kTwCap];
     int twn = 0, lg = 0;
-    if (threadIdx.x == 0) s_item = (long)next_index(&ctl[0]);
+    if (threadIdx.x == 0) s_item = next_index(&ctl[0]);
     for (;;) {
         __syncthreads();  // s_item is set; the previous item is done with LDS
-        const long it = uniform(s_item);
+        const long it = (long)__builtin_amdgcn_readfirstlane(s_item);
         if (it >= nitems) break;
-        long nxt = 0;
-        if (threadIdx.x == 0) nxt = (long)next_index(&ctl[0]);   // overlaps the item below
+        unsigned nxt = 0;
+        if (threadIdx.x == 0) nxt = next_index(&ctl[0]);   // overlaps the item below
         const vhip_batch_item item = items[it];
         const int di = __builtin_amdgcn_readfirstlane(item.desc);
         touch_desc(&descs[di]);
@@ -259,7 +257,7 @@ __global__ void __launch_bounds__(512)
                      t6 = *(const unsigned *)(sd + 384), t7 = *(const unsigned *)(sd + 448);
             asm volatile("" ::"s"(t0), "s"(t1), "s"(t2), "s"(t3), "s"(t4), "s"(t5), "s"(t6), "s"(t7));
         }
-        long none = 0;
+        unsigned none = 0;
         run_item<MAXE>(*(const vhip_fftm_desc *)sd, item.vbase, smem, s_tw, twn, lg, none);
         // every wave's stores have reached L2 at the barrier; the count's
         // agent-scope release then writes the XCD's L2 back once for all
@@ -396,7 +394,10 @@ extern "C" int vhip_batch_clear(unsigned *ctl, void *stream) {
 extern "C" int vhip_batch_launch(const vhip_fftm_desc *descs, const vhip_batch_item *items, long nitems1,
                                  long nitems2, unsigned *ctl, int threads, int maxe, int lds_bytes,
                                  int grid, void *stream) {
-    if (threads != 512 || !vhipk::pick_batch(maxe) || grid < 1 || nitems1 < 0 || nitems2 < 0) return -1;
+    // item indices are 32-bit (the claim counter and the kernel's index)
+    if (threads != 512 || !vhipk::pick_batch(maxe) || grid < 1 || nitems1 < 0 || nitems2 < 0 ||
+        nitems1 > 0x7fffffffL - grid || nitems2 > 0x7fffffffL - grid)
+        return -1;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipSuccess;
     long g1 = nitems1 < grid ? nitems1 : grid;

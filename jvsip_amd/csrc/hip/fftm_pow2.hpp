@@ -1040,7 +1040,7 @@ __device__ __forceinline__ void pow2_pipe_big_body(const vhip_fftm_desc &d, cpx<
 // vector vbase of the transform d, with the batched workgroup of 512 lanes.
 // nxt: the batched kernel's claim of its next item, made uniform after stage 0
 template <int LOG2N>
-__device__ __forceinline__ void pow2_batch_item(const vhip_fftm_desc &d, long vbase, char *smem, long &nxt) {
+__device__ __forceinline__ void pow2_batch_item(const vhip_fftm_desc &d, long vbase, char *smem, unsigned &nxt) {
     constexpr int VPB = 8192 >> LOG2N;
     using P = P2<LOG2N>;
     const cpx<float> *in = (const cpx<float> *)d.in.re;
@@ -1056,11 +1056,7 @@ __device__ __forceinline__ void pow2_batch_item(const vhip_fftm_desc &d, long vb
     p2_load<LOG2N, VPB, false>(x, make_rsrc(in + vbase * ivs, group_bytes<LOG2N, VPB>(d.nvec, g, ivs)), ivs, isgn);
     const rsrc_t o = make_rsrc(out + vbase * ovs, group_bytes<LOG2N, VPB>(d.nvec, g, ovs));
     p2_stage<LOG2N, VPB, 0, false>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
-    {
-        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)nxt);
-        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long)nxt >> 32));
-        nxt = (long)(((unsigned long)hi << 32) | lo);
-    }
+    nxt = __builtin_amdgcn_readfirstlane(nxt);
     if constexpr (P::NST > 1) p2_stage<LOG2N, VPB, (P::NST > 1 ? 1 : 0), false>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
     if constexpr (P::NST > 2) p2_stage<LOG2N, VPB, (P::NST > 2 ? 2 : 0), false>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
     if constexpr (P::NST > 3) p2_stage<LOG2N, VPB, (P::NST > 3 ? 3 : 0), false>(x, lds, tw, d.tw_off, o, ovs, osgn, scale);
