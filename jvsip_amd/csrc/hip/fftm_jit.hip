@@ -72,7 +72,8 @@ bool jit_on() {
 // the template instance of a schedule; empty when the schedule has no
 // compiled form (more than 8 stages, or not a single pass)
 std::string instance(int prec, int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
-    if (nst < 1 || nst > 8 || min < 0 || min > 7 || mout < 0 || mout > 3 || (prec != 4 && prec != 8)) return "";
+    if (nst < 1 || nst > 8 || min < 0 || min > 7 || mout < 0 || mout > 15 || (prec != 4 && prec != 8)) return "";
+    if ((mout & VHIP_JIT_PTW) && (mout & VHIP_JIT_FUSED)) return "";  // no power accumulation in a split pass
     if ((min & VHIP_JIT_REAL) && (mout & VHIP_JIT_FUSED)) return "";   // no power accumulation after a real input
     int r[8];
     for (int s = 0; s < 8; ++s) r[s] = s < nst ? radix[s] : 1;
@@ -197,6 +198,7 @@ static bool desc_layouts(const vhip_fftm_desc *d, int &min, int &mout) {
     if (min < 0 || mout < 0) return false;
     if (d->tap) min |= VHIP_JIT_FUSED;
     if (d->acc) mout |= VHIP_JIT_FUSED;
+    if (d->ptw) mout |= VHIP_JIT_PTW;
     if (d->kind == VHIP_R2C) min |= VHIP_JIT_REAL;
     return true;
 }

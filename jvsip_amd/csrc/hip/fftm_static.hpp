@@ -35,6 +35,7 @@
 #define ST_ACROSS 1
 #define ST_FUSED 2
 #define ST_REAL 4
+#define ST_PTW 8    // store side: the split passes' post-twiddle (vhip_fftm_desc.ptw)
 
 namespace vhipk {
 namespace st {
@@ -80,6 +81,7 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
     constexpr int LIN = MIN & 1, LOUT = MOUT & 1;
     constexpr bool TAP = (MIN & ST_FUSED) != 0, ACC = (MOUT & ST_FUSED) != 0;
     constexpr bool REAL = (MIN & ST_REAL) != 0;
+    constexpr bool PTW = (MOUT & ST_PTW) != 0;
     constexpr int RS = (int)sizeof(T);
     // lane mapping of this stage: the global layout's for the first and last
     // stage (the load's when one stage does both), along the vector otherwise
@@ -88,6 +90,9 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
     cpx<T> x[B][R];
+    // this stage's twiddles, loaded (from L2) ahead of the LDS reads and the
+    // barrier, so their latency overlaps both instead of following them
+    cpx<T> w[B][R > 1 ? R - 1 : 1];
     int vv[B], jj[B];
     bool ok[B];
 #pragma unroll
@@ -99,6 +104,16 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
         vv[i] = v;
         jj[i] = j;
         ok[i] = (B * NT <= NBT || b < NBT) && v < io.nv;
+        if (NS > 1 && ok[i]) {
+            const int k = j % NS;
+            const int toff = d.tw_off[S];
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[i][r - 1] = Buf<T>::template ldc<0>(io.tw, (toff + k + (r - 1) * NS) * ES, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        const int v = vv[i], j = jj[i];
         if (ok[i]) {
             if (FIRST) {
 #pragma unroll
@@ -142,18 +157,24 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
         }
     }
     if (!FIRST) __syncthreads();
+    // every butterfly of the lane first, then every write: a store's data
+    // registers are then never reused while stores are in flight (the
+    // compiler would wait for the stores before reusing them)
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        if (ok[i]) {
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) x[i][r] = cmul(x[i][r], w[i][r - 1]);
+            }
+            Dft<T, R>::run(x[i]);
+        }
+    }
 #pragma unroll
     for (int i = 0; i < B; ++i) {
         if (ok[i]) {
             const int v = vv[i], j = jj[i];
             const int k = (NS > 1) ? j % NS : 0;
-            if (NS > 1) {
-                const int toff = d.tw_off[S];
-#pragma unroll
-                for (int r = 1; r < R; ++r)
-                    x[i][r] = cmul(x[i][r], Buf<T>::template ldc<0>(io.tw, (toff + k + (r - 1) * NS) * ES, 0));
-            }
-            Dft<T, R>::run(x[i]);
             if (LAST && REAL) {     // Y in natural order, for the combine
                 cpx<T> *dst = lds + v * LS;
 #pragma unroll
@@ -162,7 +183,7 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     cpx<T> y = x[i][r];
-                    if (d.ptw) {    // split passes: e^{-2 pi i t / N} from the two-level table
+                    if (PTW) {    // split passes: e^{-2 pi i t / N} from the two-level table
                         const int t = io.vidx[v] * (j + r * NBV);
                         const cpx<T> *w = (const cpx<T> *)d.ptw;
                         y = cmul(y, cmul(w[t & ((1 << d.ptw_shift) - 1)], w[(1 << d.ptw_shift) + (t >> d.ptw_shift)]));
@@ -225,7 +246,8 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
     io.voff_in = voff;
     io.voff_out = voff + VPB;
     io.vidx = voff + 2 * VPB;
-    if (ACROSS || d.ptw) {
+    constexpr bool PTW = (MOUT & ST_PTW) != 0;
+    if (ACROSS || PTW) {
         for (int t = threadIdx.x; t < io.nv; t += NT) {
             const long v = vbase + t;
             voff[t] = (int)vec_off(d.in, v);

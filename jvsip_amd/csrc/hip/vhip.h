@@ -242,11 +242,14 @@ int vhip_rng_fill(const vhip_rng_desc *d, void *stream);
  * else returns -100.  Layouts: VHIP_JIT_CONTIG (interleaved rows of unit
  * element stride, single-level vector stride), VHIP_JIT_ACROSS (interleaved,
  * lanes across vectors, any strides within vhip_view.span); + VHIP_JIT_FUSED
- * on the load side fuses vhip_fftm_desc.tap, on the store side .acc. */
+ * on the load side fuses vhip_fftm_desc.tap, on the store side .acc;
+ * + VHIP_JIT_PTW on the store side applies .ptw (compiled in, so a kernel
+ * without it has no post-twiddle branch at its stores). */
 #define VHIP_JIT_CONTIG 0
 #define VHIP_JIT_ACROSS 1
 #define VHIP_JIT_FUSED 2
 #define VHIP_JIT_REAL 4     /* load side: real input, real-to-complex transform */
+#define VHIP_JIT_PTW 8      /* store side: post-twiddle (the split's passes before the last) */
 int vhip_jit_prepare(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in,
                      int layout_out);
 int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);

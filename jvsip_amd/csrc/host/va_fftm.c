@@ -241,7 +241,8 @@ static void va_jit_prepare(VA_fftm *p)
         const VA_sched *a = &p->pass[k];
         if (va_jit_lds_ok(a, &a->across, p->prec))
             vhip_jit_prepare(p->prec, a->n, a->nst, a->radix, a->across.vpb, a->across.threads,
-                             k == p->npass - 1 ? VHIP_JIT_CONTIG : VHIP_JIT_ACROSS, VHIP_JIT_ACROSS);
+                             k == p->npass - 1 ? VHIP_JIT_CONTIG : VHIP_JIT_ACROSS,
+                             k == p->npass - 1 ? VHIP_JIT_ACROSS : VHIP_JIT_ACROSS | VHIP_JIT_PTW);
     }
 }
 
@@ -261,7 +262,8 @@ int va_jit_ready(const VA_fftm *p)
     for (k = 0; k < p->npass; k++) {
         const VA_sched *a = &p->pass[k];
         if (!vhip_jit_ready(p->prec, a->n, a->nst, a->radix, a->across.vpb, a->across.threads,
-                            k == p->npass - 1 ? VHIP_JIT_CONTIG : VHIP_JIT_ACROSS, VHIP_JIT_ACROSS))
+                            k == p->npass - 1 ? VHIP_JIT_CONTIG : VHIP_JIT_ACROSS,
+                            k == p->npass - 1 ? VHIP_JIT_ACROSS : VHIP_JIT_ACROSS | VHIP_JIT_PTW))
             return 0;
     }
     return 1;

@@ -13,7 +13,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libvsip_amd.so")
+# VSIP_AMD_LIB: another build of the library (for example to compare two builds)
+LIB_PATH = os.environ.get("VSIP_AMD_LIB") or os.path.join(_HERE, "lib", "libvsip_amd.so")
 
 c_vi = ctypes.c_ulong      # vsip_length / vsip_offset / vsip_index
 c_st = ctypes.c_long       # vsip_stride

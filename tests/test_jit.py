@@ -39,7 +39,8 @@ def test_planner_schedules_compile(lib, so, prec, n, nvec):
     assert s["rc"] == 0
     rad = (ctypes.c_int * len(s["radix"]))(*s["radix"])
     for shape, lin, lout in (("along", 0, 0), ("across", 1, 1), ("across", 0, 1), ("along", 2, 2),
-                             ("across", 3, 3), ("along", 4, 0), ("across", 5, 1), ("across", 6, 1)):
+                             ("across", 3, 3), ("along", 4, 0), ("across", 5, 1), ("across", 6, 1),
+                             ("across", 1, 9)):
         vpb, thr, _ = s[shape]
         size = ctypes.c_size_t()
         rc = so.vhip_jit_compile_check(prec, n, len(s["radix"]), rad, vpb, thr, lin, lout, b"gfx950",
