@@ -106,11 +106,48 @@ int va_factor_pack(int n, int *radix)
     return nst;
 }
 
+/* a radix sequence for length n from VSIP_AMD_RADICES ("n:r1,r2,...;..."),
+ * used only when n is not a power of two (the power-of-two kernels take the
+ * even split), the radices multiply to n and the kernels have each of them
+ * (2..16 for float, 2, 3, 4, 5, 7, 8, 11, 13 for double); else 0 */
+static int va_factor_env(int n, int prec, int *radix)
+{
+    const char *e = getenv("VSIP_AMD_RADICES");
+    if (n < 2 || (n & (n - 1)) == 0) return 0;
+    while (e && *e) {
+        char *end;
+        long len = strtol(e, &end, 10);
+        int nst = 0, ok = 1;
+        long prod = 1;
+        if (end == e || *end != ':') return 0;
+        e = end + 1;
+        for (;;) {
+            long r = strtol(e, &end, 10);
+            if (end == e) { ok = 0; break; }
+            e = end;
+            if (nst >= VHIP_MAX_STAGES || r < 2 || r > 16 || (prec != 4 && r != 2 && r != 3 && r != 4 && r != 5 &&
+                                                              r != 7 && r != 8 && r != 11 && r != 13))
+                ok = 0;
+            else radix[nst++] = (int)r;
+            prod *= r;
+            if (*e != ',') break;
+            e++;
+        }
+        if (ok && len == n && prod == n && nst > 0) return nst;
+        while (*e && *e != ';') e++;
+        if (*e == ';') e++;
+    }
+    return 0;
+}
+
 /* the default schedule's factorisation: powers of two split evenly (the
  * specialised power-of-two kernels expect that split), other float lengths
- * packed into radices <= 16, double lengths in radix <= 8 and prime stages */
+ * packed into radices <= 16, double lengths in radix <= 8 and prime stages;
+ * VSIP_AMD_RADICES replaces it for the lengths it names (tuning) */
 static int va_factor(int n, int prec, int *radix)
 {
+    int nst = va_factor_env(n, prec, radix);
+    if (nst > 0) return nst;
     if (prec == 4 && (n & (n - 1)) != 0) return va_factor_pack(n, radix);
     return va_factor_variant(n, prec, prec == 4 ? 4 : 3, 0, radix);
 }
