@@ -10,9 +10,17 @@
 // (gen_jit_sources.py).  A schedule that does not compile or load is
 // remembered as unavailable and runs on the generic kernel.
 //
+// With VSIP_AMD_JIT_CACHE_DIR set, compiled code objects are also kept on
+// disk, one file per schedule, and later processes load them instead of
+// compiling (the file name hashes the instance, the architecture, hipRTC's
+// version and the embedded device sources, so a changed library or device
+// never loads a stale object).
+//
 // VSIP_AMD_JIT=0 turns the path off.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
+#include <unistd.h>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -139,6 +147,92 @@ int compile(const std::string &name, const std::string &arch, std::vector<char> 
     return rc;
 }
 
+// ---- code objects on disk (VSIP_AMD_JIT_CACHE_DIR) ----------------------
+
+uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+    return h;
+}
+
+uint64_t sources_hash() {
+    static const uint64_t h = [] {
+        uint64_t v = 0xcbf29ce484222325ull;
+        for (int i = 0; i < vhip_jit_nheaders; ++i) {
+            v = fnv1a(v, vhip_jit_header_names[i], strlen(vhip_jit_header_names[i]) + 1);
+            v = fnv1a(v, vhip_jit_header_texts[i], strlen(vhip_jit_header_texts[i]) + 1);
+        }
+        return v;
+    }();
+    return h;
+}
+
+// the cache file of one instance; empty when the cache is off
+std::string cache_path(const std::string &name, const std::string &arch) {
+    const char *dir = getenv("VSIP_AMD_JIT_CACHE_DIR");
+    if (!dir || !*dir) return std::string();
+    int maj = 0, mnr = 0;
+    hiprtcVersion(&maj, &mnr);
+    uint64_t h = sources_hash();
+    h = fnv1a(h, name.c_str(), name.size() + 1);
+    h = fnv1a(h, arch.c_str(), arch.size() + 1);
+    h = fnv1a(h, &maj, sizeof maj);
+    h = fnv1a(h, &mnr, sizeof mnr);
+    char file[48];
+    snprintf(file, sizeof file, "/vsip_amd_%016llx.co", (unsigned long long)h);
+    return std::string(dir) + file;
+}
+
+// file layout: a line "vsip_amd jit 1 <lowered name>", then the code object
+const char kMagic[] = "vsip_amd jit 1 ";
+
+bool cache_read(const std::string &path, std::vector<char> &code, std::string &lowered) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::vector<char> all;
+    char buf[65536];
+    size_t got;
+    while ((got = fread(buf, 1, sizeof buf, f)) > 0) all.insert(all.end(), buf, buf + got);
+    fclose(f);
+    const size_t m = sizeof kMagic - 1;
+    if (all.size() <= m || memcmp(all.data(), kMagic, m) != 0) return false;
+    size_t nl = m;
+    while (nl < all.size() && all[nl] != '\n') ++nl;
+    if (nl + 1 >= all.size()) return false;
+    lowered.assign(all.data() + m, all.data() + nl);
+    code.assign(all.begin() + (long)nl + 1, all.end());
+    // only an ELF image goes to the loader; anything else is compiled anew
+    return !lowered.empty() && code.size() > 64 && memcmp(code.data(), "\x7f" "ELF", 4) == 0;
+}
+
+// written under a temporary name and renamed, so that no process reads a
+// partial file; a failure leaves no file and is not an error
+void cache_write(const std::string &path, const std::vector<char> &code, const std::string &lowered) {
+    const std::string tmp = path + ".tmp." + std::to_string((long)getpid());
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    bool ok = fwrite(kMagic, 1, sizeof kMagic - 1, f) == sizeof kMagic - 1 &&
+              fwrite(lowered.data(), 1, lowered.size(), f) == lowered.size() && fputc('\n', f) != EOF &&
+              fwrite(code.data(), 1, code.size(), f) == code.size();
+    ok = (fclose(f) == 0) && ok;
+    if (!ok || rename(tmp.c_str(), path.c_str()) != 0) remove(tmp.c_str());
+}
+
+// load a code object and find the instance's function in it
+bool load_module(const std::vector<char> &code, const std::string &lowered, Entry &e) {
+    if (hipModuleLoadData(&e.mod, code.data()) != hipSuccess) {
+        e.mod = nullptr;
+        return false;
+    }
+    if (hipModuleGetFunction(&e.fn, e.mod, lowered.c_str()) != hipSuccess) {
+        (void)hipModuleUnload(e.mod);
+        e.mod = nullptr;
+        e.fn = nullptr;
+        return false;
+    }
+    return true;
+}
+
 }  // namespace
 
 // Compile (no device needed) the kernel of one schedule for `arch`
@@ -169,9 +263,11 @@ extern "C" int vhip_jit_prepare(int prec, int n, int nst, const int *radix, int 
     Entry e;
     std::vector<char> code;
     std::string lowered;
-    if (compile(name, arch_option(), code, lowered) == 0 &&
-        hipModuleLoadData(&e.mod, code.data()) == hipSuccess) {
-        if (hipModuleGetFunction(&e.fn, e.mod, lowered.c_str()) != hipSuccess) e.fn = nullptr;
+    const std::string arch = arch_option();
+    const std::string path = cache_path(name, arch);
+    if (path.empty() || !cache_read(path, code, lowered) || !load_module(code, lowered, e)) {
+        if (compile(name, arch, code, lowered) == 0 && load_module(code, lowered, e) && !path.empty())
+            cache_write(path, code, lowered);
     }
     (void)hipGetLastError();
     g_cache[name] = e;

@@ -203,22 +203,39 @@ static int va_jit_lds_ok(const VA_sched *s, const VA_launch *L, int prec)
     return (long)L->vpb * s->lds_stride * 2 * prec + (long)L->vpb * 12 <= lds_max;
 }
 
-/* Complex and real-to-complex plans (FP32, FP64) whose passes the
- * schedule-generic kernel would run (every pass except the FP32
- * power-of-two kernels' contiguous rows)
- * and that move at least VSIP_AMD_JIT_MIN_ELEMS elements (default 2^18) get
- * their schedules compiled now, for the layouts the plan's views normally
- * have: contiguous rows (ROW), lanes across columns (COL), and the passes
- * of the splits.  Execution finds the compiled kernel when the layout
- * matches. */
-static void va_jit_prepare(VA_fftm *p)
+/* Does a plan's expected use repay compiling its schedules?  A compile
+ * takes 0.3-1 s once per schedule and process (or one file read with
+ * VSIP_AMD_JIT_CACHE_DIR); a compiled kernel saves about half of a small
+ * launch (MI355X: 16 x 1000 rows 7.3 -> 4.0 us, 1 x 1000 6.8 -> 3.6 us, the
+ * 1-D 2^16 split 21.7 -> 10.5 us; the generic kernel's launch takes ~12x
+ * the instruction-cache misses, its paths being chosen at run time).  VSIPL
+ * gives the expected number of uses at creation: ntimes 0 stands for an
+ * unbounded number, and such plans compile from 2^12 elements; a plan with
+ * ntimes > 0 compiles when ntimes x elements >= 2^30.  VSIP_AMD_JIT_MIN_ELEMS
+ * replaces the rule by a plain floor on elements. */
+static int va_jit_worth(const VA_fftm *p)
 {
     const char *e = getenv("VSIP_AMD_JIT_MIN_ELEMS");
-    long min_elems = e ? atol(e) : (1L << 18);
+    const long elems = (long)p->mN * (long)p->MN;
+    if (e) return elems >= atol(e);
+    if (elems < (1L << 12)) return 0;
+    if (p->ntimes == 0) return 1;
+    return (double)p->ntimes * (double)elems >= (double)(1L << 30);
+}
+
+/* Complex and real-to-complex plans (FP32, FP64) whose passes the
+ * schedule-generic kernel would run (every pass except the FP32
+ * power-of-two kernels' contiguous rows) and whose use repays it
+ * (va_jit_worth) get their schedules compiled now, for the layouts the
+ * plan's views normally have: contiguous rows (ROW), lanes across columns
+ * (COL), and the passes of the splits.  Execution finds the compiled kernel
+ * when the layout matches. */
+static void va_jit_prepare(VA_fftm *p)
+{
     long n = (long)p->MN;
     int k, real = (p->type == VA_RCFFTOP) ? VHIP_JIT_REAL : 0;
     if ((p->type != VA_CCFFTIP && p->type != VA_CCFFTOP && p->type != VA_RCFFTOP) || p->staged) return;
-    if ((long)p->mN * n < min_elems) return;
+    if (!va_jit_worth(p)) return;
     if (getenv("VSIP_AMD_JIT") && getenv("VSIP_AMD_JIT")[0] == '0') return;
     p->jit = 1;
     if (p->npass == 1) {

@@ -105,7 +105,7 @@ static int va_describe(const VA_fftm *p, char *buf, size_t len)
                    types[p->type & 3], p->prec, (unsigned long)p->MN, (unsigned long)p->mN);
     pos += va_describe_passes(p, tmp + pos);
     if (p->npass >= 1)
-        pos += sprintf(tmp + pos, ", \"compiled_schedule\": %s", va_jit_ready(p) ? "true" : "false");
+        pos += sprintf(tmp + pos, ", \"compiled_schedule\": %s", (p->jit && va_jit_ready(p)) ? "true" : "false");
     if (p->staged && p->inner) {
         pos += sprintf(tmp + pos, ", \"staged\": \"%s\", \"scratch_length\": %lu, \"chunk\": %lu, "
                        "\"inner\": {", p->staged == 2 ? "bluestein" : "real", (unsigned long)p->sm,

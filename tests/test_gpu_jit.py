@@ -8,6 +8,8 @@ oracle (rel L2 <= 1e-5) and the schedule-generic kernel (<= 1e-6; the two run
 the same arithmetic).
 """
 import json
+import os
+import subprocess
 import sys
 
 import numpy as np
@@ -195,3 +197,84 @@ def test_compiled_schedule_real_to_complex(amd, monkeypatch, m, n, major):
     g = fftm.rcfftm(amd, x.ravel(), xv, z0, zv, major=major, scale=0.5)
     monkeypatch.delenv("VSIP_AMD_JIT")
     _check("vs generic kernel", z, g, 1e-6)
+
+
+@pytest.mark.parametrize("m,n,ntimes,want", [(16, 1000, 0, True), (16, 1000, 1, False), (16, 1000, 1 << 20, True),
+                                             (8, 243, 0, False), (64, 2187, 100, False)])
+def test_compile_rule_follows_ntimes(amd, monkeypatch, m, n, ntimes, want):
+    """Without VSIP_AMD_JIT_MIN_ELEMS a plan compiles its schedule when its
+    expected uses repay it: ntimes 0 (VSIPL: unbounded uses) from 2^12
+    elements, ntimes > 0 when ntimes x elements >= 2^30."""
+    monkeypatch.delenv("VSIP_AMD_JIT_MIN_ELEMS", raising=False)
+    monkeypatch.delenv("VSIP_AMD_JIT", raising=False)
+    _case("ccfftmip_f ROW %dx%d ntimes %d: compiled %s" % (m, n, ntimes, want))
+    p = amd.ccfftmip_create_f(m, n, 1.0, VSIP_FFT_FWD, VSIP_ROW, ntimes, 0)
+    assert p, amd.last_error()
+    d = json.loads(amd.describe(p))
+    amd.fftm_destroy_f(p)
+    assert bool(d.get("compiled_schedule")) is want
+    x = datagen.complex_data(96, (m, n))
+    v = fftm.dense_view(m, n)
+    y = fftm.ccfftm(amd, x.ravel(), v)
+    _check("vs oracle", y, po.ccfftm(x.ravel(), v), 1e-5)
+
+
+_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+_CHILD = r"""
+import json, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from jvsip_amd import datagen, fftm
+from jvsip_amd.vsip import load, VSIP_ROW, VSIP_FFT_FWD
+amd = load()
+m, n = int(sys.argv[2]), int(sys.argv[3])
+p = amd.ccfftmip_create_f(m, n, 1.0, VSIP_FFT_FWD, VSIP_ROW, 0, 0)
+assert p, amd.last_error()
+d = json.loads(amd.describe(p))
+amd.fftm_destroy_f(p)
+y = fftm.ccfftm(amd, datagen.complex_data(97, (m, n)).ravel(), fftm.dense_view(m, n))
+np.save(sys.argv[4], y)
+print(json.dumps({"compiled": d.get("compiled_schedule")}), flush=True)
+"""
+
+
+def _child_plan(cache_dir, m, n, out):
+    """One plan in a fresh process (an in-process cache would hide the disk)."""
+    env = dict(os.environ, VSIP_AMD_JIT_CACHE_DIR=str(cache_dir))
+    env.pop("VSIP_AMD_JIT_MIN_ELEMS", None)
+    env.pop("VSIP_AMD_JIT", None)
+    r = subprocess.run([sys.executable, "-c", _CHILD, _ROOT, str(m), str(n), str(out)], env=env,
+                       capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return json.loads(r.stdout.strip().splitlines()[-1]), np.load(out)
+
+
+def test_code_object_cache_on_disk(tmp_path):
+    """VSIP_AMD_JIT_CACHE_DIR: the first process compiles and writes the
+    schedule's code object; the next loads it (the file is not rewritten);
+    a damaged file is compiled anew and replaced.  Every run matches the
+    oracle."""
+    m, n = 3, 5005                       # [13, 11, 7, 5]: a schedule no other test compiles
+    want = po.ccfftm(datagen.complex_data(97, (m, n)).ravel(), fftm.dense_view(m, n))
+    cache = tmp_path / "cache"
+    cache.mkdir()
+    _case("code object cache: first process compiles")
+    d, y = _child_plan(cache, m, n, tmp_path / "y1.npy")
+    assert d["compiled"] is True
+    _check("vs oracle", y, want, 1e-5)
+    files = sorted(cache.glob("vsip_amd_*.co"))
+    assert len(files) == 1 and files[0].read_bytes().startswith(b"vsip_amd jit 1 ")
+    ino = files[0].stat().st_ino
+    _case("code object cache: second process loads the file")
+    d, y2 = _child_plan(cache, m, n, tmp_path / "y2.npy")
+    assert d["compiled"] is True and files[0].stat().st_ino == ino
+    np.testing.assert_array_equal(y2, y)
+    _case("code object cache: a damaged file is compiled anew")
+    files[0].write_bytes(b"vsip_amd jit 1 nothing\n" + b"\0" * 256)
+    d, y3 = _child_plan(cache, m, n, tmp_path / "y3.npy")
+    assert d["compiled"] is True
+    np.testing.assert_array_equal(y3, y)
+    data = files[0].read_bytes()
+    assert data.startswith(b"vsip_amd jit 1 ") and b"\x7fELF" in data[:4096]
+    assert not list(cache.glob("*.tmp.*"))
