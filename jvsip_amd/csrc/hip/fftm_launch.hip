@@ -42,9 +42,12 @@ extern "C" int vhip_fftm_launch(const vhip_fftm_desc *d, void *stream) {
     if (!d || d->nvec <= 0 || d->vpb <= 0 || d->threads <= 0 || d->threads > 1024) return -1;
     int cus = 256;
     vhip_cu_count(&cus);
-    int rc = vhipk::pow2_try_launch(d, stream, cus > 0 ? cus : 256);
+    // a plan that compiled its schedule runs it first (small power-of-two
+    // rows compile because their compiled schedule beats the one-group
+    // power-of-two kernel); the power-of-two kernels take the rest they fit
+    int rc = d->compiled ? vhip_jit_try_launch(d, stream) : -100;
     if (rc != -100) return rc;
-    rc = vhip_jit_try_launch(d, stream);
+    rc = vhipk::pow2_try_launch(d, stream, cus > 0 ? cus : 256);
     if (rc != -100) return rc;
     kernel_fn f = pick_kernel(d);
     if (!f) return -2;

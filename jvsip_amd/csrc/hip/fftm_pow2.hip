@@ -231,6 +231,17 @@ int pow2_try_launch(const vhip_fftm_desc *d, void *stream, int cus) {
     return -100;
 }
 
+// the test p2_pipe_gen_launch makes, for the planner
+static int p2_pipelined(int n, long nvec) {
+    static const bool off = getenv("VSIP_AMD_P2_PIPE") && atoi(getenv("VSIP_AMD_P2_PIPE")) == 0;
+    if (off || n < 256 || n > 2048 || (n & (n - 1)) != 0) return 0;
+    int cus = 256;
+    vhip_cu_count(&cus);
+    if (cus < 1) cus = 256;
+    const long vpb = 4096 / n;
+    return (nvec + vpb - 1) / vpb >= (long)p2_pipe_min(4) * 2L * cus;
+}
+
 int pow2_set_lds_limit(int bytes) {
     int rc = 0;
     for (const void *f : {(const void *)pow2_pipe_big_kernel<13, true>, (const void *)pow2_pipe_big_kernel<13, false>,
@@ -249,3 +260,5 @@ int pow2_set_lds_limit(int bytes) {
 }
 
 }  // namespace vhipk
+
+extern "C" int vhip_p2_pipelined(int n, long nvec) { return vhipk::p2_pipelined(n, nvec); }

@@ -259,6 +259,10 @@ int vhip_jit_compile_check(int prec, int n, int nst, const int *radix, int vpb, 
                            int layout_out, const char *arch, size_t *code_bytes);
 const char *vhip_jit_last_error(void);
 
+/* would complex FP32 rows of n points (256..2048), nvec of them, take the
+ * pipelined power-of-two kernel (1) or its one-group form (0) */
+int vhip_p2_pipelined(int n, long nvec);
+
 /* kernels */
 int vhip_xform_launch(const vhip_xform_desc *d, void *stream);
 int vhip_mat_copy(int prec, int cplx, const vhip_mat *src, const vhip_mat *dst,

@@ -225,7 +225,9 @@ static int va_jit_worth(const VA_fftm *p)
 
 /* Complex and real-to-complex plans (FP32, FP64) whose passes the
  * schedule-generic kernel would run (every pass except the FP32
- * power-of-two kernels' contiguous rows) and whose use repays it
+ * power-of-two kernels' contiguous rows; complex rows of 256..2048 in
+ * batches too small for the pipelined kernel compile as well) and whose use
+ * repays it
  * (va_jit_worth) get their schedules compiled now, for the layouts the
  * plan's views normally have: contiguous rows (ROW), lanes across columns
  * (COL), and the passes of the splits.  Execution finds the compiled kernel
@@ -237,21 +239,31 @@ static void va_jit_prepare(VA_fftm *p)
     if ((p->type != VA_CCFFTIP && p->type != VA_CCFFTOP && p->type != VA_RCFFTOP) || p->staged) return;
     if (!va_jit_worth(p)) return;
     if (getenv("VSIP_AMD_JIT") && getenv("VSIP_AMD_JIT")[0] == '0') return;
-    p->jit = 1;
+    p->jit_worth = 1;
     if (p->npass == 1) {
         const VA_sched *s = &p->pass[0];
         if (p->major == VSIP_ROW) {
-            if (p->prec == 4 && (n & (n - 1)) == 0 && n >= 256 && n <= 16384 && !getenv("VSIP_AMD_NO_POW2"))
-                return;                                                   /* the power-of-two kernels */
+            /* the power-of-two kernels, except complex rows of 256..2048 in
+             * batches below the pipelined kernel's: there the compiled
+             * schedule beats the one-group kernel (MI355X, per launch:
+             * 64 x 1024 5.4 -> 3.5 us, 4096 x 1024 14.2 -> 12.0, 128 x 512
+             * 5.2 -> 3.0, 2048 x 2048 13.8 -> 11.9; from 4096 points even) */
+            if (p->prec == 4 && (n & (n - 1)) == 0 && n >= 256 && n <= 16384 && !getenv("VSIP_AMD_NO_POW2") &&
+                (real || n > 2048 || vhip_p2_pipelined((int)n, (long)p->mN)))
+                return;                                   /* p->jit stays 0: never a compiled kernel */
+            p->jit = 1;
             if (va_jit_lds_ok(s, &s->along, p->prec))
                 vhip_jit_prepare(p->prec, s->n, s->nst, s->radix, s->along.vpb, s->along.threads,
                                  VHIP_JIT_CONTIG | real, VHIP_JIT_CONTIG);
-        } else if (va_jit_lds_ok(s, &s->across, p->prec)) {
-            vhip_jit_prepare(p->prec, s->n, s->nst, s->radix, s->across.vpb, s->across.threads,
-                             VHIP_JIT_ACROSS | real, VHIP_JIT_ACROSS);
+        } else {
+            p->jit = 1;
+            if (va_jit_lds_ok(s, &s->across, p->prec))
+                vhip_jit_prepare(p->prec, s->n, s->nst, s->radix, s->across.vpb, s->across.threads,
+                                 VHIP_JIT_ACROSS | real, VHIP_JIT_ACROSS);
         }
         return;
     }
+    p->jit = 1;
     /* split: every pass maps lanes across vectors at its global sides except
      * the last one's load, which reads scratch rows */
     for (k = 0; k < p->npass; k++) {

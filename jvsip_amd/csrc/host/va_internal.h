@@ -134,6 +134,7 @@ typedef struct VA_fftm {
      * between, for Bluestein), POST pass scratch -> view; `chunk` vectors per
      * slab */
     int jit;                 /* 1: schedules compiled at creation may run this plan */
+    int jit_worth;           /* 1: the plan's uses repay compiling (also fused launches' schedules) */
     int staged;              /* 0 none, 1 real transform around a long c2c, 2 Bluestein */
     vsip_length sm;
     vsip_length chunk;

@@ -80,7 +80,7 @@ int vsip_amd_komega_f(const vsip_fftm_f *rcfftm, const vsip_fftm_f *ccfftm, cons
         va_error(fn, "ts_taper must be an admitted vector of the row length", -1);
         return -1;
     }
-    if (pr->jit && vhip_jit_prepare_desc(&d[0]) == 0) d[0].compiled = 1;
+    if (pr->jit_worth && vhip_jit_prepare_desc(&d[0]) == 0) d[0].compiled = 1;
     rc = vhip_fftm_launch(&d[0], vhip_stream());
     if (rc) { va_error(fn, vhip_error_string(rc), rc); return -1; }
 
@@ -95,7 +95,7 @@ int vsip_amd_komega_f(const vsip_fftm_f *rcfftm, const vsip_fftm_f *ccfftm, cons
     d[0].acc_gain = gain;
     /* the column transform's schedule compiled with the taper and the power
      * accumulation (once per process; the generic kernel otherwise) */
-    if (pc->jit && vhip_jit_prepare_desc(&d[0]) == 0) d[0].compiled = 1;
+    if (pc->jit_worth && vhip_jit_prepare_desc(&d[0]) == 0) d[0].compiled = 1;
     rc = vhip_fftm_launch(&d[0], vhip_stream());
     if (rc) { va_error(fn, vhip_error_string(rc), rc); return -1; }
     return 0;

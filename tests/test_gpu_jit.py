@@ -278,3 +278,45 @@ def test_code_object_cache_on_disk(tmp_path):
     data = files[0].read_bytes()
     assert data.startswith(b"vsip_amd jit 1 ") and b"\x7fELF" in data[:4096]
     assert not list(cache.glob("*.tmp.*"))
+
+
+@pytest.mark.parametrize("m,n", [(64, 1024), (300, 256), (100, 512), (16, 2048), (1000, 1024), (5, 2048)])
+def test_small_power_of_two_rows_both_kernels(amd, monkeypatch, m, n):
+    """Complex rows of 256..2048 in batches below the pipelined kernel's
+    compile their schedule (it beats the one-group power-of-two kernel);
+    with VSIP_AMD_JIT=0 they run on the one-group kernel.  Both against the
+    oracle (1e-5) and each other (1e-6)."""
+    monkeypatch.delenv("VSIP_AMD_JIT_MIN_ELEMS", raising=False)
+    monkeypatch.delenv("VSIP_AMD_JIT", raising=False)
+    x = datagen.complex_data(98, (m, n))
+    v = fftm.dense_view(m, n)
+    p = amd.ccfftmip_create_f(m, n, 1.0, VSIP_FFT_FWD, VSIP_ROW, 0, 0)
+    assert p, amd.last_error()
+    d = json.loads(amd.describe(p))
+    amd.fftm_destroy_f(p)
+    assert d.get("compiled_schedule") is True
+    _case("ccfftmip_f ROW %dx%d inverse (compiled schedule)" % (m, n))
+    y = fftm.ccfftm(amd, x.ravel(), v, direction=VSIP_FFT_INV, scale=1.0 / n)
+    _check("vs oracle", y, po.ccfftm(x.ravel(), v, direction=VSIP_FFT_INV, scale=1.0 / n), 1e-5)
+    monkeypatch.setenv("VSIP_AMD_JIT", "0")
+    p = amd.ccfftmip_create_f(m, n, 1.0, VSIP_FFT_FWD, VSIP_ROW, 0, 0)
+    d = json.loads(amd.describe(p))
+    amd.fftm_destroy_f(p)
+    assert d.get("compiled_schedule") is False
+    _case("ccfftmip_f ROW %dx%d inverse (one-group power-of-two kernel)" % (m, n))
+    g = fftm.ccfftm(amd, x.ravel(), v, direction=VSIP_FFT_INV, scale=1.0 / n)
+    _check("vs oracle", g, po.ccfftm(x.ravel(), v, direction=VSIP_FFT_INV, scale=1.0 / n), 1e-5)
+    _check("compiled vs power-of-two kernel", y, g, 1e-6)
+
+
+def test_large_power_of_two_rows_stay_on_their_kernels(amd, monkeypatch):
+    """4096-point rows and batches that fill the pipelined kernel never
+    compile a schedule."""
+    monkeypatch.delenv("VSIP_AMD_JIT_MIN_ELEMS", raising=False)
+    monkeypatch.delenv("VSIP_AMD_JIT", raising=False)
+    for m, n in ((64, 4096), (8192, 1024), (4096, 2048), (32768, 256)):
+        p = amd.ccfftmip_create_f(m, n, 1.0, VSIP_FFT_FWD, VSIP_ROW, 0, 0)
+        assert p, amd.last_error()
+        d = json.loads(amd.describe(p))
+        amd.fftm_destroy_f(p)
+        assert d.get("compiled_schedule") is False, (m, n)
