@@ -28,7 +28,7 @@ import numpy as np
 import pytest
 
 from jvsip_amd import datagen, fftm
-from jvsip_amd.fftm import View
+from jvsip_amd.fftm import View, View1
 from jvsip_amd.vsip import VSIP_ROW, VSIP_COL, VSIP_FFT_FWD, VSIP_FFT_INV
 from oracle import pyoracle as po
 
@@ -190,4 +190,118 @@ def test_sweep_covers_the_planner_classes():
             seen.add("radix11/13")
     for want in ("ip", "op", "rc", "cr", 4, 8, VSIP_ROW, VSIP_COL, "pow2", "bluestein", "compiled", "split",
                  "radix11/13"):
+        assert want in seen, want
+
+
+# ---- the 1-D family (vsip_ccfftip/op, rcfftop, crfftop) --------------------
+
+NCASES_1D = 64
+_PRIMES_1D = [17, 31, 97, 127, 257, 521, 1009, 4099, 8191]
+
+
+def _draw_1d(seed):
+    """One 1-D case: (kind, prec, length, scale, direction, rng)."""
+    rng = np.random.default_rng(5000 + seed)
+    kind = ["ip", "ip", "op", "rc", "cr"][int(rng.integers(0, 5))]
+    prec = 8 if rng.random() < 0.25 else 4
+    cls = rng.random()
+    if cls < 0.28:
+        L = 1 << int(rng.integers(1, 21))
+    elif cls < 0.5:
+        L = int(rng.choice(_SMOOTH)) * int(rng.choice([1, 1, 8, 27]))
+    elif cls < 0.6:
+        L = int(rng.choice(_ELEVEN))
+    elif cls < 0.8:
+        L = int(rng.choice(_PRIMES_1D)) * int(rng.choice([1, 2, 4]))
+    elif cls < 0.88:
+        L = int(rng.choice([1 << 21, 3 << 20, 3 ** 13]))        # the three-pass split
+    else:
+        L = int(rng.integers(1, 9))
+    if kind in ("rc", "cr") and L % 2:
+        L *= 2
+    if prec == 8:
+        L = min(L, 1 << 20)
+    scale = float(rng.choice([1.0, 0.5, 1.0 / max(L, 1)]))
+    direction = VSIP_FFT_FWD if rng.random() < 0.5 else VSIP_FFT_INV
+    return kind, prec, L, scale, direction, rng
+
+
+def _vlayout(rng, n):
+    """A vector view of length n in a block of its own: stride +-1, 2 or 3,
+    sometimes offset."""
+    st = int(rng.choice([1, 1, 1, 2, 3, -1, -2]))
+    off = int(rng.choice([0, 0, 5]))
+    if st < 0:
+        off += (n - 1) * -st
+    return View1(off, st, n)
+
+
+def _vlen(v):
+    return v.offset + max(0, (v.length - 1) * v.stride) + 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(NCASES_1D))
+def test_sweep_1d(amd, seed):
+    kind, prec, L, scale, direction, rng = _draw_1d(seed)
+    ctype = np.complex64 if prec == 4 else np.complex128
+    rtype = np.float32 if prec == 4 else np.float64
+    h = L // 2 + 1
+    if kind in ("ip", "op"):
+        xv = _vlayout(rng, L)
+        yv = _vlayout(rng, L)
+    elif kind == "rc":
+        xv, yv = _vlayout(rng, L), _vlayout(rng, h)
+    else:
+        xv, yv = _vlayout(rng, h), _vlayout(rng, L)
+    print("\n[case] 1-D seed %d: %s_%s length %d scale %g dir %d views %s -> %s"
+          % (seed, {"ip": "ccfftip", "op": "ccfftop", "rc": "rcfftop", "cr": "crfftop"}[kind],
+             "f" if prec == 4 else "d", L, scale, direction, tuple(xv), tuple(yv)), flush=True)
+    sys.stdout.flush()
+    tol = TOL[prec]
+    xm, ym = fftm.as_matrix(xv), fftm.as_matrix(yv)
+    if kind == "ip":
+        x = datagen.complex_data(seed, (_vlen(xv),), dtype=ctype)
+        y = fftm.ccfft(amd, x.copy(), xv, direction=direction, scale=scale)
+        _check("vs oracle", y, po.ccfftm(x, xm, 0, direction, scale), tol)
+        out = _outside(x.size, xm)
+        np.testing.assert_array_equal(y[out], x[out])
+    elif kind == "op":
+        x = datagen.complex_data(seed, (_vlen(xv),), dtype=ctype)
+        y0 = datagen.complex_data(seed + 5000, (_vlen(yv),), dtype=ctype)
+        xr, y = fftm.ccfft(amd, x.copy(), xv, direction=direction, scale=scale, out_blk=y0.copy(), out_view=yv)
+        np.testing.assert_array_equal(xr, x)
+        _check("vs oracle", y, po.ccfftm(x, xm, 0, direction, scale, out_blk=y0, out_view=ym), tol)
+        out = _outside(y0.size, ym)
+        np.testing.assert_array_equal(y[out], y0[out])
+    elif kind == "rc":
+        x = datagen.real_data(seed, (_vlen(xv),), dtype=rtype)
+        z0 = datagen.complex_data(seed + 5000, (_vlen(yv),), dtype=ctype)
+        z = fftm.rcfft(amd, x.copy(), xv, z0.copy(), yv, scale=scale)
+        _check("vs oracle", z, po.rcfftm(x, xm, z0, ym, 0, scale), tol)
+        out = _outside(z0.size, ym)
+        np.testing.assert_array_equal(z[out], z0[out])
+    else:
+        z = datagen.complex_data(seed, (_vlen(xv),), dtype=ctype)            # not Hermitian
+        x0 = datagen.real_data(seed + 5000, (_vlen(yv),), dtype=rtype)
+        x = fftm.crfft(amd, z.copy(), xv, x0.copy(), yv, scale=scale)
+        _check("vs oracle", x, po.crfftm(z, xm, x0, ym, 0, scale), tol)
+        out = _outside(x0.size, ym)
+        np.testing.assert_array_equal(x[out], x0[out])
+
+
+def test_sweep_1d_covers_the_classes():
+    seen = set()
+    for s in range(NCASES_1D):
+        kind, prec, L, _, _, rng = _draw_1d(s)
+        seen.update((kind, prec))
+        if L & (L - 1) == 0 and L >= 2:
+            seen.add("pow2")
+        if _largest_prime(L) > 13:
+            seen.add("bluestein")
+        if L >= (1 << 17):
+            seen.add("split")
+        if L >= (1 << 21):
+            seen.add("split3")
+    for want in ("ip", "op", "rc", "cr", 4, 8, "pow2", "bluestein", "split", "split3"):
         assert want in seen, want
