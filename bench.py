@@ -483,6 +483,23 @@ def config_p2_rows(lib, hip):
     return out
 
 
+def col_pattern(lib, plan, major, m, n, parent, t):
+    """For a COL plan: the rate of its access pattern with no transform in
+    it (vsip_amd_col_probe: the plan's columns per workgroup, rows `parent`
+    apart on the load side and n apart on the store side) and the plan's
+    fraction of it; nothing for ROW plans."""
+    passes = json.loads(lib.describe(plan))["passes"]
+    if major != 1 or len(passes) != 1:
+        return {}
+    vpb = int(passes[0]["across"][0])
+    g = ctypes.c_double(0.0)
+    if lib.amd_col_probe(m, n, parent, n, vpb, 40, ctypes.byref(g)) or g.value <= 0:
+        return {}
+    lib.amd_col_probe(m, n, parent, n, vpb, 40, ctypes.byref(g))
+    return {"pattern_ceiling_gb_per_s": round(g.value, 1), "pattern_columns_per_workgroup": vpb,
+            "frac_of_pattern_ceiling": round(2 * m * n * 8 / t / 1e9 / g.value, 3)}
+
+
 def config_c3(lib, hip):
     """strided / mixed-radix ccfftmop_f (parent rows 2x the row length) and the FP64 2048^2 case"""
     from jvsip_amd import datagen, fftm
@@ -511,6 +528,7 @@ def config_c3(lib, hip):
         out[name] = {"us": round(t * 1e6, 2), "gb_per_s": round(2 * m * n * 8 / t / 1e9, 1),
                      "size_matched_ceiling_gb_per_s": ceil,
                      "frac_of_size_matched_ceiling": round(2 * m * n * 8 / t / 1e9 / ceil, 3) if ceil else None,
+                     **col_pattern(lib, p, major, m, n, parent, t),
                      "rel_l2_vs_jvsip": rel_l2(vecs[sel], f["out_rows"]),
                      "rel_l2_vs_fp64": rel_l2(vecs[sel], want),
                      "schedule": json.loads(lib.describe(p))["passes"][0]["radix"]}

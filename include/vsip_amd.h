@@ -48,6 +48,19 @@ void *vsip_amd_block_device_d(const vsip_block_d *b);
  * and *read_gbs the read-only rate.  Returns 0 or an error code. */
 int vsip_amd_hbm_probe(size_t bytes, int reps, double *rmw_gbs, double *read_gbs);
 
+/* The column-mapped transforms' access pattern with no transform in it:
+ * for a rows x cols complex FP32 view whose rows are in_row_stride elements
+ * apart, each workgroup reads every row of cols_per_group (1, 2, 4, 8 or 16;
+ * rows x cols_per_group <= 16384) adjacent columns into registers,
+ * synchronises and writes them to a view whose rows are out_row_stride
+ * apart; consecutive groups run on one XCD, as in the column kernels.  Four
+ * input/output pairs visited in turn, `reps` visits timed with stream
+ * events; *gbs gets 2 x 8 x rows x cols bytes per visit / seconds / 1e9: the
+ * ceiling of that pattern at that size.  Returns 0, -1 for arguments out of
+ * range, or an error code. */
+int vsip_amd_col_probe(long rows, long cols, long in_row_stride, long out_row_stride, int cols_per_group,
+                       int reps, double *gbs);
+
 /* Single-pass schedule the planner picks for `nvec` transforms of length n
  * (prec 4 or 8) on a device with `lds_max` bytes of LDS per workgroup, as
  * JSON text; no device work.  Returns 0, -1 for an unsupported length, -2 if

@@ -361,6 +361,8 @@ int vhip_memcpy_h2d_async(void *dst, const void *src, size_t bytes, void *stream
 /* HBM reference kernels over `bytes` of device memory: kind 0 in-place
  * read-modify-write, kind 1 read-only (sink: 4 writable bytes) */
 int vhip_probe_launch(int kind, void *buf, size_t bytes, void *sink, void *stream);
+int vhip_colprobe_launch(const void *in, void *out, int rows, int cols, long ld_in, long ld_out, int lg,
+                         void *stream);
 /* time `reps` runs of fn(arg) on stream with events; returns milliseconds per run */
 int vhip_event_time(void *stream, int (*fn)(void *arg), void *arg, int reps, float *ms);
 size_t vhip_fftm_lds_bytes(const vhip_fftm_desc *d);   /* dynamic LDS of one launch */

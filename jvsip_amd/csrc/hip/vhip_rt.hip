@@ -316,7 +316,58 @@ __global__ void probe_read(const float4 *p, long n, float *sink) {
     }
     if (acc == 1.2345e30f) sink[0] = acc;  // never true for the probe's data; keeps the loads
 }
+
+// Access pattern of the column-mapped transforms with no transform in it:
+// workgroup g takes 2^lg adjacent columns of a rows x cols complex FP32 view
+// whose rows are ld_in elements apart, reads every row of them into
+// registers (lane t: element t + 512 k of the group, adjacent lanes on
+// adjacent columns), synchronises, and writes them to the same place of a
+// view whose rows are ld_out apart.  Groups are numbered as the column
+// kernels number them: consecutive groups on one XCD, so that groups
+// sharing cache lines share that XCD's L2.
+__device__ __forceinline__ unsigned probe_xcd_group(unsigned b, unsigned g) {
+    const unsigned x = b & 7u, i = b >> 3, per = g >> 3, rem = g & 7u;
+    return x * per + (x < rem ? x : rem) + i;
+}
+__global__ void __launch_bounds__(512) probe_cols(const float2 *in, float2 *out, int rows, int cols, long ld_in,
+                                                   long ld_out, int lg) {
+    const int mask = (1 << lg) - 1;
+    const long c0 = (long)probe_xcd_group(blockIdx.x, gridDim.x) << lg;
+    const int total = rows << lg;
+    const int t = threadIdx.x;
+    float2 x[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        const int i = t + k * 512;
+        const long c = c0 + (i & mask);
+        if (i < total && c < cols) x[k] = in[(long)(i >> lg) * ld_in + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        const int i = t + k * 512;
+        const long c = c0 + (i & mask);
+        if (i < total && c < cols) {
+            x[k].x *= 1.0000001f;
+            out[(long)(i >> lg) * ld_out + c] = x[k];
+        }
+    }
+}
 }  // namespace
+
+// one launch of probe_cols; the caller's buffers hold ((rows - 1) * ld + cols)
+// complex FP32 elements on each side
+extern "C" int vhip_colprobe_launch(const void *in, void *out, int rows, int cols, long ld_in, long ld_out, int lg,
+                                    void *stream) {
+    if (rows < 1 || cols < 1 || lg < 0 || lg > 4 || ((long)rows << lg) > 32L * 512 || ld_in < cols ||
+        ld_out < cols)
+        return -1;
+    const long groups = ((long)cols + (1L << lg) - 1) >> lg;
+    hipStream_t s = stream ? (hipStream_t)stream : g_stream;
+    hipLaunchKernelGGL(probe_cols, dim3((unsigned)groups), dim3(512), 0, s, (const float2 *)in, (float2 *)out,
+                       rows, cols, ld_in, ld_out, lg);
+    return (int)hipGetLastError();
+}
 
 extern "C" int vhip_probe_launch(int kind, void *buf, size_t bytes, void *sink, void *stream) {
     const long n = (long)(bytes / 16);

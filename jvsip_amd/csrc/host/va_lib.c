@@ -122,17 +122,60 @@ static int va_describe(const VA_fftm *p, char *buf, size_t len)
 }
 
 typedef struct {
-    void *buf[4];
+    void *buf[4], *out[4];
     size_t bytes;
     int kind, i;
     void *sink;
+    long rows, cols, ld_in, ld_out;   /* kind 2: the column probe */
+    int lg;
 } VA_probe;
 
 static int va_probe_step(void *arg)
 {
     VA_probe *p = (VA_probe *)arg;
-    int rc = vhip_probe_launch(p->kind, p->buf[p->i & 3], p->bytes, p->sink, vhip_stream());
+    int rc;
+    if (p->kind == 2)
+        rc = vhip_colprobe_launch(p->buf[p->i & 3], p->out[p->i & 3], (int)p->rows, (int)p->cols, p->ld_in,
+                                  p->ld_out, p->lg, vhip_stream());
+    else rc = vhip_probe_launch(p->kind, p->buf[p->i & 3], p->bytes, p->sink, vhip_stream());
     p->i++;
+    return rc;
+}
+
+int vsip_amd_col_probe(long rows, long cols, long in_row_stride, long out_row_stride, int cols_per_group,
+                       int reps, double *gbs)
+{
+    VA_probe p;
+    float ms = 0.f;
+    int k, rc = 0, lg = 0;
+    size_t bin, bout;
+    *gbs = 0.0;
+    while (lg < 5 && (1 << lg) < cols_per_group) lg++;
+    if (rows < 1 || cols < 1 || rows > 0x7fffffffL || cols > 0x7fffffffL || lg > 4 || (1 << lg) != cols_per_group ||
+        (rows << lg) > 32L * 512 || in_row_stride < cols || out_row_stride < cols || reps < 1)
+        return -1;
+    if (va_ensure_init()) return va_init_rc;
+    bin = (size_t)((rows - 1) * in_row_stride + cols) * 8;
+    bout = (size_t)((rows - 1) * out_row_stride + cols) * 8;
+    memset(&p, 0, sizeof(p));
+    p.kind = 2;
+    p.rows = rows;
+    p.cols = cols;
+    p.ld_in = in_row_stride;
+    p.ld_out = out_row_stride;
+    p.lg = lg;
+    for (k = 0; k < 4 && !rc; k++) {
+        rc = vhip_malloc(&p.buf[k], bin);
+        if (!rc) rc = vhip_memset(p.buf[k], 0, bin);
+        if (!rc) rc = vhip_malloc(&p.out[k], bout);
+    }
+    if (!rc) rc = vhip_event_time(vhip_stream(), va_probe_step, &p, reps, &ms);
+    if (!rc && ms > 0.f) *gbs = 2.0 * 8.0 * (double)rows * (double)cols / (ms * 1e-3) / 1e9;
+    vhip_sync();
+    for (k = 0; k < 4; k++) {
+        if (p.buf[k]) vhip_free(p.buf[k]);
+        if (p.out[k]) vhip_free(p.out[k]);
+    }
     return rc;
 }
 

@@ -257,6 +257,8 @@ class AmdLib(VsipLib):
             "vsip_amd_batch_info_f": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
             "vsip_amd_hbm_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                                   ctypes.POINTER(ctypes.c_double)]),
+            "vsip_amd_col_probe": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long,
+                                                  ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
             "vsip_amd_tune": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_char_p, ctypes.c_size_t]),
             "vsip_amd_serve_f": (ctypes.c_int, [ctypes.POINTER(amd_request), ctypes.c_long, P, P, ctypes.c_size_t,
                                                 ctypes.c_int, ctypes.c_int, ctypes.POINTER(amd_result),

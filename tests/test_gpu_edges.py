@@ -91,3 +91,16 @@ def test_getattr_matches_reference(amd, ref, major):
         assert _attr(amd, pa) == _attr(ref, pr), name
         amd.fftm_destroy_f(pa)
         ref.fftm_destroy_f(pr)
+
+
+def test_column_probe(amd):
+    """vsip_amd_col_probe times the column transforms' access pattern and
+    refuses shapes it cannot hold."""
+    import ctypes
+    g = ctypes.c_double(0.0)
+    assert amd.amd_col_probe(1920, 1000, 2000, 1000, 4, 5, ctypes.byref(g)) == 0
+    assert 100.0 < g.value < 8000.0, g.value
+    print("  COL 1920 x 1000 pattern, 4 columns per workgroup: %.0f GB/s" % g.value, flush=True)
+    for args in ((1920, 1000, 2000, 1000, 3), (4096, 1000, 2000, 1000, 8), (16, 100, 50, 100, 4),
+                 (16, 100, 100, 99, 4), (0, 100, 100, 100, 4)):
+        assert amd.amd_col_probe(*args, 5, ctypes.byref(g)) == -1, args
