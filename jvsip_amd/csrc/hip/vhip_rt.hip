@@ -331,25 +331,40 @@ __device__ __forceinline__ unsigned probe_xcd_group(unsigned b, unsigned g) {
 }
 __global__ void __launch_bounds__(512) probe_cols(const float2 *in, float2 *out, int rows, int cols, long ld_in,
                                                    long ld_out, int lg) {
-    const int mask = (1 << lg) - 1;
-    const long c0 = (long)probe_xcd_group(blockIdx.x, gridDim.x) << lg;
-    const int total = rows << lg;
+    // buffer ranges end at each view's last element: rows past the end read
+    // 0 and their stores are dropped, with no per-element test
+    const unsigned in_bytes = (unsigned)(((long)(rows - 1) * ld_in + cols) * 8);
+    const unsigned out_bytes = (unsigned)(((long)(rows - 1) * ld_out + cols) * 8);
+    const __amdgpu_buffer_rsrc_t ri = __builtin_amdgcn_make_buffer_rsrc((void *)in, (short)0, in_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t ro = __builtin_amdgcn_make_buffer_rsrc(out, (short)0, out_bytes, 0x00020000);
     const int t = threadIdx.x;
+    const int rs = 512 >> lg;                                   // rows per pass of the workgroup
+    const long c = ((long)probe_xcd_group(blockIdx.x, gridDim.x) << lg) + (t & ((1 << lg) - 1));
+    const int r0 = t >> lg;
+    // a lane past the last column stays beyond both ranges on every pass
+    const bool live = c < cols;
+    const unsigned oi = live ? (unsigned)(((long)r0 * ld_in + c) * 8) : 0xfffffff0u;
+    const unsigned oo = live ? (unsigned)(((long)r0 * ld_out + c) * 8) : 0xfffffff0u;
+    const unsigned si = live ? (unsigned)(rs * ld_in * 8) : 0u, so = live ? (unsigned)(rs * ld_out * 8) : 0u;
+    const int passes = (rows + rs - 1) / rs;                    // <= 32, uniform
     float2 x[32];
 #pragma unroll
-    for (int k = 0; k < 32; ++k) {
-        const int i = t + k * 512;
-        const long c = c0 + (i & mask);
-        if (i < total && c < cols) x[k] = in[(long)(i >> lg) * ld_in + c];
+    for (int j = 0; j < 4; ++j) {
+        if (j * 8 < passes) {
+#pragma unroll
+            for (int k = j * 8; k < j * 8 + 8; ++k)
+                x[k] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(ri, oi + k * si, 0, 0));
+        }
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 32; ++k) {
-        const int i = t + k * 512;
-        const long c = c0 + (i & mask);
-        if (i < total && c < cols) {
-            x[k].x *= 1.0000001f;
-            out[(long)(i >> lg) * ld_out + c] = x[k];
+    for (int j = 0; j < 4; ++j) {
+        if (j * 8 < passes) {
+#pragma unroll
+            for (int k = j * 8; k < j * 8 + 8; ++k) {
+                x[k].x *= 1.0000001f;
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(probe_raw64_t, x[k]), ro, oo + k * so, 0, 0);
+            }
         }
     }
 }
@@ -362,6 +377,10 @@ extern "C" int vhip_colprobe_launch(const void *in, void *out, int rows, int col
     if (rows < 1 || cols < 1 || lg < 0 || lg > 4 || ((long)rows << lg) > 32L * 512 || ld_in < cols ||
         ld_out < cols)
         return -1;
+    // 32-bit buffer offsets: a chunk of 8 passes may reach 8 passes past
+    // the last row; those offsets stay below 2^31 bytes
+    const long over = (long)rows + 8L * (512 >> lg);
+    if (over * ld_in * 8 >= (1L << 31) || over * ld_out * 8 >= (1L << 31)) return -1;
     const long groups = ((long)cols + (1L << lg) - 1) >> lg;
     hipStream_t s = stream ? (hipStream_t)stream : g_stream;
     hipLaunchKernelGGL(probe_cols, dim3((unsigned)groups), dim3(512), 0, s, (const float2 *)in, (float2 *)out,
