@@ -56,18 +56,18 @@ struct FastKey {
 std::map<FastKey, hipFunction_t> g_fast;
 
 // every field in its own bits: n (31), vpb (16), threads (11), prec (4) in a;
-// min, mout, nst (4 each) and radix - 1 of each stage (4 each, radices 1..16)
-// in b; a schedule the fields cannot hold is not keyed (it runs on the
-// generic kernel)
+// min (5), mout (4), nst (4) and radix - 1 of each stage (4 each, radices
+// 1..16) in b; a schedule the fields cannot hold is not keyed (it runs on
+// the generic kernel)
 bool fast_key(int prec, int n, int nst, const int *radix, int vpb, int threads, int min, int mout, FastKey &k) {
     if (n < 1 || vpb < 1 || vpb > 0xffff || threads < 1 || threads > 2047 || nst < 1 || nst > 8) return false;
     k.a = (unsigned long long)(unsigned)n | ((unsigned long long)vpb << 31) | ((unsigned long long)threads << 47) |
           ((unsigned long long)(prec & 0xf) << 58);
-    k.b = (unsigned long long)(min & 0xf) | ((unsigned long long)(mout & 0xf) << 4) |
-          ((unsigned long long)nst << 8);
+    if (min < 0 || min > 0x1f || mout < 0 || mout > 0xf) return false;
+    k.b = (unsigned long long)min | ((unsigned long long)mout << 5) | ((unsigned long long)nst << 9);
     for (int s = 0; s < nst; ++s) {
         if (radix[s] < 1 || radix[s] > 16) return false;
-        k.b |= (unsigned long long)(radix[s] - 1) << (12 + 4 * s);
+        k.b |= (unsigned long long)(radix[s] - 1) << (13 + 4 * s);
     }
     return true;
 }
@@ -80,9 +80,14 @@ bool jit_on() {
 // the template instance of a schedule; empty when the schedule has no
 // compiled form (more than 8 stages, or not a single pass)
 std::string instance(int prec, int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
-    if (nst < 1 || nst > 8 || min < 0 || min > 7 || mout < 0 || mout > 15 || (prec != 4 && prec != 8)) return "";
+    // load-side bits: ACROSS, FUSED, REAL, C2R; store-side: ACROSS, FUSED, PTW
+    const int min_bits = VHIP_JIT_ACROSS | VHIP_JIT_FUSED | VHIP_JIT_REAL | VHIP_JIT_C2R;
+    if (nst < 1 || nst > 8 || min < 0 || (min & ~min_bits) || mout < 0 || mout > 15 || (prec != 4 && prec != 8))
+        return "";
     if ((mout & VHIP_JIT_PTW) && (mout & VHIP_JIT_FUSED)) return "";  // no power accumulation in a split pass
     if ((min & VHIP_JIT_REAL) && (mout & VHIP_JIT_FUSED)) return "";   // no power accumulation after a real input
+    // complex to real: nothing fused, no post-twiddle, not also real input
+    if ((min & VHIP_JIT_C2R) && ((min & (VHIP_JIT_FUSED | VHIP_JIT_REAL)) || mout > 1)) return "";
     int r[8];
     for (int s = 0; s < 8; ++s) r[s] = s < nst ? radix[s] : 1;
     char buf[256];
@@ -287,8 +292,10 @@ extern "C" int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vp
 // layouts of a descriptor as template arguments (bit 1: fused taper / power
 // accumulation); false when the compiled kernels cannot run it
 static bool desc_layouts(const vhip_fftm_desc *d, int &min, int &mout) {
-    if ((d->prec != 4 && d->prec != 8) || (d->kind != VHIP_C2C && d->kind != VHIP_R2C)) return false;
+    if ((d->prec != 4 && d->prec != 8) || (d->kind != VHIP_C2C && d->kind != VHIP_R2C && d->kind != VHIP_C2R))
+        return false;
     if (d->kind == VHIP_R2C && (d->acc || d->ptw)) return false;
+    if (d->kind == VHIP_C2R && (d->acc || d->ptw || d->tap)) return false;
     min = side_mode(d->in, d->map_in, d->nvec);
     mout = side_mode(d->out, d->map_out, d->nvec);
     if (min < 0 || mout < 0) return false;
@@ -296,6 +303,7 @@ static bool desc_layouts(const vhip_fftm_desc *d, int &min, int &mout) {
     if (d->acc) mout |= VHIP_JIT_FUSED;
     if (d->ptw) mout |= VHIP_JIT_PTW;
     if (d->kind == VHIP_R2C) min |= VHIP_JIT_REAL;
+    if (d->kind == VHIP_C2R) min |= VHIP_JIT_C2R;
     return true;
 }
 
@@ -312,18 +320,19 @@ extern "C" int vhip_jit_prepare_desc(const vhip_fftm_desc *d) {
 extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
     int min, mout;
     if (!d->compiled || !desc_layouts(d, min, mout)) return -100;
-    const bool real = (min & VHIP_JIT_REAL) != 0;
+    const bool real = (min & VHIP_JIT_REAL) != 0, c2r = (min & VHIP_JIT_C2R) != 0;
     const long es = 2L * d->prec;
     // contiguous sides: rows must not overlap and a group's rows span < 2 GiB
-    const long in_len = real ? 2L * d->n : d->n, in_esz = real ? d->prec : es;
-    const long out_len = real ? d->n + 1L : d->n;
+    // (real sides in real elements)
+    const long in_len = real ? 2L * d->n : c2r ? d->n + 1L : d->n, in_esz = real ? d->prec : es;
+    const long out_len = real ? d->n + 1L : c2r ? 2L * d->n : d->n, out_esz = c2r ? d->prec : es;
     if ((min & 1) == 0) {
         if (d->nvec > 1 && d->in.vs0 < in_len) return -100;
         if (((long)(d->vpb - 1) * d->in.vs0 + in_len) * in_esz >= (1L << 31)) return -100;
     }
     if ((mout & 1) == 0) {
         if (d->nvec > 1 && d->out.vs0 < out_len) return -100;
-        if (((long)(d->vpb - 1) * d->out.vs0 + out_len) * es >= (1L << 31)) return -100;
+        if (((long)(d->vpb - 1) * d->out.vs0 + out_len) * out_esz >= (1L << 31)) return -100;
     }
     hipFunction_t fn = nullptr;
     FastKey key;

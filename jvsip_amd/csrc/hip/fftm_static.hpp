@@ -27,7 +27,11 @@
 // Bit 2 of MIN makes the transform real to complex (rcfftmop): the load
 // reads real samples 2m, 2m + 1 as complex element m, the last stage leaves
 // Y in LDS, and the n + 1 outputs are combined from Y[k] and Y[n - k] with
-// the Euler table (r2c_post of the generic kernel).
+// the Euler table (r2c_post of the generic kernel).  Bit 4 of MIN makes it
+// complex to real (crfftmop): the n + 1 inputs of each vector are combined
+// into conj(Y) in LDS (c2r_pre of the generic kernel), stage 0 reads LDS,
+// and the last stage conjugates back and stores complex element m as real
+// samples 2m, 2m + 1 (offsets of the output side in real elements).
 #pragma once
 #include "fftm_core.hpp"
 
@@ -36,6 +40,7 @@
 #define ST_FUSED 2
 #define ST_REAL 4
 #define ST_PTW 8    // store side: the split passes' post-twiddle (vhip_fftm_desc.ptw)
+#define ST_C2R 16   // load side: complex to real (crfftmop)
 
 namespace vhipk {
 namespace st {
@@ -81,11 +86,14 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
     constexpr int LIN = MIN & 1, LOUT = MOUT & 1;
     constexpr bool TAP = (MIN & ST_FUSED) != 0, ACC = (MOUT & ST_FUSED) != 0;
     constexpr bool REAL = (MIN & ST_REAL) != 0;
+    constexpr bool C2R = (MIN & ST_C2R) != 0;
     constexpr bool PTW = (MOUT & ST_PTW) != 0;
     constexpr int RS = (int)sizeof(T);
+    // stage 0 of a complex-to-real transform reads the combined input in LDS
+    constexpr bool GLOAD = FIRST && !C2R;
     // lane mapping of this stage: the global layout's for the first and last
     // stage (the load's when one stage does both), along the vector otherwise
-    constexpr int MODE = FIRST ? LIN : (LAST ? LOUT : ST_CONTIG);
+    constexpr int MODE = GLOAD ? LIN : (LAST ? LOUT : ST_CONTIG);
     constexpr int LS = lstride<T>(N);
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
@@ -115,7 +123,7 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
     for (int i = 0; i < B; ++i) {
         const int v = vv[i], j = jj[i];
         if (ok[i]) {
-            if (FIRST) {
+            if (GLOAD) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     cpx<T> y;
@@ -156,7 +164,7 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
             }
         }
     }
-    if (!FIRST) __syncthreads();
+    if (!GLOAD) __syncthreads();
     // every butterfly of the lane first, then every write: a store's data
     // registers are then never reused while stores are in flight (the
     // compiler would wait for the stores before reusing them)
@@ -179,6 +187,19 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
                 cpx<T> *dst = lds + v * LS;
 #pragma unroll
                 for (int r = 0; r < R; ++r) dst[lpad<T>(j + r * NBV)] = x[i][r];
+            } else if (LAST && C2R) {   // complex element m as real samples 2m, 2m + 1
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const cpx<T> z = {x[i][r].x * io.scale, -x[i][r].y * io.scale};
+                    const int m2 = 2 * (j + r * NBV);
+                    if (LOUT == ST_CONTIG) {
+                        Buf<T>::template stc<kStData>(io.out, (int)(v * io.ovs + m2) * RS, 0, z);
+                    } else {
+                        const int o = (io.voff_out[v] + m2 * io.oes) * RS;
+                        Buf<T>::template sts<kStData>(io.out, o, 0, z.x);
+                        Buf<T>::template sts<kStData>(io.out, o + io.oes * RS, 0, z.y);
+                    }
+                }
             } else if (LAST) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
@@ -258,14 +279,17 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
     }
     io.vbase = vbase;
     constexpr bool REAL = (MIN & ST_REAL) != 0;
+    constexpr bool C2R = (MIN & ST_C2R) != 0;
     constexpr int RS = (int)sizeof(T);
     if ((MIN & 1) == ST_CONTIG && REAL)
         io.in = make_rsrc((const T *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + 2 * N) * RS);
     else if ((MIN & 1) == ST_CONTIG)
-        io.in = make_rsrc((const cpx<T> *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + N) * ES);
+        io.in = make_rsrc((const cpx<T> *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + N + (C2R ? 1 : 0)) * ES);
     else
         io.in = make_rsrc(d.in.re, d.in.span);
-    if ((MOUT & 1) == ST_CONTIG)
+    if ((MOUT & 1) == ST_CONTIG && C2R)
+        io.out = make_rsrc((T *)d.out.re + vbase * io.ovs, ((io.nv - 1) * io.ovs + 2 * N) * RS);
+    else if ((MOUT & 1) == ST_CONTIG)
         io.out = make_rsrc((cpx<T> *)d.out.re + vbase * io.ovs, ((io.nv - 1) * io.ovs + N + (REAL ? 1 : 0)) * ES);
     else
         io.out = make_rsrc(d.out.re, d.out.span);
@@ -273,6 +297,30 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
     io.isgn = d.conj_in ? (T)-1 : (T)1;
     io.osgn = d.conj_out ? (T)-1 : (T)1;
     io.scale = (T)d.scale;
+    if constexpr (C2R) {
+        // Y_k = (Z_k + conj Z_{n-k}) + i W^k (Z_k - conj Z_{n-k}), W the Euler
+        // table e^{+i pi k / n}, k = 0..n-1; conj(Y) goes to LDS, so the
+        // forward stages compute the inverse
+        const cpx<T> *eu = (const cpx<T> *)d.euler;
+        for (int t = threadIdx.x; t < io.nv * N; t += NT) {
+            int v, k;
+            if ((MIN & 1) == ST_ACROSS) { v = t % io.nv; k = t / io.nv; }
+            else { v = t / N; k = t - v * N; }
+            cpx<T> F, B;
+            if ((MIN & 1) == ST_CONTIG) {
+                F = Buf<T>::template ldc<kLdData>(io.in, (int)(v * io.ivs + k) * ES, 0);
+                B = Buf<T>::template ldc<kLdData>(io.in, (int)(v * io.ivs + N - k) * ES, 0);
+            } else {
+                F = Buf<T>::template ldc<kLdData>(io.in, (io.voff_in[v] + k * io.ies) * ES, 0);
+                B = Buf<T>::template ldc<kLdData>(io.in, (io.voff_in[v] + (N - k) * io.ies) * ES, 0);
+            }
+            const cpx<T> Bc = {B.x, -B.y};
+            const cpx<T> A = F + Bc;
+            const cpx<T> WD = cmul(eu[k], F - Bc);
+            slds[v * LS + lpad<T>(k)] = {A.x - WD.y, -(A.y + WD.x)};
+        }
+        __syncthreads();
+    }
     sstages<T, N, VPB, NT, MIN, MOUT, 0, SC>(slds, d, io);
     if constexpr (REAL) {
         // the n + 1 outputs of each vector: Z_k = (A + i^-1 W_k D) / 2 with

@@ -250,6 +250,7 @@ int vhip_rng_fill(const vhip_rng_desc *d, void *stream);
 #define VHIP_JIT_FUSED 2
 #define VHIP_JIT_REAL 4     /* load side: real input, real-to-complex transform */
 #define VHIP_JIT_PTW 8      /* store side: post-twiddle (the split's passes before the last) */
+#define VHIP_JIT_C2R 16     /* load side: complex-to-real transform (n + 1 inputs, real output) */
 int vhip_jit_prepare(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in,
                      int layout_out);
 int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);

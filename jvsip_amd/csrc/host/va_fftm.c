@@ -223,7 +223,7 @@ static int va_jit_worth(const VA_fftm *p)
     return (double)p->ntimes * (double)elems >= (double)(1L << 30);
 }
 
-/* Complex and real-to-complex plans (FP32, FP64) whose passes the
+/* Complex, real-to-complex and complex-to-real plans (FP32, FP64) whose passes the
  * schedule-generic kernel would run (every pass except the FP32
  * power-of-two kernels' contiguous rows; complex rows of 256..2048 in
  * batches too small for the pipelined kernel compile as well) and whose use
@@ -235,21 +235,22 @@ static int va_jit_worth(const VA_fftm *p)
 static void va_jit_prepare(VA_fftm *p)
 {
     long n = (long)p->MN;
-    int k, real = (p->type == VA_RCFFTOP) ? VHIP_JIT_REAL : 0;
-    if ((p->type != VA_CCFFTIP && p->type != VA_CCFFTOP && p->type != VA_RCFFTOP) || p->staged) return;
+    int k, real = (p->type == VA_RCFFTOP) ? VHIP_JIT_REAL : (p->type == VA_CRFFTOP) ? VHIP_JIT_C2R : 0;
+    if (p->staged) return;
     if (!va_jit_worth(p)) return;
     if (getenv("VSIP_AMD_JIT") && getenv("VSIP_AMD_JIT")[0] == '0') return;
     p->jit_worth = 1;
     if (p->npass == 1) {
         const VA_sched *s = &p->pass[0];
         if (p->major == VSIP_ROW) {
-            /* the power-of-two kernels, except complex rows of 256..2048 in
-             * batches below the pipelined kernel's: there the compiled
-             * schedule beats the one-group kernel (MI355X, per launch:
-             * 64 x 1024 5.4 -> 3.5 us, 4096 x 1024 14.2 -> 12.0, 128 x 512
-             * 5.2 -> 3.0, 2048 x 2048 13.8 -> 11.9; from 4096 points even) */
+            /* the power-of-two kernels, except rows of 256..2048 complex
+             * points (c2c, or the half length of r2c and c2r) in batches
+             * below the pipelined kernels': there the compiled schedule beats
+             * the one-group kernel (MI355X, per launch: 64 x 1024 5.4 -> 3.5 us,
+             * 4096 x 1024 14.2 -> 12.0, 128 x 512 5.2 -> 3.0, 2048 x 2048
+             * 13.8 -> 11.9; from 4096 points even) */
             if (p->prec == 4 && (n & (n - 1)) == 0 && n >= 256 && n <= 16384 && !getenv("VSIP_AMD_NO_POW2") &&
-                (real || n > 2048 || vhip_p2_pipelined((int)n, (long)p->mN)))
+                (n > 2048 || vhip_p2_pipelined((int)n, (long)p->mN)))
                 return;                                   /* p->jit stays 0: never a compiled kernel */
             p->jit = 1;
             if (va_jit_lds_ok(s, &s->along, p->prec))
@@ -279,7 +280,7 @@ static void va_jit_prepare(VA_fftm *p)
 int va_jit_ready(const VA_fftm *p)
 {
     const VA_sched *s = &p->pass[0];
-    int k, real = (p->type == VA_RCFFTOP) ? VHIP_JIT_REAL : 0;
+    int k, real = (p->type == VA_RCFFTOP) ? VHIP_JIT_REAL : (p->type == VA_CRFFTOP) ? VHIP_JIT_C2R : 0;
     if (p->npass == 1) {
         if (p->major == VSIP_ROW)
             return vhip_jit_ready(p->prec, s->n, s->nst, s->radix, s->along.vpb, s->along.threads,

@@ -320,3 +320,72 @@ def test_large_power_of_two_rows_stay_on_their_kernels(amd, monkeypatch):
         d = json.loads(amd.describe(p))
         amd.fftm_destroy_f(p)
         assert d.get("compiled_schedule") is False, (m, n)
+
+
+@pytest.mark.parametrize("m,n", [(64, 1024), (16, 4096), (100, 512), (1000, 2048)])
+def test_small_power_of_two_real_rows_both_kernels(amd, monkeypatch, m, n):
+    """rcfftmop rows of 512..4096 reals in batches below the pipelined R2C
+    kernel's compile their schedule; with VSIP_AMD_JIT=0 they run on the
+    one-group real kernel.  Both against the oracle (1e-5) and each other
+    (1e-6)."""
+    monkeypatch.delenv("VSIP_AMD_JIT_MIN_ELEMS", raising=False)
+    monkeypatch.delenv("VSIP_AMD_JIT", raising=False)
+    x = datagen.real_data(99, (m, n))
+    h = n // 2 + 1
+    xv, zv = fftm.dense_view(m, n), fftm.dense_view(m, h)
+    z0 = np.zeros(m * h, np.complex64)
+    p = amd.rcfftmop_create_f(m, n, 1.0, VSIP_ROW, 0, 0)
+    assert p, amd.last_error()
+    d = json.loads(amd.describe(p))
+    amd.fftm_destroy_f(p)
+    assert d.get("compiled_schedule") is True
+    _case("rcfftmop_f ROW %dx%d (compiled schedule)" % (m, n))
+    z = fftm.rcfftm(amd, x.ravel(), xv, z0, zv, scale=0.5)
+    _check("vs oracle", z, po.rcfftm(x.ravel(), xv, z0, zv, 0, 0.5), 1e-5)
+    monkeypatch.setenv("VSIP_AMD_JIT", "0")
+    _case("rcfftmop_f ROW %dx%d (one-group real kernel)" % (m, n))
+    g = fftm.rcfftm(amd, x.ravel(), xv, z0, zv, scale=0.5)
+    _check("vs oracle", g, po.rcfftm(x.ravel(), xv, z0, zv, 0, 0.5), 1e-5)
+    _check("compiled vs real kernel", z, g, 1e-6)
+
+
+@pytest.mark.parametrize("m,n,major,prec,rstride", [(300, 2000, VSIP_ROW, 4, 0), (2000, 300, VSIP_COL, 4, 0),
+                                                    (64, 2 * 2187, VSIP_ROW, 4, 0), (64, 1024, VSIP_ROW, 4, 0),
+                                                    (512, 4374, VSIP_ROW, 4, 0), (1000, 64, VSIP_COL, 4, 0),
+                                                    (30, 1000, VSIP_ROW, 4, 1), (40, 2000, VSIP_ROW, 8, 0),
+                                                    (600, 48, VSIP_COL, 8, 0)])
+def test_compiled_schedule_complex_to_real(amd, monkeypatch, m, n, major, prec, rstride):
+    """crfftmop plans compile the complex-to-real schedule (the n + 1 inputs
+    combined into LDS, real samples stored in pairs or one by one); on
+    non-Hermitian input against the oracle (1e-5 / 1e-12) and the generic
+    kernel (1e-6 / 1e-13).  rstride 1 pads the real rows to an odd stride,
+    which the compiled kernel does not take (the generic kernel runs it)."""
+    monkeypatch.delenv("VSIP_AMD_JIT_MIN_ELEMS", raising=False)
+    monkeypatch.delenv("VSIP_AMD_JIT", raising=False)
+    ctype, rtype = (np.complex64, np.float32) if prec == 4 else (np.complex128, np.float64)
+    tol, tol_g = (1e-5, 1e-6) if prec == 4 else (1e-12, 1e-13)
+    L = n if major == VSIP_ROW else m
+    if major == VSIP_ROW:
+        zv = fftm.View(0, n // 2 + 1, m, 1, n // 2 + 1)
+        zlen = m * (n // 2 + 1)
+    else:
+        zv = fftm.View(0, n, m // 2 + 1, 1, n)
+        zlen = (m // 2 + 1) * n
+    xv = fftm.View(0, n + rstride, m, 1, n)
+    z = datagen.complex_data(103, (zlen,), dtype=ctype)
+    x0 = np.zeros(fftm.block_len(xv), rtype)
+    create = amd.crfftmop_create_f if prec == 4 else amd.crfftmop_create_d
+    p = create(m, n, 1.0 / L, major, 0, 0)
+    assert p, amd.last_error()
+    d = json.loads(amd.describe(p, "f" if prec == 4 else "d"))
+    (amd.fftm_destroy_f if prec == 4 else amd.fftm_destroy_d)(p)
+    assert d.get("compiled_schedule") is True
+    _case("crfftmop_%s %s %dx%d%s (compiled schedule)" % ("f" if prec == 4 else "d",
+                                                          "ROW" if major == VSIP_ROW else "COL", m, n,
+                                                          ", odd real row stride" if rstride else ""))
+    x = fftm.crfftm(amd, z, zv, x0, xv, major=major, scale=1.0 / L)
+    _check("vs oracle", x, po.crfftm(z, zv, x0, xv, major, 1.0 / L), tol)
+    monkeypatch.setenv("VSIP_AMD_JIT", "0")
+    g = fftm.crfftm(amd, z, zv, x0, xv, major=major, scale=1.0 / L)
+    monkeypatch.delenv("VSIP_AMD_JIT")
+    _check("vs generic kernel", x, g, tol_g)

@@ -40,7 +40,7 @@ def test_planner_schedules_compile(lib, so, prec, n, nvec):
     rad = (ctypes.c_int * len(s["radix"]))(*s["radix"])
     for shape, lin, lout in (("along", 0, 0), ("across", 1, 1), ("across", 0, 1), ("along", 2, 2),
                              ("across", 3, 3), ("along", 4, 0), ("across", 5, 1), ("across", 6, 1),
-                             ("across", 1, 9)):
+                             ("across", 1, 9), ("along", 16, 0), ("across", 17, 1)):
         vpb, thr, _ = s[shape]
         size = ctypes.c_size_t()
         rc = so.vhip_jit_compile_check(prec, n, len(s["radix"]), rad, vpb, thr, lin, lout, b"gfx950",
@@ -56,3 +56,6 @@ def test_refuses_schedules_without_a_compiled_form(so):
     assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 8, 0, b"gfx950", None) != 0
     assert so.vhip_jit_compile_check(2, 16, 2, rad, 1, 64, 0, 0, b"gfx950", None) != 0
     assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 4, 2, b"gfx950", None) != 0   # real input + accumulate
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 18, 0, b"gfx950", None) != 0  # complex to real + taper
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 16, 9, b"gfx950", None) != 0  # complex to real + post-twiddle
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 20, 0, b"gfx950", None) != 0  # complex to real + real input
