@@ -54,6 +54,9 @@ struct FastKey {
     bool operator<(const FastKey &o) const { return a != o.a ? a < o.a : b < o.b; }
 };
 std::map<FastKey, hipFunction_t> g_fast;
+// launch-time compiles already attempted (each key once; a failure stays on
+// the generic kernel without another attempt)
+std::map<FastKey, bool> g_tried;
 
 // every field in its own bits: n (31), vpb (16), threads (11), prec (4) in a;
 // min (5), mout (4), nst (4) and radix - 1 of each stage (4 each, radices
@@ -317,6 +320,9 @@ extern "C" int vhip_jit_prepare_desc(const vhip_fftm_desc *d) {
 
 // Launch d on its compiled kernel when one is loaded and d has the layout
 // the kernel serves; -100 otherwise (the caller uses the generic kernel).
+// A plan compiles the layouts its major implies at creation; a view with
+// another one (a COL plan on contiguous columns, a ROW plan across
+// column-major storage) has its kernel compiled at its first launch, once.
 extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
     int min, mout;
     if (!d->compiled || !desc_layouts(d, min, mout)) return -100;
@@ -337,7 +343,16 @@ extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
     hipFunction_t fn = nullptr;
     FastKey key;
     if (!fast_key(d->prec, d->n, d->nst, d->radix, d->vpb, d->threads, min, mout, key)) return -100;
+    bool compile_now = false;
     {
+        std::lock_guard<std::mutex> lock(g_mu);
+        auto it = g_fast.find(key);
+        if (it != g_fast.end()) fn = it->second;
+        else if (g_tried.find(key) != g_tried.end()) return -100;
+        else compile_now = g_tried[key] = true;
+    }
+    if (compile_now) {
+        if (vhip_jit_prepare(d->prec, d->n, d->nst, d->radix, d->vpb, d->threads, min, mout) != 0) return -100;
         std::lock_guard<std::mutex> lock(g_mu);
         auto it = g_fast.find(key);
         if (it == g_fast.end()) return -100;

@@ -389,3 +389,26 @@ def test_compiled_schedule_complex_to_real(amd, monkeypatch, m, n, major, prec, 
     g = fftm.crfftm(amd, z, zv, x0, xv, major=major, scale=1.0 / L)
     monkeypatch.delenv("VSIP_AMD_JIT")
     _check("vs generic kernel", x, g, tol_g)
+
+
+@pytest.mark.parametrize("major,m,n", [(VSIP_ROW, 300, 1000), (VSIP_COL, 300, 1000), (VSIP_COL, 1920, 64)])
+def test_other_layout_compiles_at_first_launch(amd, monkeypatch, major, m, n):
+    """Column-major storage gives a COL plan contiguous columns and a ROW
+    plan rows across the storage: layouts the plan does not compile at
+    creation.  The first launch compiles them; every call matches the oracle
+    (1e-5), the calls match each other bit for bit and the generic kernel
+    to 1e-6."""
+    monkeypatch.delenv("VSIP_AMD_JIT_MIN_ELEMS", raising=False)
+    monkeypatch.delenv("VSIP_AMD_JIT", raising=False)
+    v = fftm.View(0, 1, m, m, n)
+    x = datagen.complex_data(104, (m * n,))
+    _case("ccfftmip_f %s %dx%d on column-major storage" % ("ROW" if major == VSIP_ROW else "COL", m, n))
+    want = po.ccfftm(x, v, major=major)
+    y1 = fftm.ccfftm(amd, x, v, major=major)
+    _check("first call vs oracle", y1, want, 1e-5)
+    y2 = fftm.ccfftm(amd, x, v, major=major)
+    np.testing.assert_array_equal(y2, y1)
+    monkeypatch.setenv("VSIP_AMD_JIT", "0")
+    g = fftm.ccfftm(amd, x, v, major=major)
+    monkeypatch.delenv("VSIP_AMD_JIT")
+    _check("vs generic kernel", y1, g, 1e-6)
