@@ -352,6 +352,10 @@ extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
         else compile_now = g_tried[key] = true;
     }
     if (compile_now) {
+        // the kernel's static LDS (the group's padded vectors and offsets) must fit
+        int lds_max = 65536;
+        vhip_lds_max(&lds_max);
+        if ((long)d->vpb * d->lds_stride * 2 * d->prec + (long)d->vpb * 12 > lds_max) return -100;
         if (vhip_jit_prepare(d->prec, d->n, d->nst, d->radix, d->vpb, d->threads, min, mout) != 0) return -100;
         std::lock_guard<std::mutex> lock(g_mu);
         auto it = g_fast.find(key);
