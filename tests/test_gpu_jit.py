@@ -412,3 +412,47 @@ def test_other_layout_compiles_at_first_launch(amd, monkeypatch, major, m, n):
     g = fftm.ccfftm(amd, x, v, major=major)
     monkeypatch.delenv("VSIP_AMD_JIT")
     _check("vs generic kernel", y1, g, 1e-6)
+
+
+@pytest.mark.parametrize("major", [VSIP_ROW, VSIP_COL])
+def test_one_plan_on_both_layouts(amd, monkeypatch, major):
+    """One plan run in turn on a row-major view (the layout it compiled at
+    creation) and a column-major view (compiled at its first launch): every
+    call matches the oracle (1e-5) and the calls on each view repeat bit for
+    bit."""
+    monkeypatch.delenv("VSIP_AMD_JIT_MIN_ELEMS", raising=False)
+    monkeypatch.delenv("VSIP_AMD_JIT", raising=False)
+    m, n = 240, 1000
+    xr = datagen.complex_data(105, (m * n,))
+    xc = datagen.complex_data(106, (m * n,))
+    vr, vc = fftm.View(0, n, m, 1, n), fftm.View(0, 1, m, m, n)
+    want_r = po.ccfftm(xr, vr, major=major)
+    want_c = po.ccfftm(xc, vc, major=major)
+    p = amd.ccfftmip_create_f(m, n, 1.0, VSIP_FFT_FWD, major, 0, 0)
+    assert p, amd.last_error()
+    br = amd.cblockcreate_f(m * n, 0)
+    bc = amd.cblockcreate_f(m * n, 0)
+    hr = amd.cmbind_f(br, *vr)
+    hc = amd.cmbind_f(bc, *vc)
+    first = {}
+    for it in range(3):
+        for name, h, x, v, want in (("row-major", hr, xr, vr, want_r), ("column-major", hc, xc, vc, want_c)):
+            _case("one %s plan, call %d on the %s view" % ("ROW" if major == VSIP_ROW else "COL", it, name))
+            xx = np.ascontiguousarray(x)
+            blk = br if h is hr else bc
+            whole = amd.cmbind_f(blk, 0, 1, 1, 1, m * n)
+            amd.cmcopyfrom_user_f(xx.ctypes.data, None, VSIP_ROW, whole)
+            amd.ccfftmip_f(p, h)
+            got = np.empty(m * n, np.complex64)
+            amd.cmcopyto_user_f(whole, VSIP_ROW, got.ctypes.data, None)
+            amd.cmdestroy_f(whole)
+            _check("vs oracle", got, want, 1e-5)
+            if name in first:
+                np.testing.assert_array_equal(got, first[name])
+            first[name] = got
+    assert not amd.last_error(), amd.last_error()
+    amd.fftm_destroy_f(p)
+    amd.cmdestroy_f(hr)
+    amd.cmdestroy_f(hc)
+    amd.cblockdestroy_f(br)
+    amd.cblockdestroy_f(bc)
