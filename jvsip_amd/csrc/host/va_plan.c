@@ -304,8 +304,23 @@ int va_sched_shape(VA_sched *s, int prec, int n, vsip_length nvec, int lds_max)
     }
     vpb_across = va_env_cap("VSIP_AMD_VPB_ACROSS", vpb_across);
     if ((vsip_length)vpb_across > nvec && nvec > 0) vpb_across = (int)nvec;
-    if (va_pick_launch(s, prec, vpb_across, lds_max, lds_max, va_env_floor("VSIP_AMD_THREADS_ACROSS"), &s->across))
-        return -2;
+    {
+        /* the rows' rule for launches of at most two workgroups per CU, for
+         * FP32 columns too: one butterfly per lane in every stage */
+        int thr_min = va_env_floor("VSIP_AMD_THREADS_ACROSS");
+        if (prec == 4 && nvec > 0 && !thr_min) {
+            int cus = 256, k;
+            long most = 0;
+            vhip_cu_count(&cus);
+            if (cus < 1) cus = 256;
+            if (((long)nvec + vpb_across - 1) / vpb_across <= 2L * cus) {
+                for (k = 0; k < s->nst; k++)
+                    if ((long)vpb_across * (n / s->radix[k]) > most) most = (long)vpb_across * (n / s->radix[k]);
+                thr_min = most <= 1024 ? (int)most : 0;
+            }
+        }
+        if (va_pick_launch(s, prec, vpb_across, lds_max, lds_max, thr_min, &s->across)) return -2;
+    }
 
     /* stage twiddle offsets */
     off = 0;
