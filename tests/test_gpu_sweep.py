@@ -8,8 +8,8 @@ lengths of the two- and three-pass splits, lengths 1..4), a vector count,
 and for every view its own storage: row- or column-major in its block,
 padded between vectors and offset from the block start.  So the cases
 cross the planner's thresholds (power-of-two row kernels, compiled
-schedules from 2^18 elements, splits, Bluestein) with view geometries the
-hand-picked tests do not combine.
+schedules from 2^12 elements and the generic kernel below, splits,
+Bluestein) with view geometries the hand-picked tests do not combine.
 
 Every case goes through the C ABI with the VSIPL user-block sequence and
 checks, as the reference's own tests do for one geometry
@@ -182,14 +182,16 @@ def test_sweep_covers_the_planner_classes():
             seen.add("pow2")
         if _largest_prime(L) > 13:
             seen.add("bluestein")
-        if L * K >= (1 << 18) and prec == 4:
+        if L * K >= (1 << 12):
             seen.add("compiled")
+        if L * K < (1 << 12):
+            seen.add("generic")
         if L > 65536:
             seen.add("split")
         if _largest_prime(L) in (11, 13):
             seen.add("radix11/13")
-    for want in ("ip", "op", "rc", "cr", 4, 8, VSIP_ROW, VSIP_COL, "pow2", "bluestein", "compiled", "split",
-                 "radix11/13"):
+    for want in ("ip", "op", "rc", "cr", 4, 8, VSIP_ROW, VSIP_COL, "pow2", "bluestein", "compiled", "generic",
+                 "split", "radix11/13"):
         assert want in seen, want
 
 
