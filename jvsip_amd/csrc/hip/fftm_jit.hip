@@ -57,6 +57,8 @@ std::map<FastKey, hipFunction_t> g_fast;
 // launch-time compiles already attempted (each key once; a failure stays on
 // the generic kernel without another attempt)
 std::map<FastKey, bool> g_tried;
+// whether a launch may compile a layout its plan did not prepare
+bool g_launch_compiles = true;
 
 // every field in its own bits: n (31), vpb (16), threads (11), prec (4) in a;
 // min (5), mout (4), nst (4) and radix - 1 of each stage (4 each, radices
@@ -310,6 +312,11 @@ static bool desc_layouts(const vhip_fftm_desc *d, int &min, int &mout) {
     return true;
 }
 
+extern "C" void vhip_jit_launch_compiles(int on) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_launch_compiles = on != 0;
+}
+
 // Compile the kernel a descriptor needs (for launches the plan did not
 // prepare at creation, such as fused pipelines); 0 when it is available.
 extern "C" int vhip_jit_prepare_desc(const vhip_fftm_desc *d) {
@@ -348,7 +355,7 @@ extern "C" int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream) {
         std::lock_guard<std::mutex> lock(g_mu);
         auto it = g_fast.find(key);
         if (it != g_fast.end()) fn = it->second;
-        else if (g_tried.find(key) != g_tried.end()) return -100;
+        else if (!g_launch_compiles || g_tried.find(key) != g_tried.end()) return -100;
         else compile_now = g_tried[key] = true;
     }
     if (compile_now) {

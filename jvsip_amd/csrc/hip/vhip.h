@@ -255,6 +255,10 @@ int vhip_jit_prepare(int prec, int n, int nst, const int *radix, int vpb, int th
                      int layout_out);
 int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);
 int vhip_jit_try_launch(const vhip_fftm_desc *d, void *stream);
+/* 0: launches never compile a layout the plan did not prepare (while a
+ * resident kernel waits for the host, which no module load may block); 1: they
+ * may (the default) */
+void vhip_jit_launch_compiles(int on);
 int vhip_jit_prepare_desc(const vhip_fftm_desc *d);
 int vhip_jit_compile_check(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in,
                            int layout_out, const char *arch, size_t *code_bytes);

@@ -454,9 +454,14 @@ static int va_res_start(VA_res *rs)
     if (rs->nslots == 0) return 0;
     /* the persistent launches of the other requests leave these CUs alone */
     rc = vhip_set_reserved_cus((int)rs->grid);
+    /* and no launch compiles a kernel (a module load) while it runs */
+    if (!rc) vhip_jit_launch_compiles(0);
     if (!rc) rc = vhip_resident_launch(&rs->args, (int)rs->grid, VA_BATCH_LDS, rs->stream);
     if (!rc) rs->launched = 1;
-    else vhip_set_reserved_cus(0);
+    else {
+        vhip_set_reserved_cus(0);
+        vhip_jit_launch_compiles(1);
+    }
     return rc;
 }
 
@@ -487,6 +492,7 @@ static int va_res_fini(VA_res *rs)
             return -1;
         }
         vhip_set_reserved_cus(0);
+        vhip_jit_launch_compiles(1);
     }
     if (rs->stream) vhip_stream_destroy(rs->stream);
     if (rs->hblk) vhip_host_free(rs->hblk);
