@@ -61,18 +61,18 @@ std::map<FastKey, bool> g_tried;
 bool g_launch_compiles = true;
 
 // every field in its own bits: n (31), vpb (16), threads (11), prec (4) in a;
-// min (5), mout (4), nst (4) and radix - 1 of each stage (4 each, radices
+// min (6), mout (5), nst (4) and radix - 1 of each stage (4 each, radices
 // 1..16) in b; a schedule the fields cannot hold is not keyed (it runs on
 // the generic kernel)
 bool fast_key(int prec, int n, int nst, const int *radix, int vpb, int threads, int min, int mout, FastKey &k) {
     if (n < 1 || vpb < 1 || vpb > 0xffff || threads < 1 || threads > 2047 || nst < 1 || nst > 8) return false;
     k.a = (unsigned long long)(unsigned)n | ((unsigned long long)vpb << 31) | ((unsigned long long)threads << 47) |
           ((unsigned long long)(prec & 0xf) << 58);
-    if (min < 0 || min > 0x1f || mout < 0 || mout > 0xf) return false;
-    k.b = (unsigned long long)min | ((unsigned long long)mout << 5) | ((unsigned long long)nst << 9);
+    if (min < 0 || min > 0x3f || mout < 0 || mout > 0x1f) return false;
+    k.b = (unsigned long long)min | ((unsigned long long)mout << 6) | ((unsigned long long)nst << 11);
     for (int s = 0; s < nst; ++s) {
         if (radix[s] < 1 || radix[s] > 16) return false;
-        k.b |= (unsigned long long)(radix[s] - 1) << (13 + 4 * s);
+        k.b |= (unsigned long long)(radix[s] - 1) << (15 + 4 * s);
     }
     return true;
 }
@@ -85,9 +85,15 @@ bool jit_on() {
 // the template instance of a schedule; empty when the schedule has no
 // compiled form (more than 8 stages, or not a single pass)
 std::string instance(int prec, int n, int nst, const int *radix, int vpb, int threads, int min, int mout) {
-    // load-side bits: ACROSS, FUSED, REAL, C2R; store-side: ACROSS, FUSED, PTW
-    const int min_bits = VHIP_JIT_ACROSS | VHIP_JIT_FUSED | VHIP_JIT_REAL | VHIP_JIT_C2R;
-    if (nst < 1 || nst > 8 || min < 0 || (min & ~min_bits) || mout < 0 || mout > 15 || (prec != 4 && prec != 8))
+    // load-side bits: ACROSS, FUSED, REAL, C2R, SPLIT_IN; store-side: ACROSS,
+    // FUSED, PTW, SPLIT_OUT
+    const int min_bits = VHIP_JIT_ACROSS | VHIP_JIT_FUSED | VHIP_JIT_REAL | VHIP_JIT_C2R | VHIP_JIT_SPLIT_IN;
+    if (nst < 1 || nst > 8 || min < 0 || (min & ~min_bits) || mout < 0 || mout > 31 || (prec != 4 && prec != 8))
+        return "";
+    // split sides: plain complex data only (no taper, no real side, no
+    // accumulation, no post-twiddle)
+    if ((min & VHIP_JIT_SPLIT_IN) && (min & (VHIP_JIT_FUSED | VHIP_JIT_REAL | VHIP_JIT_C2R))) return "";
+    if ((mout & VHIP_JIT_SPLIT_OUT) && ((mout & (VHIP_JIT_FUSED | VHIP_JIT_PTW)) || (min & (VHIP_JIT_REAL | VHIP_JIT_C2R))))
         return "";
     if ((mout & VHIP_JIT_PTW) && (mout & VHIP_JIT_FUSED)) return "";  // no power accumulation in a split pass
     if ((min & VHIP_JIT_REAL) && (mout & VHIP_JIT_FUSED)) return "";   // no power accumulation after a real input
@@ -105,7 +111,9 @@ std::string instance(int prec, int n, int nst, const int *radix, int vpb, int th
 // layout mode of one side of a descriptor: 0 contiguous rows (single-level
 // vectors), 1 across vectors (any strides, span known); -1 neither
 int side_mode(const vhip_view &g, int map, long nvec) {
-    if (g.split) return -1;
+    // split sides are addressed per vector offset and element stride (any
+    // strides within the span); the mode is then the lane mapping only
+    if (g.split) return g.span > 0 ? (map == VHIP_MAP_ACROSS ? 1 : 0) : -1;
     if (map == VHIP_MAP_ALONG && g.contig && g.nv0 >= nvec) return 0;
     if (map == VHIP_MAP_ACROSS && g.span > 0) return 1;
     return -1;
@@ -301,6 +309,8 @@ static bool desc_layouts(const vhip_fftm_desc *d, int &min, int &mout) {
         return false;
     if (d->kind == VHIP_R2C && (d->acc || d->ptw)) return false;
     if (d->kind == VHIP_C2R && (d->acc || d->ptw || d->tap)) return false;
+    if ((d->in.split || d->out.split) && d->kind != VHIP_C2C) return false;
+    if ((d->in.split && d->tap) || (d->out.split && (d->acc || d->ptw))) return false;
     min = side_mode(d->in, d->map_in, d->nvec);
     mout = side_mode(d->out, d->map_out, d->nvec);
     if (min < 0 || mout < 0) return false;
@@ -309,6 +319,8 @@ static bool desc_layouts(const vhip_fftm_desc *d, int &min, int &mout) {
     if (d->ptw) mout |= VHIP_JIT_PTW;
     if (d->kind == VHIP_R2C) min |= VHIP_JIT_REAL;
     if (d->kind == VHIP_C2R) min |= VHIP_JIT_C2R;
+    if (d->in.split) min |= VHIP_JIT_SPLIT_IN;
+    if (d->out.split) mout |= VHIP_JIT_SPLIT_OUT;
     return true;
 }
 

@@ -31,7 +31,11 @@
 // complex to real (crfftmop): the n + 1 inputs of each vector are combined
 // into conj(Y) in LDS (c2r_pre of the generic kernel), stage 0 reads LDS,
 // and the last stage conjugates back and stores complex element m as real
-// samples 2m, 2m + 1 (offsets of the output side in real elements).
+// samples 2m, 2m + 1 (offsets of the output side in real elements).  Bit 5
+// of MIN / bit 4 of MOUT make that side split complex: the real and
+// imaginary parts in two arrays at the same element offsets, addressed per
+// vector offset and element stride whatever the lane mapping (plain c2c
+// only).
 #pragma once
 #include "fftm_core.hpp"
 
@@ -41,6 +45,8 @@
 #define ST_REAL 4
 #define ST_PTW 8    // store side: the split passes' post-twiddle (vhip_fftm_desc.ptw)
 #define ST_C2R 16   // load side: complex to real (crfftmop)
+#define ST_SPLIT_IN 32   // load side: split complex (real and imaginary arrays)
+#define ST_SPLIT_OUT 16  // store side: split complex
 
 namespace vhipk {
 namespace st {
@@ -64,6 +70,7 @@ constexpr int lstride(int n) { return (lpad<T>(n) % 2 == 0) ? lpad<T>(n) + 1 : l
 template <typename T>
 struct Io {
     rsrc_t in, out, tw;
+    rsrc_t in_im, out_im;  // split sides: the imaginary arrays
     long ivs, ovs;         // ST_CONTIG vector strides
     int ies, oes;          // ST_ACROSS element strides
     int nv;                // vectors in this group
@@ -88,6 +95,7 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
     constexpr bool REAL = (MIN & ST_REAL) != 0;
     constexpr bool C2R = (MIN & ST_C2R) != 0;
     constexpr bool PTW = (MOUT & ST_PTW) != 0;
+    constexpr bool SPIN = (MIN & ST_SPLIT_IN) != 0, SPOUT = (MOUT & ST_SPLIT_OUT) != 0;
     constexpr int RS = (int)sizeof(T);
     // stage 0 of a complex-to-real transform reads the combined input in LDS
     constexpr bool GLOAD = FIRST && !C2R;
@@ -141,6 +149,10 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
                             y.x *= ((const T *)d.tap)[s0];
                             y.y *= ((const T *)d.tap)[s0 + d.tap_s];
                         }
+                    } else if (SPIN) {
+                        const int o = (io.voff_in[v] + (j + r * NBV) * io.ies) * RS;
+                        y.x = Buf<T>::template lds<kLdData>(io.in, o, 0);
+                        y.y = Buf<T>::template lds<kLdData>(io.in_im, o, 0) * io.isgn;
                     } else {
                         if (LIN == ST_CONTIG && (d.stream & 1))
                             y = Buf<T>::template ldc<2>(io.in, (int)(v * io.ivs + j) * ES, r * NBV * ES);
@@ -210,7 +222,11 @@ __device__ __forceinline__ void sstage(cpx<T> *lds, const vhip_fftm_desc &d, con
                         y = cmul(y, cmul(w[t & ((1 << d.ptw_shift) - 1)], w[(1 << d.ptw_shift) + (t >> d.ptw_shift)]));
                     }
                     const cpx<T> z = {y.x * io.scale, y.y * (io.scale * io.osgn)};
-                    if (ACC) {
+                    if (SPOUT) {
+                        const int o = (io.voff_out[v] + (j + r * NBV) * io.oes) * RS;
+                        Buf<T>::template sts<kStData>(io.out, o, 0, z.x);
+                        Buf<T>::template sts<kStData>(io.out_im, o, 0, z.y);
+                    } else if (ACC) {
                         const long o = LOUT == ST_CONTIG ? (io.vbase + v) * io.ovs + j + r * NBV
                                                          : (long)io.voff_out[v] + (long)(j + r * NBV) * io.oes;
                         T *a = (T *)d.acc + o;
@@ -268,7 +284,8 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
     io.voff_out = voff + VPB;
     io.vidx = voff + 2 * VPB;
     constexpr bool PTW = (MOUT & ST_PTW) != 0;
-    if (ACROSS || PTW) {
+    constexpr bool SPIN = (MIN & ST_SPLIT_IN) != 0, SPOUT = (MOUT & ST_SPLIT_OUT) != 0;
+    if (ACROSS || PTW || SPIN || SPOUT) {
         for (int t = threadIdx.x; t < io.nv; t += NT) {
             const long v = vbase + t;
             voff[t] = (int)vec_off(d.in, v);
@@ -281,13 +298,19 @@ __global__ void __launch_bounds__(NT) fftm_static_kernel(const vhip_fftm_desc d)
     constexpr bool REAL = (MIN & ST_REAL) != 0;
     constexpr bool C2R = (MIN & ST_C2R) != 0;
     constexpr int RS = (int)sizeof(T);
-    if ((MIN & 1) == ST_CONTIG && REAL)
+    if (SPIN) {
+        io.in = make_rsrc(d.in.re, d.in.span);
+        io.in_im = make_rsrc(d.in.im, d.in.span);
+    } else if ((MIN & 1) == ST_CONTIG && REAL)
         io.in = make_rsrc((const T *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + 2 * N) * RS);
     else if ((MIN & 1) == ST_CONTIG)
         io.in = make_rsrc((const cpx<T> *)d.in.re + vbase * io.ivs, ((io.nv - 1) * io.ivs + N + (C2R ? 1 : 0)) * ES);
     else
         io.in = make_rsrc(d.in.re, d.in.span);
-    if ((MOUT & 1) == ST_CONTIG && C2R)
+    if (SPOUT) {
+        io.out = make_rsrc(d.out.re, d.out.span);
+        io.out_im = make_rsrc(d.out.im, d.out.span);
+    } else if ((MOUT & 1) == ST_CONTIG && C2R)
         io.out = make_rsrc((T *)d.out.re + vbase * io.ovs, ((io.nv - 1) * io.ovs + 2 * N) * RS);
     else if ((MOUT & 1) == ST_CONTIG)
         io.out = make_rsrc((cpx<T> *)d.out.re + vbase * io.ovs, ((io.nv - 1) * io.ovs + N + (REAL ? 1 : 0)) * ES);

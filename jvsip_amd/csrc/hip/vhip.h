@@ -251,6 +251,8 @@ int vhip_rng_fill(const vhip_rng_desc *d, void *stream);
 #define VHIP_JIT_REAL 4     /* load side: real input, real-to-complex transform */
 #define VHIP_JIT_PTW 8      /* store side: post-twiddle (the split's passes before the last) */
 #define VHIP_JIT_C2R 16     /* load side: complex-to-real transform (n + 1 inputs, real output) */
+#define VHIP_JIT_SPLIT_IN 32   /* load side: split complex input (plain c2c) */
+#define VHIP_JIT_SPLIT_OUT 16  /* store side: split complex output (plain c2c) */
 int vhip_jit_prepare(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in,
                      int layout_out);
 int vhip_jit_ready(int prec, int n, int nst, const int *radix, int vpb, int threads, int layout_in, int layout_out);

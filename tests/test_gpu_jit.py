@@ -456,3 +456,40 @@ def test_one_plan_on_both_layouts(amd, monkeypatch, major):
     amd.cmdestroy_f(hc)
     amd.cblockdestroy_f(br)
     amd.cblockdestroy_f(bc)
+
+
+@pytest.mark.parametrize("m,n,major,op,prec", [(64, 1000, VSIP_ROW, False, 4), (300, 2187, VSIP_ROW, True, 4),
+                                               (1920, 100, VSIP_COL, False, 4), (256, 1024, VSIP_ROW, False, 4),
+                                               (1000, 64, VSIP_COL, True, 4), (40, 2000, VSIP_ROW, False, 8),
+                                               (500, 48, VSIP_COL, True, 8)])
+def test_split_storage_compiled(amd, monkeypatch, m, n, major, op, prec):
+    """Split complex blocks (real and imaginary arrays) run the compiled
+    schedule (compiled at the first launch for the split layout): against the
+    oracle (1e-5 / 1e-12), the same transform on interleaved blocks (1e-6 /
+    1e-13) and the generic kernel on the split blocks."""
+    monkeypatch.delenv("VSIP_AMD_JIT_MIN_ELEMS", raising=False)
+    monkeypatch.delenv("VSIP_AMD_JIT", raising=False)
+    ctype = np.complex64 if prec == 4 else np.complex128
+    tol, tol2 = (1e-5, 1e-6) if prec == 4 else (1e-12, 1e-13)
+    x = datagen.complex_data(107, (m * n,), dtype=ctype)
+    v = fftm.dense_view(m, n)
+    _case("ccfftm%s_%s %s %dx%d on split blocks" % ("op" if op else "ip", "f" if prec == 4 else "d",
+                                                    "ROW" if major == VSIP_ROW else "COL", m, n))
+    kw = dict(major=major, direction=VSIP_FFT_INV, scale=0.5)
+    if op:
+        y0 = np.zeros(m * n, ctype)
+        want = po.ccfftm(x, v, out_blk=y0, out_view=v, **kw)
+        xr, y = fftm.ccfftm(amd, x, v, out_blk=y0, out_view=v, split=True, **kw)
+        np.testing.assert_array_equal(xr, x)
+        _, yi = fftm.ccfftm(amd, x, v, out_blk=y0, out_view=v, **kw)
+    else:
+        want = po.ccfftm(x, v, **kw)
+        y = fftm.ccfftm(amd, x, v, split=True, **kw)
+        yi = fftm.ccfftm(amd, x, v, **kw)
+    _check("split vs oracle", y, want, tol)
+    _check("split vs interleaved", y, yi, tol2)
+    monkeypatch.setenv("VSIP_AMD_JIT", "0")
+    g = fftm.ccfftm(amd, x, v, out_blk=np.zeros(m * n, ctype), out_view=v, split=True, **kw)[1] if op else \
+        fftm.ccfftm(amd, x, v, split=True, **kw)
+    monkeypatch.delenv("VSIP_AMD_JIT")
+    _check("split compiled vs generic", y, g, tol2)

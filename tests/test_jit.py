@@ -40,7 +40,8 @@ def test_planner_schedules_compile(lib, so, prec, n, nvec):
     rad = (ctypes.c_int * len(s["radix"]))(*s["radix"])
     for shape, lin, lout in (("along", 0, 0), ("across", 1, 1), ("across", 0, 1), ("along", 2, 2),
                              ("across", 3, 3), ("along", 4, 0), ("across", 5, 1), ("across", 6, 1),
-                             ("across", 1, 9), ("along", 16, 0), ("across", 17, 1)):
+                             ("across", 1, 9), ("along", 16, 0), ("across", 17, 1), ("along", 32, 16),
+                             ("across", 33, 17), ("across", 33, 9), ("along", 0, 16)):
         vpb, thr, _ = s[shape]
         size = ctypes.c_size_t()
         rc = so.vhip_jit_compile_check(prec, n, len(s["radix"]), rad, vpb, thr, lin, lout, b"gfx950",
@@ -59,3 +60,7 @@ def test_refuses_schedules_without_a_compiled_form(so):
     assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 18, 0, b"gfx950", None) != 0  # complex to real + taper
     assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 16, 9, b"gfx950", None) != 0  # complex to real + post-twiddle
     assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 20, 0, b"gfx950", None) != 0  # complex to real + real input
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 36, 0, b"gfx950", None) != 0  # split input + real input
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 34, 0, b"gfx950", None) != 0  # split input + taper
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 1, 24, b"gfx950", None) != 0  # split output + post-twiddle
+    assert so.vhip_jit_compile_check(4, 16, 2, rad, 1, 64, 0, 18, b"gfx950", None) != 0  # split output + accumulate
