@@ -320,3 +320,45 @@ def test_serve_plain_back_to_back(amd, pools):
         want = po.ccfftm(x[io:io + rows * n], fftm.dense_view(rows, n), direction=d, scale=s)
         assert po.rel_l2(got[oo:oo + rows * n], want) <= 1e-5
     lib.cmalldestroy_f(ov)
+
+
+_SWEEP_LENGTHS = [256, 512, 1000, 1001, 1024, 2187, 3000, 4096, 6000, 8192, 16384, 32768, 40000, 65536]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_serve_sweep(amd, pools, seed):
+    """Random request streams (20..80 requests of mixed lengths, including
+    two-pass ones, 1..64 rows, arrival rates from spread out to all at once,
+    1..4 lanes): every served result equals the plain calls' (1e-6) and a
+    sample the oracle's (1e-5)."""
+    lib = amd
+    x, inb = pools
+    rng = np.random.default_rng(9000 + seed)
+    rate = float(rng.choice([1e3, 2e4, 1e6]))
+    t = np.cumsum(rng.exponential(1.0 / rate, int(rng.integers(20, 81))))
+    reqs, total = [], 0
+    for ti in t:
+        n = int(rng.choice(_SWEEP_LENGTHS))
+        rows = int(rng.integers(1, min(64, POOL // n) + 1))
+        d = -1 if rng.random() < 0.5 else 1
+        scale = float(rng.choice([1.0, 0.5, 1.0 / n]))
+        reqs.append((float(ti), rows, n, d, scale, int(rng.integers(0, POOL - rows * n + 1)), total))
+        total += rows * n
+    lanes, max_jobs = int(rng.integers(1, 5)), int(rng.choice([8, 16, 64]))
+    _case("serve sweep seed %d: %d requests at %g/s, %d lanes, %d jobs" % (seed, len(reqs), rate, lanes, max_jobs))
+    ob, ov = _out_pool(lib, total)
+    res, stats = lib.serve(reqs, inb, ob, budget=1 << 30, lanes=lanes, max_jobs=max_jobs)
+    print("  stats", stats, flush=True)
+    served = _read(lib, ov, total)
+    lib.serve(reqs, inb, ob, budget=1 << 30, plain=True, sync_each=False)
+    plain = _read(lib, ov, total)
+    for i, (ti, rows, n, d, s, io, oo) in enumerate(reqs):
+        sl = slice(oo, oo + rows * n)
+        e_pl = po.rel_l2(served[sl], plain[sl])
+        assert e_pl <= 1e-6, (seed, i, rows, n, e_pl)
+        if i % 7 == 0:
+            want = po.ccfftm(x[io:io + rows * n], fftm.dense_view(rows, n), direction=d, scale=s)
+            e_or = po.rel_l2(served[sl], want)
+            assert e_or <= 1e-5, (seed, i, rows, n, e_or)
+    assert np.all(res["t_done"] >= res["t_admit"])
+    lib.cmalldestroy_f(ov)
