@@ -111,10 +111,12 @@ def ccfftm(lib, blk, view, major=VSIP_ROW, direction=VSIP_FFT_FWD, scale=1.0,
 
     blk: 1-D complex64/complex128 array (the block).  Returns the block(s)
     after release: a new array for in-place, (x_block, y_block) for out of place.
+    split: both blocks split complex (True), or a pair (input, output).
     """
     t = _t(blk.real.dtype.itemsize)
     M, N = view.col_length, view.row_length
-    b = _CBlock(lib, blk, split)
+    split_in, split_out = (split, split) if isinstance(split, bool) else split
+    b = _CBlock(lib, blk, split_in)
     v = getattr(lib, "cmbind_" + t)(b.h, *view)
     if out_blk is None:
         plan = getattr(lib, "ccfftmip_create_" + t)(M, N, scale, direction, major, 0, VSIP_ALG_TIME)
@@ -125,7 +127,7 @@ def ccfftm(lib, blk, view, major=VSIP_ROW, direction=VSIP_FFT_FWD, scale=1.0,
         getattr(lib, "cmdestroy_" + t)(v)
         b.destroy()
         return res
-    ob = _CBlock(lib, out_blk, split)
+    ob = _CBlock(lib, out_blk, split_out)
     ov = getattr(lib, "cmbind_" + t)(ob.h, *out_view)
     plan = getattr(lib, "ccfftmop_create_" + t)(M, N, scale, direction, major, 0, VSIP_ALG_TIME)
     assert plan, "ccfftmop_create failed"

@@ -195,6 +195,74 @@ def test_sweep_covers_the_planner_classes():
         assert want in seen, want
 
 
+# ---- split complex storage ---------------------------------------------------
+
+NCASES_SPLIT = 48
+
+
+def _draw_split(seed):
+    """A complex case on split blocks: (kind, prec, major, L, K, scale,
+    direction, (split in, split out), rng), lengths from the same classes."""
+    kind, prec, major, L, K, scale, direction, rng = _draw(20000 + seed)
+    kind = "ip" if kind == "ip" else "op"
+    split = (True, True) if kind == "ip" else [(True, True), (True, False), (False, True)][int(rng.integers(0, 3))]
+    return kind, prec, major, L, K, scale, direction, split, rng
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(NCASES_SPLIT))
+def test_sweep_split_storage(amd, seed):
+    """ccfftmip / ccfftmop on split complex blocks (real and imaginary parts
+    in two arrays; for out of place either side or both): against the
+    oracle, against the same transform on interleaved blocks (1e-6 / 1e-13),
+    every element outside the output view and the input block unchanged."""
+    kind, prec, major, L, K, scale, direction, split, rng = _draw_split(seed)
+    m, n = (K, L) if major == VSIP_ROW else (L, K)
+    ctype = np.complex64 if prec == 4 else np.complex128
+    tol, tol2 = TOL[prec], (1e-6 if prec == 4 else 1e-13)
+    xv = _layout(rng, m, n)
+    print("\n[case] split seed %d: ccfftm%s_%s %s %dx%d (length %d, %d vectors) scale %g dir %d split %s view %s"
+          % (seed, kind, "f" if prec == 4 else "d", "ROW" if major == VSIP_ROW else "COL", m, n, L, K, scale,
+             direction, split, tuple(xv)), flush=True)
+    sys.stdout.flush()
+    x = datagen.complex_data(seed + 300, (fftm.block_len(xv),), dtype=ctype)
+    kw = dict(major=major, direction=direction, scale=scale)
+    if kind == "ip":
+        y = fftm.ccfftm(amd, x.copy(), xv, split=True, **kw)
+        yi = fftm.ccfftm(amd, x.copy(), xv, **kw)
+        want = po.ccfftm(x, xv, **kw)
+        out = _outside(x.size, xv)
+        np.testing.assert_array_equal(y[out], x[out])
+    else:
+        yv = _layout(rng, m, n)
+        y0 = datagen.complex_data(seed + 7000, (fftm.block_len(yv),), dtype=ctype)
+        xr, y = fftm.ccfftm(amd, x.copy(), xv, out_blk=y0.copy(), out_view=yv, split=split, **kw)
+        np.testing.assert_array_equal(xr, x)
+        yi = fftm.ccfftm(amd, x.copy(), xv, out_blk=y0.copy(), out_view=yv, **kw)[1]
+        want = po.ccfftm(x, xv, out_blk=y0, out_view=yv, **kw)
+        out = _outside(y0.size, yv)
+        np.testing.assert_array_equal(y[out], y0[out])
+    _check("split vs oracle", y, want, tol)
+    _check("split vs interleaved", y, yi, tol2)
+
+
+def test_split_sweep_covers_the_classes():
+    """The split-storage cases reach both kinds, precisions and majors, each
+    mix of split sides, compiled and generic sizes, the splits and Bluestein."""
+    seen = set()
+    for s in range(NCASES_SPLIT):
+        kind, prec, major, L, K, _, _, split, _ = _draw_split(s)
+        seen.update([kind, prec, major, split])
+        seen.add("compiled" if L * K >= (1 << 12) else "generic")
+        if L > 65536:
+            seen.add("split")
+        if _largest_prime(L) > 13:
+            seen.add("bluestein")
+    for want in ("ip", "op", 4, 8, VSIP_ROW, VSIP_COL, (True, True), (True, False), (False, True), "compiled",
+                 "generic", "split", "bluestein"):
+        assert want in seen, want
+
+
 # ---- the 1-D family (vsip_ccfftip/op, rcfftop, crfftop) --------------------
 
 NCASES_1D = 64
