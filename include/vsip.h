@@ -59,6 +59,8 @@ typedef enum { VSIP_FFT_FWD = -1, VSIP_FFT_INV = +1 } vsip_fft_dir;
 typedef enum { VSIP_FFT_IP = 0, VSIP_FFT_OP = 1 } vsip_fft_place;
 typedef enum { VSIP_ALG_SPACE = 0, VSIP_ALG_TIME = 1, VSIP_ALG_NOISE = 2 } vsip_alg_hint;
 typedef enum { VSIP_PRNG = 0, VSIP_NPRNG = 1 } vsip_rng;                  /* vsip.h:159 */
+typedef enum { VSIP_SUPPORT_FULL = 0, VSIP_SUPPORT_SAME = 1, VSIP_SUPPORT_MIN = 2 } vsip_support_region; /* vsip.h:155 */
+typedef enum { VSIP_BIASED = 0, VSIP_UNBIASED = 1 } vsip_bias;          /* vsip.h:156 */
 
 /* ---- opaque handles (vsip.h:165-208) -------------------------------- */
 typedef struct vsip_blockattributes_d vsip_block_d;
@@ -78,6 +80,10 @@ typedef struct vsip_cvviewattributes_f vsip_cvview_f;
 typedef struct vsip_fftattributes_f vsip_fft_f;
 typedef struct vsip_fftattributes_d vsip_fft_d;
 typedef struct vsip_randobject vsip_randstate;                           /* vsip.h:191 */
+typedef struct vsip_corr1dattributes_f vsip_corr1d_f;                    /* vsip.h:215 */
+typedef struct vsip_ccorr1dattributes_f vsip_ccorr1d_f;                  /* vsip.h:216 */
+typedef struct vsip_corr1dattributes_d vsip_corr1d_d;                    /* vsip.h:217 */
+typedef struct vsip_ccorr1dattributes_d vsip_ccorr1d_d;                  /* vsip.h:218 */
 
 /* ---- attribute structs (vsip.h:242-245, 277-278) -------------------- */
 typedef struct { vsip_offset offset; vsip_stride row_stride; vsip_length row_length;
@@ -101,6 +107,9 @@ typedef struct { vsip_scalar_vi input; vsip_scalar_vi output; vsip_fft_place pla
                  vsip_scalar_d scale; vsip_fft_dir dir; } vsip_fft_attr_d;
 typedef struct { vsip_scalar_vi input; vsip_scalar_vi output; vsip_fft_place place;
                  vsip_scalar_f scale; vsip_fft_dir dir; } vsip_fft_attr_f;
+/* vsip.h:282 */
+typedef struct { vsip_scalar_vi ref_len; vsip_scalar_vi data_len; vsip_support_region support;
+                 vsip_scalar_vi lag_len; } vsip_corr1d_attr;
 
 /* ---- library lifecycle ---------------------------------------------- */
 int vsip_init(void *ptr);                                         /* vsip.h:432 */
@@ -321,6 +330,39 @@ int vsip_fft_destroy_f(vsip_fft_f*);                                            
 int vsip_fft_destroy_d(vsip_fft_d*);                                                  /* vsip.h:423 */
 void vsip_fft_getattr_f(const vsip_fft_f*, vsip_fft_attr_f*);                         /* vsip.h:945 */
 void vsip_fft_getattr_d(const vsip_fft_d*, vsip_fft_attr_d*);                         /* vsip.h:944 */
+
+/* ---- 1-D correlation (reference length M, data length N, M <= N) ---------
+ * y = the correlation of reference h with data x over the support region
+ * (FULL: M + N - 1 lags, SAME: N, MIN: N - M + 1), unbiased on request.  The
+ * reference computes it with zero-padded FFTs of a power-of-two length
+ * above M + N - 1 (vsip_ccorrelate1d_f.c:123-196), and so does this library
+ * on the device.  Operands are views of admitted blocks.  create returns
+ * NULL for M < 1 or M > N.  Calls on one object share its transform buffer,
+ * so one object serves one thread at a time. */
+vsip_corr1d_f* vsip_corr1d_create_f(vsip_length M, vsip_length N, vsip_support_region support,
+                                    unsigned int ntimes, vsip_alg_hint hint);          /* vsip.h:1798 */
+vsip_corr1d_d* vsip_corr1d_create_d(vsip_length M, vsip_length N, vsip_support_region support,
+                                    unsigned int ntimes, vsip_alg_hint hint);          /* vsip.h:1797 */
+vsip_ccorr1d_f* vsip_ccorr1d_create_f(vsip_length M, vsip_length N, vsip_support_region support,
+                                      unsigned int ntimes, vsip_alg_hint hint);        /* vsip.h:1785 */
+vsip_ccorr1d_d* vsip_ccorr1d_create_d(vsip_length M, vsip_length N, vsip_support_region support,
+                                      unsigned int ntimes, vsip_alg_hint hint);        /* vsip.h:1784 */
+int vsip_corr1d_destroy_f(vsip_corr1d_f*);                                             /* vsip.h:408 */
+int vsip_corr1d_destroy_d(vsip_corr1d_d*);                                             /* vsip.h:407 */
+int vsip_ccorr1d_destroy_f(vsip_ccorr1d_f*);                                           /* vsip.h:380 */
+int vsip_ccorr1d_destroy_d(vsip_ccorr1d_d*);                                           /* vsip.h:379 */
+void vsip_corr1d_getattr_f(const vsip_corr1d_f*, vsip_corr1d_attr*);                   /* vsip.h:821 */
+void vsip_corr1d_getattr_d(const vsip_corr1d_d*, vsip_corr1d_attr*);                   /* vsip.h:820 */
+void vsip_ccorr1d_getattr_f(const vsip_ccorr1d_f*, vsip_corr1d_attr*);                 /* vsip.h:723 */
+void vsip_ccorr1d_getattr_d(const vsip_ccorr1d_d*, vsip_corr1d_attr*);                 /* vsip.h:722 */
+void vsip_correlate1d_f(const vsip_corr1d_f*, vsip_bias, const vsip_vview_f *h, const vsip_vview_f *x,
+                        const vsip_vview_f *y);                                         /* vsip.h:823 */
+void vsip_correlate1d_d(const vsip_corr1d_d*, vsip_bias, const vsip_vview_d *h, const vsip_vview_d *x,
+                        const vsip_vview_d *y);                                         /* vsip.h:822 */
+void vsip_ccorrelate1d_f(const vsip_ccorr1d_f*, vsip_bias, const vsip_cvview_f *h, const vsip_cvview_f *x,
+                         const vsip_cvview_f *y);                                       /* vsip.h:725 */
+void vsip_ccorrelate1d_d(const vsip_ccorr1d_d*, vsip_bias, const vsip_cvview_d *h, const vsip_cvview_d *x,
+                         const vsip_cvview_d *y);                                       /* vsip.h:724 */
 
 /* ---- element operations on transform operands ---------------------------- */
 void vsip_rscvmul_f(vsip_scalar_f, const vsip_cvview_f*, const vsip_cvview_f*);        /* vsip.h:1216 */

@@ -214,6 +214,24 @@ int vhip_elem_sumsq(int prec, const vhip_mat *a, long m, long n, double *out, vo
 int vhip_elem_real(int prec, int op, const vhip_mat *a, const vhip_mat *b, const vhip_mat *r, const void *vec,
                    long vs, double alpha, long m, long n, void *stream);
 
+/* 1-D correlation passes (corr_ops.hip) on a transform buffer of two rows
+ * of N complex elements (interleaved, precision prec):
+ *   pack     row 0 = N - n zeros then x (n elements), row 1 = h (m elements)
+ *            then zeros; x_cplx / h_cplx 0 for real vector views
+ *   product  row0[k] = conj(row0[k] conj(row1[k])) / N
+ *   window   y[i] = w_i row0[o + i], i < len, w_i the unbiasing weights of
+ *            the support (VHIP_CORR_*) when unbiased; y_cplx 0 stores the
+ *            real part to a real view
+ * x, h, y are vector views: element j at re (+ im) + j * rs. */
+#define VHIP_CORR_FULL 0
+#define VHIP_CORR_SAME 1
+#define VHIP_CORR_MIN 2
+int vhip_corr_pack(int prec, const vhip_mat *x, int x_cplx, long n, const vhip_mat *h, int h_cplx, long m,
+                   void *buf, long N, void *stream);
+int vhip_corr_product(int prec, void *buf, long N, void *stream);
+int vhip_corr_window(int prec, const void *buf, long N, long o, const vhip_mat *y, int y_cplx, long len,
+                     int support, int unbiased, long m, long n, void *stream);
+
 /* One fill of a strided m x n view by a VSIPL generator (rng.hip).  Element
  * e (column-major order when by_col, else row-major) takes draws
  * D e + 1 .. D e + D of this launch.  x0 / x10 are the generator states

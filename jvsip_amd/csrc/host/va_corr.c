@@ -1,0 +1,228 @@
+/*
+ * va_corr.c -- 1-D correlation: vsip_corr1d (real) and vsip_ccorr1d
+ * (complex) objects and vsip_[c]correlate1d_[fd], a caller of the FFT path.
+ *
+ * The reference (c_VSIP_src/vsip_ccorrelate1d_f.c:123-196,
+ * vsip_correlate1d_f.c:82-151, creation in vsip_ccorr1d_create_f.c:17-54)
+ * pads the data x (length n) and the reference h (length m) into two
+ * complex vectors of length N, the smallest power of two above
+ * mn = m + n - 1 (x at the end after N - n zeros, h at the start), takes
+ * their forward FFTs, forms conj(X conj(H)) / N, takes one more forward FFT
+ * and copies lags N - mn + s .. of it to y (s = 0 FULL, (m - 1)/2 SAME,
+ * m - 1 MIN), divided by the support's unbiasing weights on request; the
+ * real form uses zero imaginary parts and keeps the real part.
+ *
+ * Here the same steps run on the device: the two padded rows of one
+ * 2 x N transform buffer are filled by one element pass, transformed by
+ * one multiple-FFT launch (two ROW vectors), multiplied in one pass,
+ * transformed again (row 0) and windowed into y in one pass (corr_ops.hip);
+ * the FFTs are this library's own plans, so every length class they serve
+ * applies.  create returns NULL for m < 1 or m > n (the reference's SAME
+ * and MIN windows assume m <= n) and for an unknown support.
+ *   create   vsip.h:1784-1785, 1797-1798
+ *   destroy  vsip.h:379-380, 407-408 (returns 0)
+ *   getattr  vsip_ccorr1d_getattr_f.c:17-26 (ref_len m, data_len n,
+ *            lag_len, support)
+ *   execute  vsip.h:724-725, 822-823
+ *
+ * C89.
+ */
+#include <stdlib.h>
+#include "va_internal.h"
+
+typedef struct {
+    int markings;
+    int prec;                  /* 4 or 8 */
+    int cplx;                  /* 1 ccorr1d, 0 corr1d */
+    vsip_length m, n, mn, N, lag_len;
+    vsip_support_region support;
+    VA_block *buf;             /* 2 x N complex: row 0 the data, row 1 the reference */
+    VA_mview *rows, *row0;
+    VA_fftm fft2, fft1;        /* forward, scale 1: both rows; row 0 */
+    int have2, have1;
+} VA_corr;
+
+struct vsip_corr1dattributes_f { VA_corr c; };
+struct vsip_corr1dattributes_d { VA_corr c; };
+struct vsip_ccorr1dattributes_f { VA_corr c; };
+struct vsip_ccorr1dattributes_d { VA_corr c; };
+
+static void va_corr_free(VA_corr *c)
+{
+    if (!c) return;
+    if (c->have2) va_fftm_release(&c->fft2);
+    if (c->have1) va_fftm_release(&c->fft1);
+    if (c->rows) va_mview_destroy(c->rows);
+    if (c->row0) va_mview_destroy(c->row0);
+    if (c->buf) va_block_destroy(c->buf);
+    c->markings = VA_FREED;
+    free(c);
+}
+
+static VA_corr *va_corr_new(const char *fn, int prec, int cplx, vsip_length m, vsip_length n,
+                            vsip_support_region support, unsigned int ntimes, vsip_alg_hint hint)
+{
+    VA_corr *c;
+    vsip_length N = 2;
+    if (va_ensure_init()) return NULL;
+    if (m < 1 || n < m || n > ((vsip_length)1 << 40)) {
+        va_error(fn, "needs 1 <= M <= N", -1);
+        return NULL;
+    }
+    if (support != VSIP_SUPPORT_FULL && support != VSIP_SUPPORT_SAME && support != VSIP_SUPPORT_MIN) {
+        va_error(fn, "unknown support region", (int)support);
+        return NULL;
+    }
+    c = (VA_corr *)calloc(1, sizeof(VA_corr));
+    if (!c) return NULL;
+    c->prec = prec;
+    c->cplx = cplx;
+    c->m = m;
+    c->n = n;
+    c->mn = m + n - 1;
+    c->support = support;
+    c->lag_len = support == VSIP_SUPPORT_MIN ? n - m + 1 : support == VSIP_SUPPORT_SAME ? n : c->mn;
+    while (N <= c->mn) N *= 2;
+    c->N = N;
+    c->buf = va_block_create(prec, 1, 2 * N, VSIP_MEM_NONE);
+    if (!c->buf) goto fail;
+    c->rows = va_mview_bind(c->buf, 0, (vsip_stride)N, 2, 1, N);
+    c->row0 = va_mview_bind(c->buf, 0, (vsip_stride)N, 1, 1, N);
+    if (!c->rows || !c->row0) goto fail;
+    /* a plan whose creation failed is still released */
+    c->have2 = 1;
+    if (va_fftm_init(&c->fft2, prec, VA_CCFFTIP, 2, N, 1.0, VSIP_FFT_FWD, VSIP_ROW, ntimes, hint)) goto fail;
+    c->have1 = 1;
+    if (va_fftm_init(&c->fft1, prec, VA_CCFFTIP, 1, N, 1.0, VSIP_FFT_FWD, VSIP_ROW, ntimes, hint)) goto fail;
+    c->markings = VA_VALID;
+    return c;
+fail:
+    va_error(fn, "allocation or plan creation failed", -1);
+    va_corr_free(c);
+    return NULL;
+}
+
+static int va_corr_view(const char *fn, const VA_mview *v, vsip_length len)
+{
+    if (!v || !v->block || !v->block->admitted || !v->block->dev_r) {
+        va_error(fn, "view of a released block", -1);
+        return -1;
+    }
+    if (v->row_length != len || v->col_length != 1) {
+        va_error(fn, "view length does not match the correlation object", (int)v->row_length);
+        return -1;
+    }
+    return 0;
+}
+
+static void va_correlate(const char *fn, const VA_corr *c, int prec, int cplx, vsip_bias bias, const VA_mview *h,
+                         const VA_mview *x, const VA_mview *y)
+{
+    vhip_mat mh, mx, my;
+    void *st, *buf;
+    long o;
+    int rc;
+    if (va_ensure_init()) return;
+    if (!c || c->markings != VA_VALID || c->prec != prec || c->cplx != cplx) {
+        va_error(fn, "not a correlation object of this type", -1);
+        return;
+    }
+    if (va_corr_view(fn, h, c->m) || va_corr_view(fn, x, c->n) || va_corr_view(fn, y, c->lag_len)) return;
+    va_view_mat(h, 1, &mh);
+    va_view_mat(x, 1, &mx);
+    va_view_mat(y, 1, &my);
+    st = vhip_stream();
+    buf = c->buf->dev_r;
+    o = (long)(c->N - c->mn);
+    if (c->support == VSIP_SUPPORT_SAME) o += (long)(c->m - 1) / 2;
+    else if (c->support == VSIP_SUPPORT_MIN) o += (long)c->m - 1;
+    rc = vhip_corr_pack(prec, &mx, cplx, (long)c->n, &mh, cplx, (long)c->m, buf, (long)c->N, st);
+    if (!rc) rc = va_fftm_exec(&c->fft2, c->rows, c->rows, st);
+    if (!rc) rc = vhip_corr_product(prec, buf, (long)c->N, st);
+    if (!rc) rc = va_fftm_exec(&c->fft1, c->row0, c->row0, st);
+    if (!rc)
+        rc = vhip_corr_window(prec, buf, (long)c->N, o, &my, cplx, (long)c->lag_len,
+                              c->support == VSIP_SUPPORT_FULL ? VHIP_CORR_FULL
+                              : c->support == VSIP_SUPPORT_SAME ? VHIP_CORR_SAME : VHIP_CORR_MIN,
+                              bias == VSIP_UNBIASED, (long)c->m, (long)c->n, st);
+    if (rc) va_error(fn, vhip_error_string(rc), rc);
+}
+
+static void va_corr_attr(const VA_corr *c, vsip_corr1d_attr *a)
+{
+    if (!c || !a) return;
+    a->ref_len = c->m;
+    a->data_len = c->n;
+    a->support = c->support;
+    a->lag_len = c->lag_len;
+}
+
+/* ---- create ------------------------------------------------------------- */
+
+vsip_corr1d_f *vsip_corr1d_create_f(vsip_length M, vsip_length N, vsip_support_region support,
+                                    unsigned int ntimes, vsip_alg_hint hint)
+{
+    return (vsip_corr1d_f *)va_corr_new("vsip_corr1d_create_f", 4, 0, M, N, support, ntimes, hint);
+}
+
+vsip_corr1d_d *vsip_corr1d_create_d(vsip_length M, vsip_length N, vsip_support_region support,
+                                    unsigned int ntimes, vsip_alg_hint hint)
+{
+    return (vsip_corr1d_d *)va_corr_new("vsip_corr1d_create_d", 8, 0, M, N, support, ntimes, hint);
+}
+
+vsip_ccorr1d_f *vsip_ccorr1d_create_f(vsip_length M, vsip_length N, vsip_support_region support,
+                                      unsigned int ntimes, vsip_alg_hint hint)
+{
+    return (vsip_ccorr1d_f *)va_corr_new("vsip_ccorr1d_create_f", 4, 1, M, N, support, ntimes, hint);
+}
+
+vsip_ccorr1d_d *vsip_ccorr1d_create_d(vsip_length M, vsip_length N, vsip_support_region support,
+                                      unsigned int ntimes, vsip_alg_hint hint)
+{
+    return (vsip_ccorr1d_d *)va_corr_new("vsip_ccorr1d_create_d", 8, 1, M, N, support, ntimes, hint);
+}
+
+/* ---- destroy ------------------------------------------------------------ */
+
+int vsip_corr1d_destroy_f(vsip_corr1d_f *c) { va_corr_free(c ? &c->c : NULL); return 0; }
+int vsip_corr1d_destroy_d(vsip_corr1d_d *c) { va_corr_free(c ? &c->c : NULL); return 0; }
+int vsip_ccorr1d_destroy_f(vsip_ccorr1d_f *c) { va_corr_free(c ? &c->c : NULL); return 0; }
+int vsip_ccorr1d_destroy_d(vsip_ccorr1d_d *c) { va_corr_free(c ? &c->c : NULL); return 0; }
+
+/* ---- getattr ------------------------------------------------------------ */
+
+void vsip_corr1d_getattr_f(const vsip_corr1d_f *c, vsip_corr1d_attr *a) { va_corr_attr(c ? &c->c : NULL, a); }
+void vsip_corr1d_getattr_d(const vsip_corr1d_d *c, vsip_corr1d_attr *a) { va_corr_attr(c ? &c->c : NULL, a); }
+void vsip_ccorr1d_getattr_f(const vsip_ccorr1d_f *c, vsip_corr1d_attr *a) { va_corr_attr(c ? &c->c : NULL, a); }
+void vsip_ccorr1d_getattr_d(const vsip_ccorr1d_d *c, vsip_corr1d_attr *a) { va_corr_attr(c ? &c->c : NULL, a); }
+
+/* ---- execute ------------------------------------------------------------ */
+
+void vsip_correlate1d_f(const vsip_corr1d_f *c, vsip_bias bias, const vsip_vview_f *h, const vsip_vview_f *x,
+                        const vsip_vview_f *y)
+{
+    va_correlate("vsip_correlate1d_f", c ? &c->c : NULL, 4, 0, bias, h ? &h->v : NULL, x ? &x->v : NULL,
+                 y ? &y->v : NULL);
+}
+
+void vsip_correlate1d_d(const vsip_corr1d_d *c, vsip_bias bias, const vsip_vview_d *h, const vsip_vview_d *x,
+                        const vsip_vview_d *y)
+{
+    va_correlate("vsip_correlate1d_d", c ? &c->c : NULL, 8, 0, bias, h ? &h->v : NULL, x ? &x->v : NULL,
+                 y ? &y->v : NULL);
+}
+
+void vsip_ccorrelate1d_f(const vsip_ccorr1d_f *c, vsip_bias bias, const vsip_cvview_f *h, const vsip_cvview_f *x,
+                         const vsip_cvview_f *y)
+{
+    va_correlate("vsip_ccorrelate1d_f", c ? &c->c : NULL, 4, 1, bias, h ? &h->v : NULL, x ? &x->v : NULL,
+                 y ? &y->v : NULL);
+}
+
+void vsip_ccorrelate1d_d(const vsip_ccorr1d_d *c, vsip_bias bias, const vsip_cvview_d *h, const vsip_cvview_d *x,
+                         const vsip_cvview_d *y)
+{
+    va_correlate("vsip_ccorrelate1d_d", c ? &c->c : NULL, 8, 1, bias, h ? &h->v : NULL, x ? &x->v : NULL,
+                 y ? &y->v : NULL);
+}

@@ -262,3 +262,32 @@ def crfft(lib, zblk, zview, xblk, xview, scale=1.0):
     zb.destroy()
     xb.destroy()
     return res
+
+
+def correlate1d(lib, h_blk, h_view, x_blk, x_view, y_blk, y_view, support=0, bias=0, split=False, ntimes=0):
+    """1-D correlation of reference view h with data view x into view y
+    (vsip_correlate1d for real blocks, vsip_ccorrelate1d for complex ones):
+    the object is created for (h length, x length, support), each block is a
+    user block admitted before and released after the call.  Returns the y
+    block after release (split: complex blocks split)."""
+    cplx = np.iscomplexobj(h_blk)
+    t = _t(h_blk.real.dtype.itemsize)
+    if cplx:
+        blocks = [_CBlock(lib, a, split) for a in (h_blk, x_blk, y_blk)]
+        bind, kind = "cvbind_", "ccorr1d"
+    else:
+        blocks = [_RBlock(lib, a) for a in (h_blk, x_blk, y_blk)]
+        bind, kind = "vbind_", "corr1d"
+    views = [getattr(lib, bind + t)(b.h, *v) for b, v in zip(blocks, (h_view, x_view, y_view))]
+    obj = getattr(lib, kind + "_create_" + t)(h_view.length, x_view.length, support, ntimes, VSIP_ALG_TIME)
+    assert obj, kind + "_create failed"
+    getattr(lib, ("ccorrelate1d_" if cplx else "correlate1d_") + t)(obj, bias, *views)
+    res = blocks[2].release()
+    blocks[0].release()
+    blocks[1].release()
+    getattr(lib, kind + "_destroy_" + t)(obj)
+    for v in views:
+        getattr(lib, ("cvdestroy_" if cplx else "vdestroy_") + t)(v)
+    for b in blocks:
+        b.destroy()
+    return res

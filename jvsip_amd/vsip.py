@@ -26,6 +26,8 @@ VSIP_FFT_FWD, VSIP_FFT_INV = -1, 1
 VSIP_FFT_IP, VSIP_FFT_OP = 0, 1
 VSIP_MEM_NONE = 0
 VSIP_ALG_TIME = 1
+VSIP_SUPPORT_FULL, VSIP_SUPPORT_SAME, VSIP_SUPPORT_MIN = 0, 1, 2
+VSIP_BIASED, VSIP_UNBIASED = 0, 1
 
 
 class cscalar_f(ctypes.Structure):
@@ -67,6 +69,10 @@ class fft_attr_f(ctypes.Structure):
 
 class fft_attr_d(ctypes.Structure):
     _fields_ = [("input", c_vi), ("output", c_vi), ("place", c_bl), ("scale", ctypes.c_double), ("dir", c_bl)]
+
+
+class corr1d_attr(ctypes.Structure):
+    _fields_ = [("ref_len", c_vi), ("data_len", c_vi), ("support", c_bl), ("lag_len", c_vi)]
 
 
 class vattr(ctypes.Structure):
@@ -187,6 +193,15 @@ def _sig(t):
         "crfftop": (None, [P, P, P]),
         "fft_destroy": (ctypes.c_int, [P]),
         "fft_getattr": (None, [P, ctypes.POINTER(attr1)]),
+        # 1-D correlation
+        "corr1d_create": (P, [c_vi, c_vi, c_bl, ctypes.c_uint, c_bl]),
+        "ccorr1d_create": (P, [c_vi, c_vi, c_bl, ctypes.c_uint, c_bl]),
+        "corr1d_destroy": (ctypes.c_int, [P]),
+        "ccorr1d_destroy": (ctypes.c_int, [P]),
+        "corr1d_getattr": (None, [P, ctypes.POINTER(corr1d_attr)]),
+        "ccorr1d_getattr": (None, [P, ctypes.POINTER(corr1d_attr)]),
+        "correlate1d": (None, [P, c_bl, P, P, P]),
+        "ccorrelate1d": (None, [P, c_bl, P, P, P]),
     }
 
 

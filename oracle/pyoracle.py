@@ -127,6 +127,63 @@ def komega(data, ts_taper, array_taper, gram, gain, rc_scale=1.0, cc_scale=-1.0,
     return np.asarray(gram, np.float64) + gain * (y.real ** 2 + y.imag ** 2)
 
 
+def corr_length(m, n):
+    """FFT length of a correlation object: the smallest power of two above
+    m + n - 1 (c_VSIP_src/vsip_ccorr1d_create_f.c:35-41)."""
+    N = 2
+    while N <= m + n - 1:
+        N *= 2
+    return N
+
+
+def corr_divisors(m, n, support):
+    """Divisors of the unbiased correlation at each output lag, as the
+    reference's loops apply them (m <= n):
+      FULL  VI_cvunbiasfull_f (vsip_ccorrelate1d_f.c:16-58): the first m lags
+            by 1, 2, ..., m; then by m; the last min(m, n - 1) by their count
+            down to 1
+      SAME  VI_cvunbiassame_f (:60-99): i < m/2 by i + ceil(m/2); i < n - m/2
+            by m; then by n + m/2 - i
+      MIN   (:186-188): by m"""
+    if support == 2:
+        return np.full(n - m + 1, float(m))
+    i = np.arange(n if support == 1 else m + n - 1, dtype=np.float64)
+    if support == 1:
+        n0 = m // 2
+        return np.where(i < n0, i + n0 + (m & 1), np.where(i < n - n0, float(m), n + n0 - i))
+    mn, l3 = m + n - 1, min(m, n - 1)
+    return np.where(i < m, i + 1, np.where(i < mn - l3, float(m), mn - i))
+
+
+def correlate1d(h, x, support=0, bias=0):
+    """1-D correlation of reference h (length m) with data x (length n >= m),
+    the steps of vsip_ccorrelate1d_f (c_VSIP_src/vsip_ccorrelate1d_f.c:123-196;
+    the real form vsip_correlate1d_f.c:82-151 takes zero imaginary parts and
+    keeps the real part) in double precision:
+      xt = N - n zeros then x; ht = h then N - m zeros              (:135-153)
+      z = FFT(conj(FFT(xt) conj(FFT(ht))) / N), forward transforms  (:155-161)
+      y = z[N - mn + s : N - mn + s + lag_len], s = 0 (FULL),
+          (m - 1)/2 (SAME), m - 1 (MIN)                             (:164-193)
+      unbiased: y / corr_divisors(m, n, support)
+    Returns complex128 for complex inputs, float64 for real ones."""
+    real = not (np.iscomplexobj(h) or np.iscomplexobj(x))
+    h = np.asarray(h, dtype=np.complex128)
+    x = np.asarray(x, dtype=np.complex128)
+    m, n = h.size, x.size
+    mn, N = m + n - 1, corr_length(m, n)
+    xt = np.zeros(N, np.complex128)
+    xt[N - n:] = x
+    ht = np.zeros(N, np.complex128)
+    ht[:m] = h
+    z = fft(np.conj(fft(xt, -1) * np.conj(fft(ht, -1))) / N, -1)
+    s = 0 if support == 0 else (m - 1) // 2 if support == 1 else m - 1
+    lag = mn if support == 0 else n if support == 1 else n - m + 1
+    y = z[N - mn + s:N - mn + s + lag]
+    if bias == 1:
+        y = y / corr_divisors(m, n, support)
+    return y.real.copy() if real else y.copy()
+
+
 _REF = None
 
 
