@@ -226,6 +226,10 @@ def cpu_configs(ref):
     out["f3_komega_1024x8192"] = {"s_per_call": timed(lambda: k.step(d))}
     ref.malldestroy_f(d)
     k.destroy()
+    for cplx in (True, False):
+        c = _corr_setup(ref, cplx, 1024, 1 << 20)
+        out[("ccorrelate1d" if cplx else "correlate1d") + "_f_1024x2^20"] = {"s_per_call": timed(c["call"])}
+        c["cleanup"]()
     return out
 
 
@@ -693,6 +697,81 @@ def config_fft1d(lib, hip):
     return out
 
 
+def _corr_setup(lib, cplx, m, n, support=0, bias=1):
+    """A vsip_[c]correlate1d_f object for reference length m and data length
+    n on user blocks (N(0,1) data) admitted to the library; returns a dict
+    with the call, the release that returns y, the cleanup and the inputs."""
+    from jvsip_amd import datagen
+    lag = m + n - 1 if support == 0 else n if support == 1 else n - m + 1
+    if cplx:
+        arrs = [datagen.complex_data(21, (m,)), datagen.complex_data(22, (n,)), np.zeros(lag, np.complex64)]
+        blocks = [lib.cblockbind_f(a.ctypes.data, None, a.size, 0) for a in arrs]
+        for b in blocks:
+            lib.cblockadmit_f(b, 1)
+        views = [lib.cvbind_f(b, 0, 1, a.size) for b, a in zip(blocks, arrs)]
+        obj = lib.ccorr1d_create_f(m, n, support, 0, 1)
+        run, destroy = lib.ccorrelate1d_f, lib.ccorr1d_destroy_f
+    else:
+        arrs = [datagen.real_data(21, (m,)), datagen.real_data(22, (n,)), np.zeros(lag, np.float32)]
+        blocks = [lib.blockbind_f(a.ctypes.data, a.size, 0) for a in arrs]
+        for b in blocks:
+            lib.blockadmit_f(b, 1)
+        views = [lib.vbind_f(b, 0, 1, a.size) for b, a in zip(blocks, arrs)]
+        obj = lib.corr1d_create_f(m, n, support, 0, 1)
+        run, destroy = lib.correlate1d_f, lib.corr1d_destroy_f
+    assert obj and all(views), lib.last_error() if hasattr(lib, "last_error") else "create failed"
+
+    def result():
+        if cplx:
+            r, i = ctypes.c_void_p(), ctypes.c_void_p()
+            lib.cblockrelease_f(blocks[2], 1, ctypes.byref(r), ctypes.byref(i))
+        else:
+            lib.blockrelease_f(blocks[2], 1)
+        return arrs[2].copy()
+
+    def cleanup():
+        destroy(obj)
+        for v in views:
+            (lib.cvdestroy_f if cplx else lib.vdestroy_f)(v)
+        for b in blocks:
+            (lib.cblockdestroy_f if cplx else lib.blockdestroy_f)(b)
+
+    return {"call": lambda: run(obj, bias, *views), "result": result, "cleanup": cleanup,
+            "h": arrs[0], "x": arrs[1], "lag": lag}
+
+
+def _corr_numpy(h, x):
+    """FULL unbiased correlation in float64 with numpy's FFT:
+    y[i] = sum_j h[j] conj(x[i + j - (m - 1)]) / min(i + 1, m, m + n - 1 - i)."""
+    m, n = h.size, x.size
+    L = 1 << int(np.ceil(np.log2(m + n - 1)))
+    c = np.fft.ifft(np.fft.fft(np.asarray(x, np.complex128), L) * np.conj(np.fft.fft(np.asarray(h, np.complex128), L)))
+    i = np.arange(m + n - 1)
+    y = np.conj(c[(i - (m - 1)) % L]) / np.minimum(np.minimum(i + 1, m), m + n - 1 - i)
+    return y if np.iscomplexobj(h) else y.real
+
+
+def config_corr1d(lib, hip):
+    """vsip_ccorrelate1d_f and vsip_correlate1d_f, FULL support, unbiased:
+    reference 1024, data 2^20 (transforms of N = 2^21, the three-pass
+    split), on admitted unit-stride views; accuracy against numpy FP64"""
+    out = {}
+    m, n = 1024, 1 << 20
+    for cplx in (True, False):
+        s = _corr_setup(lib, cplx, m, n)
+        _, t = _time_calls(lib, hip, s["call"], 20)
+        y = s["result"]()
+        want = _corr_numpy(s["h"], s["x"])
+        esz = 8 if cplx else 4
+        out[("ccorrelate1d" if cplx else "correlate1d") + "_f_1024x2^20"] = {
+            "us": round(t * 1e6, 2), "fft_length": 1 << 21,
+            "io_bytes": (m + n + s["lag"]) * esz,
+            "gb_per_s_io": round((m + n + s["lag"]) * esz / t / 1e9, 1),
+            "rel_l2_vs_numpy_fp64": float(np.linalg.norm(y - want) / np.linalg.norm(want))}
+        s["cleanup"]()
+    return out
+
+
 def config_f3_komega(lib, hip):
     """k-omega beamformer step (the reference's kw.c komega()): the VSIPL call
     sequence (vmmul ROW, vmmul COL, rcfftmop ROW, ccfftmip COL, mcmagsq,
@@ -891,6 +970,7 @@ def main():
                          ("c4_batched_uneven", lambda: config_c4(lib, hip, legs=args.c4_legs)),
                          ("c5_served", lambda: config_c5(lib)),
                          ("f1_fft1d", lambda: config_fft1d(lib, hip)),
+                         ("corr1d", lambda: config_corr1d(lib, hip)),
                          ("f3_komega", lambda: config_f3_komega(lib, hip))):
             if args.only and not any(name.startswith(o.strip()) for o in args.only.split(",") if o.strip()):
                 continue
