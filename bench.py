@@ -751,6 +751,18 @@ def _corr_numpy(h, x):
     return y if np.iscomplexobj(h) else y.real
 
 
+def _corr_length(mn):
+    """The library's transform length for mn lags: the smallest 2^a 3^b 5^c
+    >= mn, or the reference's power of two above mn when that is within
+    1.15x of it."""
+    p2 = 2
+    while p2 <= mn:
+        p2 *= 2
+    smooth = min(v for v in (2 ** a * 3 ** b * 5 ** c for a in range(64) for b in range(41) for c in range(28)
+                             if 2 ** a * 3 ** b * 5 ** c < p2) if v >= mn) if mn < p2 else p2
+    return p2 if p2 <= 1.15 * smooth else smooth
+
+
 def config_corr1d(lib, hip):
     """vsip_ccorrelate1d_f and vsip_correlate1d_f, FULL support, unbiased:
     reference 1024, data 2^20 (transforms of N = 2^21, the three-pass
@@ -764,7 +776,7 @@ def config_corr1d(lib, hip):
         want = _corr_numpy(s["h"], s["x"])
         esz = 8 if cplx else 4
         out[("ccorrelate1d" if cplx else "correlate1d") + "_f_1024x2^20"] = {
-            "us": round(t * 1e6, 2), "fft_length": 1 << 21,
+            "us": round(t * 1e6, 2), "transform_length": _corr_length(m + n - 1), "reference_length": 1 << 21,
             "io_bytes": (m + n + s["lag"]) * esz,
             "gb_per_s_io": round((m + n + s["lag"]) * esz / t / 1e9, 1),
             "rel_l2_vs_numpy_fp64": float(np.linalg.norm(y - want) / np.linalg.norm(want))}
