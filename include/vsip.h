@@ -61,6 +61,7 @@ typedef enum { VSIP_ALG_SPACE = 0, VSIP_ALG_TIME = 1, VSIP_ALG_NOISE = 2 } vsip_
 typedef enum { VSIP_PRNG = 0, VSIP_NPRNG = 1 } vsip_rng;                  /* vsip.h:159 */
 typedef enum { VSIP_SUPPORT_FULL = 0, VSIP_SUPPORT_SAME = 1, VSIP_SUPPORT_MIN = 2 } vsip_support_region; /* vsip.h:155 */
 typedef enum { VSIP_BIASED = 0, VSIP_UNBIASED = 1 } vsip_bias;          /* vsip.h:156 */
+typedef enum { VSIP_NONSYM = 0, VSIP_SYM_EVEN_LEN_ODD = 1, VSIP_SYM_EVEN_LEN_EVEN = 2 } vsip_symmetry; /* vsip.h:154 */
 
 /* ---- opaque handles (vsip.h:165-208) -------------------------------- */
 typedef struct vsip_blockattributes_d vsip_block_d;
@@ -84,6 +85,8 @@ typedef struct vsip_corr1dattributes_f vsip_corr1d_f;                    /* vsip
 typedef struct vsip_ccorr1dattributes_f vsip_ccorr1d_f;                  /* vsip.h:216 */
 typedef struct vsip_corr1dattributes_d vsip_corr1d_d;                    /* vsip.h:217 */
 typedef struct vsip_ccorr1dattributes_d vsip_ccorr1d_d;                  /* vsip.h:218 */
+typedef struct vsip_conv1dattributes_f vsip_conv1d_f;                    /* vsip.h:219 */
+typedef struct vsip_conv1dattributes_d vsip_conv1d_d;                    /* vsip.h:220 */
 
 /* ---- attribute structs (vsip.h:242-245, 277-278) -------------------- */
 typedef struct { vsip_offset offset; vsip_stride row_stride; vsip_length row_length;
@@ -110,6 +113,9 @@ typedef struct { vsip_scalar_vi input; vsip_scalar_vi output; vsip_fft_place pla
 /* vsip.h:282 */
 typedef struct { vsip_scalar_vi ref_len; vsip_scalar_vi data_len; vsip_support_region support;
                  vsip_scalar_vi lag_len; } vsip_corr1d_attr;
+/* vsip.h:283 */
+typedef struct { vsip_scalar_vi kernel_len; vsip_symmetry symm; vsip_scalar_vi data_len;
+                 vsip_support_region support; vsip_scalar_vi out_len; int decimation; } vsip_conv1d_attr;
 
 /* ---- library lifecycle ---------------------------------------------- */
 int vsip_init(void *ptr);                                         /* vsip.h:432 */
@@ -363,6 +369,25 @@ void vsip_ccorrelate1d_f(const vsip_ccorr1d_f*, vsip_bias, const vsip_cvview_f *
                          const vsip_cvview_f *y);                                       /* vsip.h:725 */
 void vsip_ccorrelate1d_d(const vsip_ccorr1d_d*, vsip_bias, const vsip_cvview_d *h, const vsip_cvview_d *x,
                          const vsip_cvview_d *y);                                       /* vsip.h:724 */
+
+/* ---- 1-D convolution (real, decimated) -------------------------------------
+ * y[k] = z[k D + s] of z = kernel * x, the kernel expanded by its symmetry
+ * (length M), s = 0 (FULL), M/2 (SAME), M - 1 (MIN).  The kernel view is read
+ * at create.  Every lag is computed (the reference's sectioned method loses
+ * the lags past its last full section).  create returns NULL for D < 1 or a
+ * MIN support with no output. */
+vsip_conv1d_f* vsip_conv1d_create_f(const vsip_vview_f *h, vsip_symmetry symm, vsip_length N, int D,
+                                    vsip_support_region support, unsigned int ntimes,
+                                    vsip_alg_hint hint);                                /* vsip.h:1796 */
+vsip_conv1d_d* vsip_conv1d_create_d(const vsip_vview_d *h, vsip_symmetry symm, vsip_length N, int D,
+                                    vsip_support_region support, unsigned int ntimes,
+                                    vsip_alg_hint hint);                                /* vsip.h:1795 */
+int vsip_conv1d_destroy_f(vsip_conv1d_f*);                                             /* vsip.h:406 */
+int vsip_conv1d_destroy_d(vsip_conv1d_d*);                                             /* vsip.h:405 */
+void vsip_conv1d_getattr_f(const vsip_conv1d_f*, vsip_conv1d_attr*);                   /* vsip.h:817 */
+void vsip_conv1d_getattr_d(const vsip_conv1d_d*, vsip_conv1d_attr*);                   /* vsip.h:816 */
+void vsip_convolve1d_f(const vsip_conv1d_f*, const vsip_vview_f *x, const vsip_vview_f *y); /* vsip.h:819 */
+void vsip_convolve1d_d(const vsip_conv1d_d*, const vsip_vview_d *x, const vsip_vview_d *y); /* vsip.h:818 */
 
 /* ---- element operations on transform operands ---------------------------- */
 void vsip_rscvmul_f(vsip_scalar_f, const vsip_cvview_f*, const vsip_cvview_f*);        /* vsip.h:1216 */

@@ -9,6 +9,10 @@
 //   product  after the forward transform of both rows:
 //            row0[k] = conj(row0[k] conj(row1[k])) * (1/N)
 //            (vsip_ccorrelate1d_f.c:157-159)
+//   conv     convolution (vsip_convolve1d_[fd]): row 0 = x then zeros; row 1
+//            holds the kernel's conjugate spectrum, so the product's
+//            transform has the convolution as its real part, read at lags
+//            s + k D
 //   window   after the forward transform of row 0: y[i] = w_i * row0[o + i],
 //            o = N - (m + n - 1) plus 0 (FULL), (m - 1)/2 (SAME) or m - 1
 //            (MIN), w_i the unbiasing weights of the support
@@ -110,6 +114,21 @@ __global__ void __launch_bounds__(256) corr_window_kernel(const Cx<T> *buf, long
     }
 }
 
+// convolution (vsip_convolve1d_[fd]): row 0 = x then N - n zeros (real x)
+template <typename T>
+__global__ void __launch_bounds__(256) conv_pack_kernel(const vhip_mat x, long n, Cx<T> *buf, long N) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long)gridDim.x * blockDim.x)
+        buf[t] = {t < n ? ((const T *)x.re)[t * x.rs] : (T)0, (T)0};
+}
+
+// y[k] = Re buf[s + k D], the decimated lags of the convolution
+template <typename T>
+__global__ void __launch_bounds__(256) conv_window_kernel(const Cx<T> *buf, long s, long D, const vhip_mat y,
+                                                          long len) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < len; k += (long)gridDim.x * blockDim.x)
+        ((T *)y.re)[k * y.rs] = buf[s + k * D].r;
+}
+
 long grid_for(long total) {
     int cus = 256;
     vhip_cu_count(&cus);
@@ -163,6 +182,35 @@ extern "C" int vhip_corr_window(int prec, const void *buf, long N, long o, const
     else if (prec == 8)
         hipLaunchKernelGGL(corr_window_kernel<double>, dim3((unsigned)g), dim3(256), 0, s, (const Cx<double> *)buf, o,
                            *y, y_cplx, len, support, unbiased, m, n);
+    else
+        return -1;
+    return (int)hipGetLastError();
+}
+
+extern "C" int vhip_conv_pack(int prec, const vhip_mat *x, long n, void *buf, long N, void *stream) {
+    if (n < 1 || n > N) return -1;
+    const long g = grid_for(N);
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == 4)
+        hipLaunchKernelGGL(conv_pack_kernel<float>, dim3((unsigned)g), dim3(256), 0, s, *x, n, (Cx<float> *)buf, N);
+    else if (prec == 8)
+        hipLaunchKernelGGL(conv_pack_kernel<double>, dim3((unsigned)g), dim3(256), 0, s, *x, n, (Cx<double> *)buf, N);
+    else
+        return -1;
+    return (int)hipGetLastError();
+}
+
+extern "C" int vhip_conv_window(int prec, const void *buf, long N, long s0, long D, const vhip_mat *y, long len,
+                                void *stream) {
+    if (len < 1 || D < 1 || s0 < 0 || s0 + (len - 1) * D >= N) return -1;
+    const long g = grid_for(len);
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == 4)
+        hipLaunchKernelGGL(conv_window_kernel<float>, dim3((unsigned)g), dim3(256), 0, s, (const Cx<float> *)buf, s0, D,
+                           *y, len);
+    else if (prec == 8)
+        hipLaunchKernelGGL(conv_window_kernel<double>, dim3((unsigned)g), dim3(256), 0, s, (const Cx<double> *)buf, s0,
+                           D, *y, len);
     else
         return -1;
     return (int)hipGetLastError();

@@ -1,6 +1,8 @@
 /*
  * va_corr.c -- 1-D correlation: vsip_corr1d (real) and vsip_ccorr1d
- * (complex) objects and vsip_[c]correlate1d_[fd], a caller of the FFT path.
+ * (complex) objects and vsip_[c]correlate1d_[fd]; 1-D convolution: vsip_conv1d
+ * objects and vsip_convolve1d_[fd] (at the end of the file).  Both are
+ * callers of the FFT path.
  *
  * The reference (c_VSIP_src/vsip_ccorrelate1d_f.c:123-196,
  * vsip_correlate1d_f.c:82-151, creation in vsip_ccorr1d_create_f.c:17-54)
@@ -247,4 +249,201 @@ void vsip_ccorrelate1d_d(const vsip_ccorr1d_d *c, vsip_bias bias, const vsip_cvv
 {
     va_correlate("vsip_ccorrelate1d_d", c ? &c->c : NULL, 8, 1, bias, h ? &h->v : NULL, x ? &x->v : NULL,
                  y ? &y->v : NULL);
+}
+
+/* ---- convolution -----------------------------------------------------------
+ * The reference (vsip_conv1d_create_f.c, vsip_convolve1d_f.c) expands the
+ * kernel by its symmetry (VSIP_NONSYM: h; VSIP_SYM_EVEN_LEN_ODD: h then h
+ * reversed without its last element; VSIP_SYM_EVEN_LEN_EVEN: h then h
+ * reversed), transforms it at creation and convolves by FFT, in sections for
+ * short kernels, keeping y[k] = z[k D + s] of z = kernel * x with s = 0
+ * (FULL, (M + N - 2)/D + 1 outputs), M/2 (SAME, (N - 1)/D + 1) or M - 1
+ * (MIN, (N - 1)/D - (M - 1)/D + 1).  Its sectioned form loses the lags of z
+ * at and past (number of sections) x (section data length), the tail past
+ * the last full section; this computes every lag.
+ * Here: one transform of length L >= N + M - 1 (the correlation's rule).  At
+ * creation the kernel, circularly reversed, goes to row 1 of the buffer and
+ * is transformed once: for a real kernel that is the conjugate of its
+ * spectrum.  A call packs x into row 0, transforms it, forms
+ * conj(X conj(row 1)) / L = conj(X K) / L (the correlation's product pass),
+ * transforms that forward -- the real part is z -- and reads the decimated
+ * lags into y.  create returns NULL for an empty kernel, D < 1, an unknown
+ * symmetry or support, or a MIN support with no output.
+ *   create   vsip.h:1795-1796      destroy  vsip.h:405-406 (returns 0)
+ *   getattr  vsip_conv1d_getattr_f.c:17-28     execute  vsip.h:818-819 */
+
+typedef struct {
+    int markings;
+    int prec;
+    vsip_length m, n, L, out_len;   /* kernel length (expanded), data length, transform, outputs */
+    int D;
+    vsip_symmetry symm;
+    vsip_support_region support;
+    VA_block *buf;                  /* 2 x L complex: row 0 the data, row 1 the kernel's spectrum */
+    VA_mview *row0, *row1;
+    VA_fftm fft1;
+    int have1;
+} VA_conv;
+
+struct vsip_conv1dattributes_f { VA_conv c; };
+struct vsip_conv1dattributes_d { VA_conv c; };
+
+static void va_conv_free(VA_conv *c)
+{
+    if (!c) return;
+    if (c->have1) va_fftm_release(&c->fft1);
+    if (c->row0) va_mview_destroy(c->row0);
+    if (c->row1) va_mview_destroy(c->row1);
+    if (c->buf) va_block_destroy(c->buf);
+    c->markings = VA_FREED;
+    free(c);
+}
+
+static VA_conv *va_conv_new(const char *fn, int prec, const VA_mview *h, vsip_symmetry symm, vsip_length n, int D,
+                            vsip_support_region support, unsigned int ntimes, vsip_alg_hint hint)
+{
+    VA_conv *c;
+    vsip_length hl, m, L, k;
+    double *kern = NULL, *spec = NULL;
+    int rc;
+    if (va_ensure_init()) return NULL;
+    if (!h || !h->block || h->col_length != 1 || h->row_length < 1 || n < 1 || D < 1 ||
+        n > ((vsip_length)1 << 40)) {
+        va_error(fn, "needs a kernel view, N >= 1 and D >= 1", -1);
+        return NULL;
+    }
+    if (symm != VSIP_NONSYM && symm != VSIP_SYM_EVEN_LEN_ODD && symm != VSIP_SYM_EVEN_LEN_EVEN) {
+        va_error(fn, "unknown symmetry", (int)symm);
+        return NULL;
+    }
+    if (support != VSIP_SUPPORT_FULL && support != VSIP_SUPPORT_SAME && support != VSIP_SUPPORT_MIN) {
+        va_error(fn, "unknown support region", (int)support);
+        return NULL;
+    }
+    hl = h->row_length;
+    m = symm == VSIP_SYM_EVEN_LEN_ODD ? 2 * hl - 1 : symm == VSIP_SYM_EVEN_LEN_EVEN ? 2 * hl : hl;
+    if (support == VSIP_SUPPORT_MIN && (m > n || (n - 1) / (vsip_length)D < (m - 1) / (vsip_length)D)) {
+        va_error(fn, "MIN support with no output (kernel longer than the data)", -1);
+        return NULL;
+    }
+    c = (VA_conv *)calloc(1, sizeof(VA_conv));
+    if (!c) return NULL;
+    c->prec = prec;
+    c->m = m;
+    c->n = n;
+    c->D = D;
+    c->symm = symm;
+    c->support = support;
+    c->out_len = support == VSIP_SUPPORT_FULL ? (m + n - 2) / (vsip_length)D + 1
+                 : support == VSIP_SUPPORT_SAME ? (n - 1) / (vsip_length)D + 1
+                 : (n - 1) / (vsip_length)D - (m - 1) / (vsip_length)D + 1;
+    L = c->L = va_corr_length(m + n - 1);
+    c->buf = va_block_create(prec, 1, 2 * L, VSIP_MEM_NONE);
+    if (!c->buf) goto fail;
+    c->row0 = va_mview_bind(c->buf, 0, (vsip_stride)L, 1, 1, L);
+    c->row1 = va_mview_bind(c->buf, L, (vsip_stride)L, 1, 1, L);
+    if (!c->row0 || !c->row1) goto fail;
+    c->have1 = 1;
+    if (va_fftm_init(&c->fft1, prec, VA_CCFFTIP, 1, L, 1.0, VSIP_FFT_FWD, VSIP_ROW, ntimes, hint)) goto fail;
+    /* the kernel on the host, expanded, then circularly reversed into row 1
+     * as interleaved complex of the working precision */
+    kern = (double *)malloc(sizeof(double) * (size_t)m);
+    spec = (double *)calloc((size_t)L * 2, sizeof(double));
+    if (!kern || !spec) goto fail;
+    {
+        void *hv = malloc((size_t)h->block->prec * (size_t)hl);
+        if (!hv) goto fail;
+        if (va_mview_copy_host(h, 0, hv, NULL, VSIP_ROW)) { free(hv); goto fail; }
+        for (k = 0; k < hl; k++)
+            kern[k] = h->block->prec == 4 ? (double)((float *)hv)[k] : ((double *)hv)[k];
+        free(hv);
+    }
+    for (k = hl; k < m; k++) kern[k] = kern[symm == VSIP_SYM_EVEN_LEN_ODD ? 2 * hl - 2 - k : 2 * hl - 1 - k];
+    for (k = 0; k < m; k++) spec[2 * ((L - k) % L)] = kern[k];
+    if (prec == 4) {
+        float *f = (float *)malloc(sizeof(float) * 2 * (size_t)L);
+        if (!f) goto fail;
+        for (k = 0; k < 2 * L; k++) f[k] = (float)spec[k];
+        rc = vhip_memcpy_h2d((char *)c->buf->dev_r + (size_t)L * 8, f, (size_t)L * 8);
+        free(f);
+    } else {
+        rc = vhip_memcpy_h2d((char *)c->buf->dev_r + (size_t)L * 16, spec, (size_t)L * 16);
+    }
+    if (rc) goto fail;
+    if (va_fftm_exec(&c->fft1, c->row1, c->row1, vhip_stream())) goto fail;
+    free(kern);
+    free(spec);
+    c->markings = VA_VALID;
+    return c;
+fail:
+    va_error(fn, "allocation, kernel copy or plan creation failed", -1);
+    free(kern);
+    free(spec);
+    va_conv_free(c);
+    return NULL;
+}
+
+static void va_convolve(const char *fn, const VA_conv *c, int prec, const VA_mview *x, const VA_mview *y)
+{
+    vhip_mat mx, my;
+    void *st, *buf;
+    long s0;
+    int rc;
+    if (va_ensure_init()) return;
+    if (!c || c->markings != VA_VALID || c->prec != prec) {
+        va_error(fn, "not a convolution object of this type", -1);
+        return;
+    }
+    if (va_corr_view(fn, x, c->n) || va_corr_view(fn, y, c->out_len)) return;
+    va_view_mat(x, 1, &mx);
+    va_view_mat(y, 1, &my);
+    st = vhip_stream();
+    buf = c->buf->dev_r;
+    s0 = c->support == VSIP_SUPPORT_FULL ? 0 : c->support == VSIP_SUPPORT_SAME ? (long)(c->m / 2) : (long)c->m - 1;
+    rc = vhip_conv_pack(prec, &mx, (long)c->n, buf, (long)c->L, st);
+    if (!rc) rc = va_fftm_exec(&c->fft1, c->row0, c->row0, st);
+    if (!rc) rc = vhip_corr_product(prec, buf, (long)c->L, st);
+    if (!rc) rc = va_fftm_exec(&c->fft1, c->row0, c->row0, st);
+    if (!rc) rc = vhip_conv_window(prec, buf, (long)c->L, s0, (long)c->D, &my, (long)c->out_len, st);
+    if (rc) va_error(fn, vhip_error_string(rc), rc);
+}
+
+static void va_conv_attr(const VA_conv *c, vsip_conv1d_attr *a)
+{
+    if (!c || !a) return;
+    a->kernel_len = c->m;
+    a->symm = c->symm;
+    a->data_len = c->n;
+    a->support = c->support;
+    a->out_len = c->out_len;
+    a->decimation = c->D;
+}
+
+vsip_conv1d_f *vsip_conv1d_create_f(const vsip_vview_f *h, vsip_symmetry symm, vsip_length N, int D,
+                                    vsip_support_region support, unsigned int ntimes, vsip_alg_hint hint)
+{
+    return (vsip_conv1d_f *)va_conv_new("vsip_conv1d_create_f", 4, h ? &h->v : NULL, symm, N, D, support, ntimes,
+                                        hint);
+}
+
+vsip_conv1d_d *vsip_conv1d_create_d(const vsip_vview_d *h, vsip_symmetry symm, vsip_length N, int D,
+                                    vsip_support_region support, unsigned int ntimes, vsip_alg_hint hint)
+{
+    return (vsip_conv1d_d *)va_conv_new("vsip_conv1d_create_d", 8, h ? &h->v : NULL, symm, N, D, support, ntimes,
+                                        hint);
+}
+
+int vsip_conv1d_destroy_f(vsip_conv1d_f *c) { va_conv_free(c ? &c->c : NULL); return 0; }
+int vsip_conv1d_destroy_d(vsip_conv1d_d *c) { va_conv_free(c ? &c->c : NULL); return 0; }
+void vsip_conv1d_getattr_f(const vsip_conv1d_f *c, vsip_conv1d_attr *a) { va_conv_attr(c ? &c->c : NULL, a); }
+void vsip_conv1d_getattr_d(const vsip_conv1d_d *c, vsip_conv1d_attr *a) { va_conv_attr(c ? &c->c : NULL, a); }
+
+void vsip_convolve1d_f(const vsip_conv1d_f *c, const vsip_vview_f *x, const vsip_vview_f *y)
+{
+    va_convolve("vsip_convolve1d_f", c ? &c->c : NULL, 4, x ? &x->v : NULL, y ? &y->v : NULL);
+}
+
+void vsip_convolve1d_d(const vsip_conv1d_d *c, const vsip_vview_d *x, const vsip_vview_d *y)
+{
+    va_convolve("vsip_convolve1d_d", c ? &c->c : NULL, 8, x ? &x->v : NULL, y ? &y->v : NULL);
 }

@@ -291,3 +291,24 @@ def correlate1d(lib, h_blk, h_view, x_blk, x_view, y_blk, y_view, support=0, bia
     for b in blocks:
         b.destroy()
     return res
+
+
+def convolve1d(lib, h_blk, h_view, symm, x_blk, x_view, y_blk, y_view, D=1, support=0, ntimes=0):
+    """Decimated 1-D convolution (vsip_convolve1d) of data view x with the
+    kernel view h (expanded by symm) into view y, on real user blocks
+    admitted before and released after; returns the y block."""
+    t = _t(h_blk.dtype.itemsize)
+    blocks = [_RBlock(lib, a) for a in (h_blk, x_blk, y_blk)]
+    views = [getattr(lib, "vbind_" + t)(b.h, *v) for b, v in zip(blocks, (h_view, x_view, y_view))]
+    obj = getattr(lib, "conv1d_create_" + t)(views[0], symm, x_view.length, D, support, ntimes, VSIP_ALG_TIME)
+    assert obj, "conv1d_create failed"
+    getattr(lib, "convolve1d_" + t)(obj, views[1], views[2])
+    res = blocks[2].release()
+    blocks[0].release()
+    blocks[1].release()
+    getattr(lib, "conv1d_destroy_" + t)(obj)
+    for v in views:
+        getattr(lib, "vdestroy_" + t)(v)
+    for b in blocks:
+        b.destroy()
+    return res

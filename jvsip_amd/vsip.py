@@ -28,6 +28,7 @@ VSIP_MEM_NONE = 0
 VSIP_ALG_TIME = 1
 VSIP_SUPPORT_FULL, VSIP_SUPPORT_SAME, VSIP_SUPPORT_MIN = 0, 1, 2
 VSIP_BIASED, VSIP_UNBIASED = 0, 1
+VSIP_NONSYM, VSIP_SYM_EVEN_LEN_ODD, VSIP_SYM_EVEN_LEN_EVEN = 0, 1, 2
 
 
 class cscalar_f(ctypes.Structure):
@@ -73,6 +74,11 @@ class fft_attr_d(ctypes.Structure):
 
 class corr1d_attr(ctypes.Structure):
     _fields_ = [("ref_len", c_vi), ("data_len", c_vi), ("support", c_bl), ("lag_len", c_vi)]
+
+
+class conv1d_attr(ctypes.Structure):
+    _fields_ = [("kernel_len", c_vi), ("symm", c_bl), ("data_len", c_vi), ("support", c_bl), ("out_len", c_vi),
+                ("decimation", ctypes.c_int)]
 
 
 class vattr(ctypes.Structure):
@@ -202,6 +208,11 @@ def _sig(t):
         "ccorr1d_getattr": (None, [P, ctypes.POINTER(corr1d_attr)]),
         "correlate1d": (None, [P, c_bl, P, P, P]),
         "ccorrelate1d": (None, [P, c_bl, P, P, P]),
+        # 1-D convolution
+        "conv1d_create": (P, [P, c_bl, c_vi, ctypes.c_int, c_bl, ctypes.c_uint, c_bl]),
+        "conv1d_destroy": (ctypes.c_int, [P]),
+        "conv1d_getattr": (None, [P, ctypes.POINTER(conv1d_attr)]),
+        "convolve1d": (None, [P, P, P]),
     }
 
 

@@ -184,6 +184,43 @@ def correlate1d(h, x, support=0, bias=0):
     return y.real.copy() if real else y.copy()
 
 
+def conv_kernel(h, symm):
+    """The kernel a convolution object holds (c_VSIP_src/vsip_conv1d_create_f.c:
+    symm 0 h; 1 (VSIP_SYM_EVEN_LEN_ODD) h then h reversed without its last
+    element, length 2 len(h) - 1; 2 (VSIP_SYM_EVEN_LEN_EVEN) h then h
+    reversed, length 2 len(h))."""
+    h = np.asarray(h, dtype=np.float64)
+    if symm == 1:
+        return np.concatenate([h, h[-2::-1]])
+    if symm == 2:
+        return np.concatenate([h, h[::-1]])
+    return h.copy()
+
+
+def conv_out_len(m, n, D, support):
+    """Output length (vsip_conv1d_create_f.c, the switch on support) for
+    kernel length m, data length n, decimation D."""
+    if support == 0:
+        return (m + n - 2) // D + 1
+    if support == 1:
+        return (n - 1) // D + 1
+    return (n - 1) // D - (m - 1) // D + 1
+
+
+def convolve1d(h, symm, x, D=1, support=0):
+    """Decimated 1-D convolution as vsip_convolve1d_f computes it (its FFT
+    methods, vsip_convolve1d_f.c, reduce to the linear convolution
+    z = kernel * x): y[k] = z[k D + s], s = 0 (FULL), kernel length / 2
+    (SAME), kernel length - 1 (MIN), in double precision."""
+    k = conv_kernel(h, symm)
+    x = np.asarray(x, dtype=np.float64)
+    z = np.convolve(x, k)
+    m, n = k.size, x.size
+    s = 0 if support == 0 else m // 2 if support == 1 else m - 1
+    L = conv_out_len(m, n, D, support)
+    return z[s + D * np.arange(L)]
+
+
 _REF = None
 
 
