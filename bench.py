@@ -230,6 +230,9 @@ def cpu_configs(ref):
         c = _corr_setup(ref, cplx, 1024, 1 << 20)
         out[("ccorrelate1d" if cplx else "correlate1d") + "_f_1024x2^20"] = {"s_per_call": timed(c["call"])}
         c["cleanup"]()
+    c = _conv_setup(ref, 1024, 1 << 20)
+    out["convolve1d_f_1024x2^20"] = {"s_per_call": timed(c["call"])}
+    c["cleanup"]()
     return out
 
 
@@ -763,10 +766,39 @@ def _corr_length(mn):
     return p2 if p2 <= 1.15 * smooth else smooth
 
 
+def _conv_setup(lib, m, n, D=1, support=0):
+    """A vsip_convolve1d_f object (kernel m, data n, no symmetry) on user
+    blocks admitted to the library; the call, the release of y, the cleanup."""
+    from jvsip_amd import datagen
+    L = ((m + n - 2) // D + 1) if support == 0 else ((n - 1) // D + 1) if support == 1 else \
+        ((n - 1) // D - (m - 1) // D + 1)
+    arrs = [datagen.real_data(23, (m,)), datagen.real_data(24, (n,)), np.zeros(L, np.float32)]
+    blocks = [lib.blockbind_f(a.ctypes.data, a.size, 0) for a in arrs]
+    for b in blocks:
+        lib.blockadmit_f(b, 1)
+    views = [lib.vbind_f(b, 0, 1, a.size) for b, a in zip(blocks, arrs)]
+    obj = lib.conv1d_create_f(views[0], 0, n, D, support, 0, 1)
+    assert obj, "conv1d_create failed"
+
+    def result():
+        lib.blockrelease_f(blocks[2], 1)
+        return arrs[2].copy()
+
+    def cleanup():
+        lib.conv1d_destroy_f(obj)
+        for v in views:
+            lib.vdestroy_f(v)
+        for b in blocks:
+            lib.blockdestroy_f(b)
+
+    return {"call": lambda: lib.convolve1d_f(obj, views[1], views[2]), "result": result, "cleanup": cleanup,
+            "h": arrs[0], "x": arrs[1], "len": L}
+
+
 def config_corr1d(lib, hip):
-    """vsip_ccorrelate1d_f and vsip_correlate1d_f, FULL support, unbiased:
-    reference 1024, data 2^20 (transforms of N = 2^21, the three-pass
-    split), on admitted unit-stride views; accuracy against numpy FP64"""
+    """vsip_ccorrelate1d_f and vsip_correlate1d_f, FULL support, unbiased,
+    and vsip_convolve1d_f, FULL, no decimation: reference / kernel 1024, data
+    2^20, on admitted unit-stride views; accuracy against numpy FP64"""
     out = {}
     m, n = 1024, 1 << 20
     for cplx in (True, False):
@@ -781,6 +813,18 @@ def config_corr1d(lib, hip):
             "gb_per_s_io": round((m + n + s["lag"]) * esz / t / 1e9, 1),
             "rel_l2_vs_numpy_fp64": float(np.linalg.norm(y - want) / np.linalg.norm(want))}
         s["cleanup"]()
+    # convolution, kernel 1024, data 2^20, FULL, no decimation
+    s = _conv_setup(lib, m, n)
+    _, t = _time_calls(lib, hip, s["call"], 20)
+    y = s["result"]()
+    L = 1 << int(np.ceil(np.log2(m + n - 1)))
+    want = np.fft.irfft(np.fft.rfft(s["x"].astype(np.float64), L) * np.fft.rfft(s["h"].astype(np.float64), L),
+                        L)[:m + n - 1]
+    out["convolve1d_f_1024x2^20"] = {
+        "us": round(t * 1e6, 2), "transform_length": _corr_length(m + n - 1),
+        "io_bytes": (m + n + s["len"]) * 4, "gb_per_s_io": round((m + n + s["len"]) * 4 / t / 1e9, 1),
+        "rel_l2_vs_numpy_fp64": float(np.linalg.norm(y - want) / np.linalg.norm(want))}
+    s["cleanup"]()
     return out
 
 
