@@ -9,10 +9,10 @@
 //   product  after the forward transform of both rows:
 //            row0[k] = conj(row0[k] conj(row1[k])) * (1/N)
 //            (vsip_ccorrelate1d_f.c:157-159)
-//   conv     convolution (vsip_convolve1d_[fd]): row 0 = x then zeros; row 1
-//            holds the kernel's conjugate spectrum, so the product's
-//            transform has the convolution as its real part, read at lags
-//            s + k D
+//   conv     convolution (vsip_convolve1d_[fd]) on a real row of even
+//            length L around real-to-complex and complex-to-real
+//            transforms: x then zeros; the half-spectrum product with the
+//            kernel's; the convolution read at lags s + k D
 //   window   after the forward transform of row 0: y[i] = w_i * row0[o + i],
 //            o = N - (m + n - 1) plus 0 (FULL), (m - 1)/2 (SAME) or m - 1
 //            (MIN), w_i the unbiasing weights of the support
@@ -114,19 +114,29 @@ __global__ void __launch_bounds__(256) corr_window_kernel(const Cx<T> *buf, long
     }
 }
 
-// convolution (vsip_convolve1d_[fd]): row 0 = x then N - n zeros (real x)
+// real-input convolution: r = x then L - n zeros (real row), for the
+// real-to-complex transform
 template <typename T>
-__global__ void __launch_bounds__(256) conv_pack_kernel(const vhip_mat x, long n, Cx<T> *buf, long N) {
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long)gridDim.x * blockDim.x)
-        buf[t] = {t < n ? ((const T *)x.re)[t * x.rs] : (T)0, (T)0};
+__global__ void __launch_bounds__(256) conv_pack_r_kernel(const vhip_mat x, long n, T *r, long L) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < L; t += (long)gridDim.x * blockDim.x)
+        r[t] = t < n ? ((const T *)x.re)[t * x.rs] : (T)0;
 }
 
-// y[k] = Re buf[s + k D], the decimated lags of the convolution
+// half spectra: s[k] = s[k] k[k] / L, k <= L/2 (no conjugate)
 template <typename T>
-__global__ void __launch_bounds__(256) conv_window_kernel(const Cx<T> *buf, long s, long D, const vhip_mat y,
-                                                          long len) {
+__global__ void __launch_bounds__(256) conv_mul_half_kernel(Cx<T> *s, const Cx<T> *kspec, long nb, T inv_l) {
+#pragma clang fp contract(off)
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += (long)gridDim.x * blockDim.x) {
+        const Cx<T> a = s[k], b = kspec[k];
+        s[k] = {(a.r * b.r - a.i * b.i) * inv_l, (a.r * b.i + a.i * b.r) * inv_l};
+    }
+}
+
+// y[k] = r[s + k D] from the real row
+template <typename T>
+__global__ void __launch_bounds__(256) conv_window_r_kernel(const T *r, long s, long D, const vhip_mat y, long len) {
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < len; k += (long)gridDim.x * blockDim.x)
-        ((T *)y.re)[k * y.rs] = buf[s + k * D].r;
+        ((T *)y.re)[k * y.rs] = r[s + k * D];
 }
 
 long grid_for(long total) {
@@ -187,29 +197,44 @@ extern "C" int vhip_corr_window(int prec, const void *buf, long N, long o, const
     return (int)hipGetLastError();
 }
 
-extern "C" int vhip_conv_pack(int prec, const vhip_mat *x, long n, void *buf, long N, void *stream) {
-    if (n < 1 || n > N) return -1;
-    const long g = grid_for(N);
+extern "C" int vhip_conv_pack_r(int prec, const vhip_mat *x, long n, void *r, long L, void *stream) {
+    if (n < 1 || n > L) return -1;
+    const long g = grid_for(L);
     hipStream_t s = (hipStream_t)stream;
     if (prec == 4)
-        hipLaunchKernelGGL(conv_pack_kernel<float>, dim3((unsigned)g), dim3(256), 0, s, *x, n, (Cx<float> *)buf, N);
+        hipLaunchKernelGGL(conv_pack_r_kernel<float>, dim3((unsigned)g), dim3(256), 0, s, *x, n, (float *)r, L);
     else if (prec == 8)
-        hipLaunchKernelGGL(conv_pack_kernel<double>, dim3((unsigned)g), dim3(256), 0, s, *x, n, (Cx<double> *)buf, N);
+        hipLaunchKernelGGL(conv_pack_r_kernel<double>, dim3((unsigned)g), dim3(256), 0, s, *x, n, (double *)r, L);
     else
         return -1;
     return (int)hipGetLastError();
 }
 
-extern "C" int vhip_conv_window(int prec, const void *buf, long N, long s0, long D, const vhip_mat *y, long len,
-                                void *stream) {
-    if (len < 1 || D < 1 || s0 < 0 || s0 + (len - 1) * D >= N) return -1;
+extern "C" int vhip_conv_mul_half(int prec, void *spec, const void *kspec, long L, void *stream) {
+    if (L < 2 || (L & 1)) return -1;
+    const long nb = L / 2 + 1, g = grid_for(nb);
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == 4)
+        hipLaunchKernelGGL(conv_mul_half_kernel<float>, dim3((unsigned)g), dim3(256), 0, s, (Cx<float> *)spec,
+                           (const Cx<float> *)kspec, nb, (float)1 / (float)L);
+    else if (prec == 8)
+        hipLaunchKernelGGL(conv_mul_half_kernel<double>, dim3((unsigned)g), dim3(256), 0, s, (Cx<double> *)spec,
+                           (const Cx<double> *)kspec, nb, (double)1 / (double)L);
+    else
+        return -1;
+    return (int)hipGetLastError();
+}
+
+extern "C" int vhip_conv_window_r(int prec, const void *r, long L, long s0, long D, const vhip_mat *y, long len,
+                                  void *stream) {
+    if (len < 1 || D < 1 || s0 < 0 || s0 + (len - 1) * D >= L) return -1;
     const long g = grid_for(len);
     hipStream_t s = (hipStream_t)stream;
     if (prec == 4)
-        hipLaunchKernelGGL(conv_window_kernel<float>, dim3((unsigned)g), dim3(256), 0, s, (const Cx<float> *)buf, s0, D,
+        hipLaunchKernelGGL(conv_window_r_kernel<float>, dim3((unsigned)g), dim3(256), 0, s, (const float *)r, s0, D,
                            *y, len);
     else if (prec == 8)
-        hipLaunchKernelGGL(conv_window_kernel<double>, dim3((unsigned)g), dim3(256), 0, s, (const Cx<double> *)buf, s0,
+        hipLaunchKernelGGL(conv_window_r_kernel<double>, dim3((unsigned)g), dim3(256), 0, s, (const double *)r, s0,
                            D, *y, len);
     else
         return -1;

@@ -229,10 +229,13 @@ int vhip_elem_real(int prec, int op, const vhip_mat *a, const vhip_mat *b, const
 int vhip_corr_pack(int prec, const vhip_mat *x, int x_cplx, long n, const vhip_mat *h, int h_cplx, long m,
                    void *buf, long N, void *stream);
 int vhip_corr_product(int prec, void *buf, long N, void *stream);
-/* convolution: pack row 0 = x (n reals) then zeros; window y[k] =
- * Re row0[s + k D] for k < len */
-int vhip_conv_pack(int prec, const vhip_mat *x, long n, void *buf, long N, void *stream);
-int vhip_conv_window(int prec, const void *buf, long N, long s, long D, const vhip_mat *y, long len, void *stream);
+/* convolution on a real row of even length L around real-to-complex and
+ * complex-to-real transforms: pack_r r = x (n reals) then zeros; mul_half
+ * spec[k] = spec[k] kspec[k] / L for k <= L/2; window_r y[k] = r[s + k D]
+ * for k < len */
+int vhip_conv_pack_r(int prec, const vhip_mat *x, long n, void *r, long L, void *stream);
+int vhip_conv_mul_half(int prec, void *spec, const void *kspec, long L, void *stream);
+int vhip_conv_window_r(int prec, const void *r, long L, long s, long D, const vhip_mat *y, long len, void *stream);
 int vhip_corr_window(int prec, const void *buf, long N, long o, const vhip_mat *y, int y_cplx, long len,
                      int support, int unbiased, long m, long n, void *stream);
 

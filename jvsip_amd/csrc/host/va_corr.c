@@ -261,13 +261,13 @@ void vsip_ccorrelate1d_d(const vsip_ccorr1d_d *c, vsip_bias bias, const vsip_cvv
  * (MIN, (N - 1)/D - (M - 1)/D + 1).  Its sectioned form loses the lags of z
  * at and past (number of sections) x (section data length), the tail past
  * the last full section; this computes every lag.
- * Here: one transform of length L >= N + M - 1 (the correlation's rule).  At
- * creation the kernel, circularly reversed, goes to row 1 of the buffer and
- * is transformed once: for a real kernel that is the conjugate of its
- * spectrum.  A call packs x into row 0, transforms it, forms
- * conj(X conj(row 1)) / L = conj(X K) / L (the correlation's product pass),
- * transforms that forward -- the real part is z -- and reads the decimated
- * lags into y.  create returns NULL for an empty kernel, D < 1, an unknown
+ * Here, with real transforms of an even length L >= N + M - 1 (twice the
+ * correlation's length rule for half of it): at creation the kernel,
+ * zero-padded, goes through a real-to-complex transform to its half
+ * spectrum K (L/2 + 1 bins).  A call packs x into a real row, transforms it
+ * to X, forms X K / L on the half spectrum, and its complex-to-real
+ * transform (the inverse sign, scale 1) is z, whose decimated lags go to y:
+ * about half the transform work of complex rows.  create returns NULL for an empty kernel, D < 1, an unknown
  * symmetry or support, or a MIN support with no output.
  *   create   vsip.h:1795-1796      destroy  vsip.h:405-406 (returns 0)
  *   getattr  vsip_conv1d_getattr_f.c:17-28     execute  vsip.h:818-819 */
@@ -275,14 +275,15 @@ void vsip_ccorrelate1d_d(const vsip_ccorr1d_d *c, vsip_bias bias, const vsip_cvv
 typedef struct {
     int markings;
     int prec;
-    vsip_length m, n, L, out_len;   /* kernel length (expanded), data length, transform, outputs */
+    vsip_length m, n, L, out_len;   /* kernel length (expanded), data length, transform (even), outputs */
     int D;
     vsip_symmetry symm;
     vsip_support_region support;
-    VA_block *buf;                  /* 2 x L complex: row 0 the data, row 1 the kernel's spectrum */
-    VA_mview *row0, *row1;
-    VA_fftm fft1;
-    int have1;
+    VA_block *rbuf;                 /* L reals: the padded data, then the convolution */
+    VA_block *sbuf;                 /* 2 x (L/2 + 1) complex: the data's half spectrum, the kernel's */
+    VA_mview *rrow, *s0, *s1;
+    VA_fftm rc, cr;                 /* real to complex, complex to real, scale 1 */
+    int have_rc, have_cr;
 } VA_conv;
 
 struct vsip_conv1dattributes_f { VA_conv c; };
@@ -291,10 +292,13 @@ struct vsip_conv1dattributes_d { VA_conv c; };
 static void va_conv_free(VA_conv *c)
 {
     if (!c) return;
-    if (c->have1) va_fftm_release(&c->fft1);
-    if (c->row0) va_mview_destroy(c->row0);
-    if (c->row1) va_mview_destroy(c->row1);
-    if (c->buf) va_block_destroy(c->buf);
+    if (c->have_rc) va_fftm_release(&c->rc);
+    if (c->have_cr) va_fftm_release(&c->cr);
+    if (c->rrow) va_mview_destroy(c->rrow);
+    if (c->s0) va_mview_destroy(c->s0);
+    if (c->s1) va_mview_destroy(c->s1);
+    if (c->rbuf) va_block_destroy(c->rbuf);
+    if (c->sbuf) va_block_destroy(c->sbuf);
     c->markings = VA_FREED;
     free(c);
 }
@@ -303,8 +307,9 @@ static VA_conv *va_conv_new(const char *fn, int prec, const VA_mview *h, vsip_sy
                             vsip_support_region support, unsigned int ntimes, vsip_alg_hint hint)
 {
     VA_conv *c;
-    vsip_length hl, m, L, k;
-    double *kern = NULL, *spec = NULL;
+    vsip_length hl, m, L, nb, k;
+    double *kern = NULL;
+    void *up = NULL;
     int rc;
     if (va_ensure_init()) return NULL;
     if (!h || !h->block || h->col_length != 1 || h->row_length < 1 || n < 1 || D < 1 ||
@@ -337,19 +342,26 @@ static VA_conv *va_conv_new(const char *fn, int prec, const VA_mview *h, vsip_sy
     c->out_len = support == VSIP_SUPPORT_FULL ? (m + n - 2) / (vsip_length)D + 1
                  : support == VSIP_SUPPORT_SAME ? (n - 1) / (vsip_length)D + 1
                  : (n - 1) / (vsip_length)D - (m - 1) / (vsip_length)D + 1;
-    L = c->L = va_corr_length(m + n - 1);
-    c->buf = va_block_create(prec, 1, 2 * L, VSIP_MEM_NONE);
-    if (!c->buf) goto fail;
-    c->row0 = va_mview_bind(c->buf, 0, (vsip_stride)L, 1, 1, L);
-    c->row1 = va_mview_bind(c->buf, L, (vsip_stride)L, 1, 1, L);
-    if (!c->row0 || !c->row1) goto fail;
-    c->have1 = 1;
-    if (va_fftm_init(&c->fft1, prec, VA_CCFFTIP, 1, L, 1.0, VSIP_FFT_FWD, VSIP_ROW, ntimes, hint)) goto fail;
-    /* the kernel on the host, expanded, then circularly reversed into row 1
-     * as interleaved complex of the working precision */
-    kern = (double *)malloc(sizeof(double) * (size_t)m);
-    spec = (double *)calloc((size_t)L * 2, sizeof(double));
-    if (!kern || !spec) goto fail;
+    /* an even length >= m + n - 1 for the real transforms */
+    L = c->L = 2 * va_corr_length((m + n) / 2);
+    nb = L / 2 + 1;
+    c->rbuf = va_block_create(prec, 0, L, VSIP_MEM_NONE);
+    c->sbuf = va_block_create(prec, 1, 2 * nb, VSIP_MEM_NONE);
+    if (!c->rbuf || !c->sbuf) goto fail;
+    c->rrow = va_mview_bind(c->rbuf, 0, (vsip_stride)L, 1, 1, L);
+    c->s0 = va_mview_bind(c->sbuf, 0, (vsip_stride)nb, 1, 1, nb);
+    c->s1 = va_mview_bind(c->sbuf, nb, (vsip_stride)nb, 1, 1, nb);
+    if (!c->rrow || !c->s0 || !c->s1) goto fail;
+    /* a plan whose creation failed is still released */
+    c->have_rc = 1;
+    if (va_fftm_init(&c->rc, prec, VA_RCFFTOP, 1, L, 1.0, VSIP_FFT_FWD, VSIP_ROW, ntimes, hint)) goto fail;
+    c->have_cr = 1;
+    if (va_fftm_init(&c->cr, prec, VA_CRFFTOP, 1, L, 1.0, VSIP_FFT_INV, VSIP_ROW, ntimes, hint)) goto fail;
+    /* the kernel, expanded on the host and zero-padded to L, then its half
+     * spectrum into row 1 of the spectrum buffer */
+    kern = (double *)calloc((size_t)L, sizeof(double));
+    up = malloc((size_t)prec * (size_t)L);
+    if (!kern || !up) goto fail;
     {
         void *hv = malloc((size_t)h->block->prec * (size_t)hl);
         if (!hv) goto fail;
@@ -359,26 +371,20 @@ static VA_conv *va_conv_new(const char *fn, int prec, const VA_mview *h, vsip_sy
         free(hv);
     }
     for (k = hl; k < m; k++) kern[k] = kern[symm == VSIP_SYM_EVEN_LEN_ODD ? 2 * hl - 2 - k : 2 * hl - 1 - k];
-    for (k = 0; k < m; k++) spec[2 * ((L - k) % L)] = kern[k];
-    if (prec == 4) {
-        float *f = (float *)malloc(sizeof(float) * 2 * (size_t)L);
-        if (!f) goto fail;
-        for (k = 0; k < 2 * L; k++) f[k] = (float)spec[k];
-        rc = vhip_memcpy_h2d((char *)c->buf->dev_r + (size_t)L * 8, f, (size_t)L * 8);
-        free(f);
-    } else {
-        rc = vhip_memcpy_h2d((char *)c->buf->dev_r + (size_t)L * 16, spec, (size_t)L * 16);
+    for (k = 0; k < L; k++) {
+        if (prec == 4) ((float *)up)[k] = (float)kern[k];
+        else ((double *)up)[k] = kern[k];
     }
-    if (rc) goto fail;
-    if (va_fftm_exec(&c->fft1, c->row1, c->row1, vhip_stream())) goto fail;
+    rc = vhip_memcpy_h2d(c->rbuf->dev_r, up, (size_t)prec * (size_t)L);
+    if (rc || va_fftm_exec(&c->rc, c->rrow, c->s1, vhip_stream())) goto fail;
     free(kern);
-    free(spec);
+    free(up);
     c->markings = VA_VALID;
     return c;
 fail:
     va_error(fn, "allocation, kernel copy or plan creation failed", -1);
     free(kern);
-    free(spec);
+    free(up);
     va_conv_free(c);
     return NULL;
 }
@@ -386,8 +392,8 @@ fail:
 static void va_convolve(const char *fn, const VA_conv *c, int prec, const VA_mview *x, const VA_mview *y)
 {
     vhip_mat mx, my;
-    void *st, *buf;
-    long s0;
+    void *st, *r, *spec;
+    long s0, nb;
     int rc;
     if (va_ensure_init()) return;
     if (!c || c->markings != VA_VALID || c->prec != prec) {
@@ -398,13 +404,15 @@ static void va_convolve(const char *fn, const VA_conv *c, int prec, const VA_mvi
     va_view_mat(x, 1, &mx);
     va_view_mat(y, 1, &my);
     st = vhip_stream();
-    buf = c->buf->dev_r;
+    r = c->rbuf->dev_r;
+    spec = c->sbuf->dev_r;
+    nb = (long)c->L / 2 + 1;
     s0 = c->support == VSIP_SUPPORT_FULL ? 0 : c->support == VSIP_SUPPORT_SAME ? (long)(c->m / 2) : (long)c->m - 1;
-    rc = vhip_conv_pack(prec, &mx, (long)c->n, buf, (long)c->L, st);
-    if (!rc) rc = va_fftm_exec(&c->fft1, c->row0, c->row0, st);
-    if (!rc) rc = vhip_corr_product(prec, buf, (long)c->L, st);
-    if (!rc) rc = va_fftm_exec(&c->fft1, c->row0, c->row0, st);
-    if (!rc) rc = vhip_conv_window(prec, buf, (long)c->L, s0, (long)c->D, &my, (long)c->out_len, st);
+    rc = vhip_conv_pack_r(prec, &mx, (long)c->n, r, (long)c->L, st);
+    if (!rc) rc = va_fftm_exec(&c->rc, c->rrow, c->s0, st);
+    if (!rc) rc = vhip_conv_mul_half(prec, spec, (char *)spec + (size_t)nb * 2 * (size_t)prec, (long)c->L, st);
+    if (!rc) rc = va_fftm_exec(&c->cr, c->s0, c->rrow, st);
+    if (!rc) rc = vhip_conv_window_r(prec, r, (long)c->L, s0, (long)c->D, &my, (long)c->out_len, st);
     if (rc) va_error(fn, vhip_error_string(rc), rc);
 }
 
