@@ -120,3 +120,27 @@ def test_attributes_and_refusals(amd):
     assert not amd.conv1d_create_f(hv, 0, 100, 0, 0, 0, 1)        # D = 0
     amd.amd_clear_error()
     amd.valldestroy_f(hv)
+
+
+@pytest.mark.parametrize("seed", range(48))
+def test_convolve_sweep(amd, seed):
+    """Seeded random convolution cases: kernel 1..1500 (expanded by a random
+    symmetry), data 1..5000, decimation 1..5, support, FP32 or FP64, strides
+    -3..3 with offsets; against the oracle."""
+    rng = np.random.default_rng(6000 + seed)
+    prec = 8 if rng.random() < 0.3 else 4
+    dt = np.float32 if prec == 4 else np.float64
+    n = int(np.exp(rng.uniform(0, np.log(5000))))
+    hl = int(np.exp(rng.uniform(0, np.log(1500))))
+    symm, D, support = int(rng.integers(0, 3)), int(rng.integers(1, 6)), int(rng.integers(0, 3))
+    m = po.conv_kernel(np.zeros(hl), symm).size
+    if support == 2 and (m > n or po.conv_out_len(m, n, D, support) < 1):
+        support = 0
+    strides = tuple(int(rng.choice([-3, -1, 1, 1, 2, 3])) for _ in range(3))
+    offsets = tuple(int(rng.integers(0, 5)) for _ in range(3))
+    _case("conv sweep %d: prec %d h %d symm %d n %d D %d support %d strides %s offsets %s"
+          % (seed, prec, hl, symm, n, D, support, strides, offsets))
+    h, x = rng.standard_normal(hl).astype(dt), rng.standard_normal(n).astype(dt)
+    y, _, _, _ = _run(amd, h, symm, x, D, support, strides, offsets)
+    err = po.rel_l2(y, po.convolve1d(h, symm, x, D, support))
+    assert err < (1e-5 if prec == 4 else 1e-12), err

@@ -172,3 +172,27 @@ def test_object_reuse_and_attributes(amd):
     amd.amd_clear_error()
     for v in (hv, xv, yv):
         amd.cvalldestroy_f(v)
+
+
+NSWEEP = 48
+
+
+@pytest.mark.parametrize("seed", range(NSWEEP))
+def test_correlate_sweep(amd, seed):
+    """Seeded random correlation cases: lengths 1..3000 (m <= n), support,
+    bias, real or complex, FP32 or FP64, strides -3..3 with offsets, split
+    blocks for complex; against the oracle."""
+    rng = np.random.default_rng(5000 + seed)
+    cplx, prec = bool(rng.random() < 0.5), 8 if rng.random() < 0.3 else 4
+    n = int(np.exp(rng.uniform(0, np.log(3000))))
+    m = int(rng.integers(1, n + 1))
+    support, bias = int(rng.integers(0, 3)), int(rng.integers(0, 2))
+    strides = tuple(int(rng.choice([-3, -1, 1, 1, 2, 3])) for _ in range(3))
+    offsets = tuple(int(rng.integers(0, 5)) for _ in range(3))
+    split = cplx and bool(rng.random() < 0.3)
+    _case("corr sweep %d: %s prec %d m %d n %d support %d bias %d strides %s offsets %s split %s"
+          % (seed, "complex" if cplx else "real", prec, m, n, support, bias, strides, offsets, split))
+    h, x = _data(rng, m, n, cplx, prec)
+    y, _, _, _ = _run(amd, h, x, support, bias, strides, offsets, split=split)
+    err = po.rel_l2(y, po.correlate1d(h, x, support, bias))
+    assert err < (1e-5 if prec == 4 else 1e-12), err
