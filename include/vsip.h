@@ -375,7 +375,8 @@ void vsip_ccorrelate1d_d(const vsip_ccorr1d_d*, vsip_bias, const vsip_cvview_d *
  * (length M), s = 0 (FULL), M/2 (SAME), M - 1 (MIN).  The kernel view is read
  * at create.  Every lag is computed (the reference's sectioned method loses
  * the lags past its last full section).  create returns NULL for D < 1 or a
- * MIN support with no output. */
+ * MIN support with no output.  Calls on one object share its buffers, so one
+ * object serves one thread at a time. */
 vsip_conv1d_f* vsip_conv1d_create_f(const vsip_vview_f *h, vsip_symmetry symm, vsip_length N, int D,
                                     vsip_support_region support, unsigned int ntimes,
                                     vsip_alg_hint hint);                                /* vsip.h:1796 */
